@@ -1,0 +1,235 @@
+"""ecsg_amd -- MI355X-native ECSG time-stepping path.
+
+The compute path is libecsg_hip.so (hand-written gfx950 HIP kernels behind
+the C ABI of include/ecsg_hip.h).  This module binds that ABI with ctypes and
+mirrors the reference's driver vocabulary: a Simulation owns the device
+state of all composite grids (main.cpp:66-155) and advances it with the
+phase order of main.cpp:165-261.  There is no CPU fallback: if the library
+is missing, importing works but creating a Simulation raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import model  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libecsg_hip.so")
+
+F32, F64 = 0, 1
+MAXG = 8
+
+# symbols declared in include/ecsg_hip.h
+ABI_SYMBOLS = (
+    "ecsg_create", "ecsg_destroy", "ecsg_last_error", "ecsg_version", "ecsg_set_params", "ecsg_set_source",
+    "ecsg_set_receivers", "ecsg_update_vel", "ecsg_record_receivers", "ecsg_add_source", "ecsg_update_ghost_vz",
+    "ecsg_compute_energy", "ecsg_update_stress", "ecsg_update_ghost_sxz", "ecsg_update_ghost_syz", "ecsg_end_step",
+    "ecsg_compute_res", "ecsg_step_n", "ecsg_steps_done", "ecsg_synchronize", "ecsg_field_dims", "ecsg_get_field",
+    "ecsg_set_field", "ecsg_get_traces", "ecsg_get_energy", "ecsg_jacobi_stats", "ecsg_profile_kernels",
+)
+
+
+class EcsgConfig(C.Structure):
+    _fields_ = [
+        ("num_grids", C.c_int),
+        ("n", (C.c_int * 3) * MAXG),
+        ("h", (C.c_double * 3) * MAXG),
+        ("dt", C.c_double),
+        ("radius", C.c_int),
+        ("top_bc", C.c_int),
+        ("imode", C.c_int),
+        ("check_energy", C.c_int),
+        ("precision", C.c_int),
+        ("max_steps", C.c_int),
+        ("vy_mirror_all", C.c_int),
+    ]
+
+
+class EcsgError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load_library(path: Optional[str] = None):
+    """Load libecsg_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise EcsgError("libecsg_hip.so not built (%s); run python -m ecsg_amd.build" % p)
+    L = C.CDLL(p)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    P = C.POINTER
+    L.ecsg_create.argtypes = [P(EcsgConfig), P(vp)]
+    L.ecsg_destroy.argtypes = [vp]
+    L.ecsg_last_error.argtypes = [vp]
+    L.ecsg_last_error.restype = C.c_char_p
+    L.ecsg_version.restype = C.c_char_p
+    L.ecsg_set_params.argtypes = [vp, i, P(d), P(P(d)), i]
+    L.ecsg_set_source.argtypes = [vp, i, i, i, i, i, P(C.c_float), i]
+    L.ecsg_set_receivers.argtypes = [vp, i, P(i)]
+    for f in ("ecsg_update_vel", "ecsg_update_ghost_vz", "ecsg_update_stress", "ecsg_update_ghost_sxz",
+              "ecsg_update_ghost_syz"):
+        getattr(L, f).argtypes = [vp, i]
+    for f in ("ecsg_record_receivers", "ecsg_add_source", "ecsg_end_step", "ecsg_synchronize", "ecsg_steps_done"):
+        getattr(L, f).argtypes = [vp]
+    L.ecsg_compute_energy.argtypes = [vp, P(d)]
+    L.ecsg_compute_res.argtypes = [vp, i, P(d)]
+    L.ecsg_step_n.argtypes = [vp, i, i]
+    L.ecsg_field_dims.argtypes = [vp, i, i, P(i)]
+    L.ecsg_get_field.argtypes = [vp, i, i, P(d)]
+    L.ecsg_set_field.argtypes = [vp, i, i, P(d)]
+    L.ecsg_get_traces.argtypes = [vp, P(d)]
+    L.ecsg_get_energy.argtypes = [vp, P(d)]
+    L.ecsg_jacobi_stats.argtypes = [vp, i, P(C.c_longlong), P(i), P(i)]
+    L.ecsg_profile_kernels.argtypes = [vp, i, P(d), P(i)]
+    if path is None:
+        _LIB = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Simulation:
+    """Device state of one composite model (all grids), advanced on one stream.
+
+    precision: "f32" (wavefields and parameters in float; interface solves,
+    energy and residuals in double) or "f64" (everything double).
+    """
+
+    def __init__(self, m, precision: str = "f64", check_energy: bool = True, max_steps: Optional[int] = None,
+                 vy_mirror_all: bool = False):
+        self.L = load_library()
+        self.model = m
+        cfg = EcsgConfig()
+        cfg.num_grids = m.num_grids
+        for g, G in enumerate(m.grids):
+            for k in range(3):
+                cfg.n[g][k] = int(G.n[k])
+                cfg.h[g][k] = float(G.h[k])
+        cfg.dt = float(m.dt)
+        cfg.radius = m.radius
+        cfg.top_bc = m.top_bc
+        cfg.imode = m.imode
+        cfg.check_energy = int(check_energy)
+        cfg.precision = {"f32": F32, "f64": F64}[precision]
+        cfg.max_steps = int(m.nsteps if max_steps is None else max_steps)
+        cfg.vy_mirror_all = int(vy_mirror_all)
+        self.max_steps = cfg.max_steps
+        self.precision = precision
+        self.h = C.c_void_p()
+        rc = self.L.ecsg_create(C.byref(cfg), C.byref(self.h))
+        if rc != 0:
+            msg = self.L.ecsg_last_error(self.h).decode() if self.h else ""
+            self.L.ecsg_destroy(self.h)
+            self.h = None
+            raise EcsgError("ecsg_create failed (%d): %s" % (rc, msg))
+        for g in range(m.num_grids):
+            self._set_params(g)
+        if m.src_trace is not None and len(m.src_trace):
+            tr = np.ascontiguousarray(m.src_trace, np.float32)
+            self._chk(self.L.ecsg_set_source(self.h, m.src_grid, *map(int, m.src_loc), m.src_type,
+                                             tr.ctypes.data_as(C.POINTER(C.c_float)), len(tr)))
+        rec = np.ascontiguousarray(m.rec_loc, np.int32).reshape(-1, 4)
+        self.nrec = len(rec)
+        self._chk(self.L.ecsg_set_receivers(self.h, len(rec), rec.ctypes.data_as(C.POINTER(C.c_int))))
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise EcsgError("ecsg error %d: %s" % (rc, self.L.ecsg_last_error(self.h).decode()))
+
+    def _set_params(self, g):
+        G = self.model.grids[g]
+        st = [np.ascontiguousarray(a, np.float64) for a in self.model.stiff9(g)]
+        arr = (C.POINTER(C.c_double) * 9)(*[_dp(a) for a in st])
+        b = np.ascontiguousarray(G.buoy, np.float64)
+        self._chk(self.L.ecsg_set_params(self.h, g, _dp(b), arr, int(self.model.iso)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ecsg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- time stepping -----------------------------------------------------
+    def step(self, n: int = 1, graph: bool = True):
+        self._chk(self.L.ecsg_step_n(self.h, int(n), int(graph)))
+
+    def sync(self):
+        self._chk(self.L.ecsg_synchronize(self.h))
+
+    @property
+    def steps_done(self) -> int:
+        return self.L.ecsg_steps_done(self.h)
+
+    # per-phase entry points (main.cpp:179-251)
+    def update_vel(self, g): self._chk(self.L.ecsg_update_vel(self.h, g))
+    def record_receivers(self): self._chk(self.L.ecsg_record_receivers(self.h))
+    def add_source(self): self._chk(self.L.ecsg_add_source(self.h))
+    def ghost_vz(self, k): self._chk(self.L.ecsg_update_ghost_vz(self.h, k))
+    def update_stress(self, g): self._chk(self.L.ecsg_update_stress(self.h, g))
+    def ghost_sxz(self, k): self._chk(self.L.ecsg_update_ghost_sxz(self.h, k))
+    def ghost_syz(self, k): self._chk(self.L.ecsg_update_ghost_syz(self.h, k))
+    def end_step(self): self._chk(self.L.ecsg_end_step(self.h))
+
+    def energy_now(self) -> float:
+        e = C.c_double()
+        self._chk(self.L.ecsg_compute_energy(self.h, C.byref(e)))
+        return e.value
+
+    def residuals(self, which: int) -> np.ndarray:
+        out = np.zeros(max(1, self.model.num_grids - 1))
+        self._chk(self.L.ecsg_compute_res(self.h, which, _dp(out)))
+        return out
+
+    # --- data ----------------------------------------------------------------
+    def dims(self, g, sel):
+        l3 = (C.c_int * 3)()
+        self._chk(self.L.ecsg_field_dims(self.h, g, sel, l3))
+        return (l3[2], l3[1], l3[0])
+
+    def get(self, g, sel) -> np.ndarray:
+        out = np.empty(self.dims(g, sel), np.float64)
+        self._chk(self.L.ecsg_get_field(self.h, g, sel, _dp(out)))
+        return out
+
+    def set(self, g, sel, a):
+        a = np.ascontiguousarray(a, np.float64).reshape(self.dims(g, sel))
+        self._chk(self.L.ecsg_set_field(self.h, g, sel, _dp(a)))
+
+    def traces(self) -> np.ndarray:
+        n = min(self.steps_done, self.max_steps)
+        out = np.empty((self.nrec, n), np.float64)
+        self._chk(self.L.ecsg_get_traces(self.h, _dp(out)))
+        return out
+
+    def energy(self) -> np.ndarray:
+        n = min(self.steps_done, self.max_steps)
+        out = np.empty(n, np.float64)
+        self._chk(self.L.ecsg_get_energy(self.h, _dp(out)))
+        return out
+
+    def jacobi_stats(self, k: int = 0):
+        t, l, c = C.c_longlong(), C.c_int(), C.c_int()
+        self._chk(self.L.ecsg_jacobi_stats(self.h, k, C.byref(t), C.byref(l), C.byref(c)))
+        return t.value, l.value, c.value
+
+    def profile(self, nsteps: int):
+        ms = (C.c_double * 6)()
+        cnt = (C.c_int * 6)()
+        self._chk(self.L.ecsg_profile_kernels(self.h, int(nsteps), ms, cnt))
+        return list(ms), list(cnt)

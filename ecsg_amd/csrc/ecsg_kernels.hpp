@@ -1,0 +1,853 @@
+// ecsg_kernels.hpp -- CDNA4 (gfx950) kernels of the ECSG composite
+// staggered-grid time step.  Included by ecsg_capi.hip only.
+//
+// Device layout of every array of a grid (wavefields, staged stiffness,
+// buoyancy): x fastest, index range [-1, n+1] in each dimension,
+//   idx(ix, iy, iz) = o + iz*S2 + iy*P + ix,
+// where the row pitch P is padded so that ix = 0 starts a 128-byte line.
+//
+// Stencil kernels: one thread owns an (ix, iy) column and marches a chunk of
+// z planes, keeping the z-neighbours of its own column in registers; x/y
+// neighbours come from the L1/L2-resident rows of the same and adjacent
+// workgroups.  Workgroups are 64 x 4 threads (one wavefront per x row of
+// 64 nodes); the tile order is remapped so that consecutive tiles (which
+// share y halo rows) land on the same XCD and its L2.
+//
+// All floating-point expressions keep the reference's operand order and the
+// build disables contraction, so the double-precision path reproduces the
+// reference arithmetic operation for operation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+
+namespace ecsg {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+constexpr int NTHR = BX * BY;
+constexpr int MAXG = 8;
+
+template <typename T>
+struct GridDev {
+    T *f[9];        // vx vy vz sxx syy szz syz sxz sxy
+    const T *c[9];  // c11 c22 c33 c23 c13 c12 c44 c55 c66 at the stress positions
+    const T *b;     // buoyancy, primary nodes
+    int nx, ny, nz;
+    long P, S2, o;
+    T cx, cy, cz;   // dt/h per dimension (init_del_fdcoeff.cpp:19)
+    int top_eff;    // 1: vx/vy not updated on iz = 0 (rigid top, update_vel.cpp:17-18,30)
+    int bottom;     // 1: last grid, rigid bottom
+    int free_top;   // 1: free-surface top (update_stress.cpp:28-30)
+    int iso;        // isotropic compliance formula for the energy
+    int zy;         // last z plane on which vy is mirrored across the y walls
+    int zc, gx, gy, nblk;  // tiling
+};
+
+// Workgroup id -> tile id such that each XCD (ids dispatched round robin over
+// 8 XCDs) receives one contiguous range of tiles.
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int q = n >> 3, rem = n & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
+// Fixed-order workgroup reduction of one double per thread (256 threads).
+__device__ __forceinline__ double block_sum_256(double v, double *lds)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) lds[lane >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (lane == 0) r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    return r;  // valid in lane 0
+}
+
+// ---------------------------------------------------------------------------
+// UpdateVel (update_vel.cpp:4-249) with its radius-1 ghost mirrors fused.
+// EN: also the kinetic part of ComputeEnergy (compute_energy.cpp:38-39,
+// TriMulFields field.cpp:482-518) from the pre- and post-update velocities.
+template <typename T, bool EN>
+__global__ __launch_bounds__(NTHR) void k_vel(GridDev<T> g, double *__restrict__ part,
+                                              const int *__restrict__ src, T *__restrict__ src_old)
+{
+    __shared__ double lds[4];
+    const int tile = xcd_tile(blockIdx.x, g.nblk);
+    const int bx = tile % g.gx;
+    const int rest = tile / g.gx;
+    const int by = rest % g.gy;
+    const int bz = rest / g.gy;
+    const int ix = bx * BX + threadIdx.x;
+    const int iy = by * BY + threadIdx.y;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int z0 = bz * g.zc;
+    const int z1 = min(z0 + g.zc, nz + 1);
+    double acc = 0.0;
+    if (ix <= nx && iy <= ny) {
+        const long S2 = g.S2, P = g.P;
+        long p = g.o + (long)z0 * S2 + (long)iy * P + ix;
+        T *__restrict__ vx = g.f[0];
+        T *__restrict__ vy = g.f[1];
+        T *__restrict__ vz = g.f[2];
+        const T *__restrict__ sxx = g.f[3];
+        const T *__restrict__ syy = g.f[4];
+        const T *__restrict__ szz = g.f[5];
+        const T *__restrict__ syz = g.f[6];
+        const T *__restrict__ sxz = g.f[7];
+        const T *__restrict__ sxy = g.f[8];
+        const T *__restrict__ b = g.b;
+        const bool ok_x = ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1;
+        const bool ok_y = ix <= nx - 1 && iy <= ny - 1;
+        const bool ok_z = ix <= nx - 1 && iy >= 1 && iy <= ny - 1;
+        const int zlo = g.top_eff, zhi = nz - g.bottom;
+        const T cx = g.cx, cy = g.cy, cz = g.cz;
+        T sxz_m = sxz[p - S2];
+        T syz_m = syz[p - S2];
+        T szz_c = szz[p];
+        T b0 = b[p];
+        T b1 = b[p + 1];
+        for (int iz = z0; iz < z1; ++iz, p += S2) {
+            const T sxz_c = sxz[p];
+            const T syz_c = syz[p];
+            const T szz_n = szz[p + S2];
+            const T b0n = b[p + S2];
+            const T b1n = b[p + S2 + 1];
+            const bool zin = iz >= zlo && iz <= zhi;
+            // vx, update_vel.cpp:30-37
+            const T vx_o = vx[p];
+            T vx_c = vx_o;
+            if (ok_x && zin) {
+                vx_c = vx_o + b0 * (cx * (sxx[p] - sxx[p - 1]) + cy * (sxy[p] - sxy[p - P]) + cz * (sxz_c - sxz_m));
+                vx[p] = vx_c;
+            }
+            // vy, update_vel.cpp:43-53
+            const T vy_o = vy[p];
+            T vy_c = vy_o;
+            T bP = 0, bP1 = 0;
+            if (ok_y) {
+                bP = b[p + P];
+                bP1 = b[p + P + 1];
+                if (zin) {
+                    const T bav = T(0.25) * (b0 + b1 + bP + bP1);
+                    vy_c = vy_o + bav * (cx * (sxy[p + 1] - sxy[p]) + cz * (syz_c - syz_m) + cy * (syy[p + P] - syy[p]));
+                    vy[p] = vy_c;
+                }
+            }
+            // vz, update_vel.cpp:59-69
+            const T vz_o = vz[p];
+            T vz_c = vz_o;
+            if (ok_z && iz <= nz - 1) {
+                const T bav = T(0.25) * (b0 + b0n + b1 + b1n);
+                vz_c = vz_o + bav * (cx * (sxz[p + 1] - sxz_c) + cz * (szz_n - szz_c) + cy * (syz_c - syz[p - P]));
+                vz[p] = vz_c;
+            }
+            // ghost velocities for radius 1 (update_vel.cpp:75-246), same
+            // index ranges as the reference loops
+            if (g.top_eff && iz == 0 && ix <= nx - 1) vz[p - S2] = -vz_c;
+            if (g.bottom && iz == nz - 1 && ix <= nx - 1) vz[p + S2] = -vz_c;
+            if (ix == 0) {
+                if (iz <= nz - 1) vz[p - 1] = -vz_c;
+                if (iy <= ny - 1) vy[p - 1] = -vy_c;
+            }
+            if (ix == nx - 1) {
+                if (iz <= nz - 1) vz[p + 1] = -vz_c;
+                if (iy <= ny - 1) vy[p + 1] = -vy_c;
+            }
+            if (ix <= nx - 1) {
+                if (iy == 0 && iz <= g.zy) vy[p - P] = -vy_c;
+                if (iy == ny - 1) {
+                    // the x mirrors of row iy = ny copy that halo row before
+                    // the y mirror below overwrites it (reference loop order)
+                    if (ix == 0) vy[p + P - 1] = -vy[p + P];
+                    if (ix == nx - 1) vy[p + P + 1] = -vy[p + P];
+                    if (iz <= g.zy) vy[p + P] = -vy_c;
+                }
+            }
+            if (EN) {
+                const double m2 = (iz == 0 || iz == nz) ? 0.5 : 1.0;
+                const double m1 = (iy == 0 || iy == ny) ? 0.5 : 1.0;
+                const double m0 = (ix == 0 || ix == nx) ? 0.5 : 1.0;
+                // density for vx: 1/buoy (init_del_par_field.cpp:231)
+                acc += m2 * m1 * m0 * (1.0 / (double)b0) * (double)vx_c * (double)vx_o;
+                if (src && ix == src[0] && iy == src[1] && iz == src[2]) *src_old = vx_o;
+                if (ok_y) {
+                    // density for vy: 1/ReshapeField(buoy,(1,1,0)) (field.cpp:465-472)
+                    const double rb = ((((((((double)b0 + (double)b0) + (double)bP) + (double)bP) + (double)b1) + (double)b1)
+                                        + (double)bP1) + (double)bP1) / 8;
+                    acc += m2 * 1.0 * 1.0 * (1.0 / rb) * (double)vy_c * (double)vy_o;
+                }
+                if (ix <= nx - 1 && iz <= nz - 1) {
+                    const double rb = ((((((((double)b0 + (double)b0n) + (double)b0) + (double)b0n) + (double)b1)
+                                         + (double)b1n) + (double)b1) + (double)b1n) / 8;
+                    acc += 1.0 * m1 * 1.0 * (1.0 / rb) * (double)vz_c * (double)vz_o;
+                }
+            }
+            sxz_m = sxz_c;
+            syz_m = syz_c;
+            szz_c = szz_n;
+            b0 = b0n;
+            b1 = b1n;
+        }
+    }
+    if (EN) {
+        const double s = block_sum_256(acc, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = s;
+    }
+}
+
+// compliance of the normal block at one point (energy weights)
+template <typename T>
+__device__ __forceinline__ void compliance(const GridDev<T> &g, T c11, T c22, T c33, T c23, T c13, T c12,
+                                           double &s11, double &s22, double &s33, double &s23, double &s13,
+                                           double &s12)
+{
+    if (g.iso) {
+        // init_del_par_field.cpp:259-278
+        const double a = c11, b = c13;
+        double base = a * b * b * 3 - b * b * b * 2 - a * a * a;
+        base = 1 / base;
+        s11 = s22 = s33 = (b * b - a * a) * base;
+        s23 = s13 = s12 = (b * a - b * b) * base;
+    } else {
+        const double m11 = c11, m22 = c22, m33 = c33, m23 = c23, m13 = c13, m12 = c12;
+        const double a11 = m22 * m33 - m23 * m23, a22 = m11 * m33 - m13 * m13, a33 = m11 * m22 - m12 * m12;
+        const double a23 = m12 * m13 - m11 * m23, a13 = m12 * m23 - m13 * m22, a12 = m13 * m23 - m12 * m33;
+        const double det = m11 * a11 + m12 * a12 + m13 * a13;
+        s11 = a11 / det; s22 = a22 / det; s33 = a33 / det;
+        s23 = a23 / det; s13 = a13 / det; s12 = a12 / det;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// UpdateStress (update_stress.cpp:4-311), free surface and its radius-1 ghost
+// stresses fused.  EN: strain part of ComputeEnergy (compute_energy.cpp:40-49)
+// from the pre-update stresses.
+template <typename T, bool EN>
+__global__ __launch_bounds__(NTHR) void k_stress(GridDev<T> g, double *__restrict__ part)
+{
+    __shared__ double lds[4];
+    const int tile = xcd_tile(blockIdx.x, g.nblk);
+    const int bx = tile % g.gx;
+    const int rest = tile / g.gx;
+    const int by = rest % g.gy;
+    const int bz = rest / g.gy;
+    const int ix = bx * BX + threadIdx.x;
+    const int iy = by * BY + threadIdx.y;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int z0 = bz * g.zc;
+    const int z1 = min(z0 + g.zc, nz + 1);
+    double acc = 0.0;
+    if (ix <= nx && iy <= ny) {
+        const long S2 = g.S2, P = g.P;
+        long p = g.o + (long)z0 * S2 + (long)iy * P + ix;
+        const T *__restrict__ vx = g.f[0];
+        const T *__restrict__ vy = g.f[1];
+        const T *__restrict__ vz = g.f[2];
+        T *__restrict__ sxx = g.f[3];
+        T *__restrict__ syy = g.f[4];
+        T *__restrict__ szz = g.f[5];
+        T *__restrict__ syz = g.f[6];
+        T *__restrict__ sxz = g.f[7];
+        T *__restrict__ sxy = g.f[8];
+        const T *__restrict__ C11 = g.c[0];
+        const T *__restrict__ C22 = g.c[1];
+        const T *__restrict__ C33 = g.c[2];
+        const T *__restrict__ C23 = g.c[3];
+        const T *__restrict__ C13 = g.c[4];
+        const T *__restrict__ C12 = g.c[5];
+        const T *__restrict__ C44 = g.c[6];
+        const T *__restrict__ C55 = g.c[7];
+        const T *__restrict__ C66 = g.c[8];
+        const bool ok_n = ix <= nx - 1;
+        const bool ok_yz = ix <= nx - 1 && iy <= ny - 1;
+        const bool ok_xy = iy <= ny - 1;
+        const T cx = g.cx, cy = g.cy, cz = g.cz;
+        T vz_m = vz[p - S2];
+        T vx_c = vx[p];
+        T vy_c = vy[p];
+        for (int iz = z0; iz < z1; ++iz, p += S2) {
+            const T vx_n = vx[p + S2];
+            const T vy_n = vy[p + S2];
+            const T vz_c = vz[p];
+            if (ok_n) {
+                const T sxx_o = sxx[p], syy_o = syy[p], szz_o = szz[p];
+                const T c11 = C11[p], c22 = C22[p], c33 = C33[p], c23 = C23[p], c13 = C13[p], c12 = C12[p];
+                const T dvx = vx[p + 1] - vx_c;
+                const T dvy = vy_c - vy[p - P];
+                if (g.free_top && iz == 0) {
+                    // update_stress.cpp:37-65
+                    sxx[p] = sxx_o + ((c11 - c13 * c13 / c33) * cx * dvx + (c12 - c13 * c23 / c33) * cy * dvy);
+                    syy[p] = syy_o + ((c12 - c23 * c13 / c33) * cx * dvx + (c22 - c23 * c23 / c33) * cy * dvy);
+                } else {
+                    // update_stress.cpp:69-97
+                    const T dvz = vz_c - vz_m;
+                    sxx[p] = sxx_o + (c11 * cx * dvx + c12 * cy * dvy + c13 * cz * dvz);
+                    syy[p] = syy_o + (c12 * cx * dvx + c22 * cy * dvy + c23 * cz * dvz);
+                    szz[p] = szz_o + (c13 * cx * dvx + c23 * cy * dvy + c33 * cz * dvz);
+                }
+                if (EN) {
+                    double s11, s22, s33, s23, s13, s12;
+                    compliance(g, c11, c22, c33, c23, c13, c12, s11, s22, s33, s23, s13, s12);
+                    const double m = ((iz == 0 || iz == nz) ? 0.5 : 1.0) * ((iy == 0 || iy == ny) ? 0.5 : 1.0);
+                    const double a = sxx_o, bb = syy_o, c = szz_o;
+                    acc += m * s11 * a * a + m * s22 * bb * bb + m * s33 * c * c
+                         + 2 * (m * s23 * bb * c) + 2 * (m * s13 * a * c) + 2 * (m * s12 * a * bb);
+                }
+            }
+            // syz, update_stress.cpp:103-109
+            if (ok_yz && iz <= nz - 1) {
+                const T syz_o = syz[p];
+                const T c44 = C44[p];
+                const T syz_n = syz_o + T(0.5) * c44 * (cz * (vy_n - vy_c) + cy * (vz[p + P] - vz_c));
+                syz[p] = syz_n;
+                if (g.free_top && iz == 0) syz[p - S2] = -syz_n;  // update_stress.cpp:264-267
+                if (EN) acc += 2 * ((1.0 / (double)c44) * (double)syz_o * (double)syz_o);
+            }
+            // sxz, update_stress.cpp:115-121
+            if (iz <= nz - 1) {
+                const T sxz_o = sxz[p];
+                const T c55 = C55[p];
+                const T sxz_n = sxz_o + T(0.5) * c55 * (cz * (vx_n - vx_c) + cx * (vz_c - vz[p - 1]));
+                sxz[p] = sxz_n;
+                if (g.free_top && iz == 0 && ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1)
+                    sxz[p - S2] = -sxz_n;  // update_stress.cpp:274-277
+                if (EN) {
+                    const double m = ((iy == 0 || iy == ny) ? 0.5 : 1.0) * ((ix == 0 || ix == nx) ? 0.5 : 1.0);
+                    acc += 2 * (m * (1.0 / (double)c55) * (double)sxz_o * (double)sxz_o);
+                }
+            }
+            // sxy, update_stress.cpp:127-133
+            if (ok_xy) {
+                const T sxy_o = sxy[p];
+                const T c66 = C66[p];
+                sxy[p] = sxy_o + T(0.5) * c66 * (cy * (vx[p + P] - vx_c) + cx * (vy_c - vy[p - 1]));
+                if (EN) {
+                    const double m = ((iz == 0 || iz == nz) ? 0.5 : 1.0) * ((ix == 0 || ix == nx) ? 0.5 : 1.0);
+                    acc += 2 * (m * (1.0 / (double)c66) * (double)sxy_o * (double)sxy_o);
+                }
+            }
+            vz_m = vz_c;
+            vx_c = vx_n;
+            vy_c = vy_n;
+        }
+    }
+    if (EN) {
+        const double s = block_sum_256(acc, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Interface kernels.  Fine grid F (upper, z index nz1 on the interface plane)
+// and coarse grid C (lower, z index 0 on the plane); refinement ratio r.
+template <typename T>
+struct IfaceDev {
+    GridDev<T> F, C;
+    int r;
+    int nx2, ny2, nz1;
+    double k1x, k1y, k1z, k2x, k2y, k2z;  // coefficients as the stencils use them
+};
+
+__device__ __forceinline__ double hatw(int a, int r)
+{
+    return (double)(r - (a < 0 ? -a : a)) / (double)r;
+}
+
+template <typename T>
+__device__ __forceinline__ double ld(const T *a, long i) { return (double)a[i]; }
+
+#define FIDX(G, i, j, k) ((G).o + (long)(k) * (G).S2 + (long)(j) * (G).P + (i))
+
+// vz-szz pair: right-hand side of the tridiagonal system of each coarse
+// x-line (derived ratio-r form of update_ghost_vz.cpp:152-203).
+// B layout [J-1][I].
+template <typename T>
+__global__ __launch_bounds__(256) void k_gvz_rhs(IfaceDev<T> a, double *__restrict__ B)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx2 = a.nx2, ny2 = a.ny2, r = a.r, nz1 = a.nz1;
+    if (t >= nx2 * (ny2 - 1)) return;
+    const int I = t % nx2;
+    const int J = t / nx2 + 1;
+    const GridDev<T> &F = a.F;
+    const GridDev<T> &C = a.C;
+    const T *vx1 = F.f[0], *vy1 = F.f[1], *vz1 = F.f[2], *szz1 = F.f[5];
+    const T *vx2 = C.f[0], *vy2 = C.f[1], *vz2 = C.f[2], *szz2 = C.f[5];
+    const T *c33_1 = F.c[2], *c23_1 = F.c[3], *c13_1 = F.c[4];
+    const T *c33_2 = C.c[2], *c23_2 = C.c[3], *c13_2 = C.c[4];
+    double fs = 0;
+    for (int bo = -(r - 1); bo <= r - 1; bo++) {
+        const int j = r * J + bo;
+        const double wj = (double)(r - (bo < 0 ? -bo : bo));
+        const int Jl = (bo >= 0) ? J : J - 1;
+        const int bb = j - r * Jl;
+        const double pv = hatw(bb, r) * ld(vz2, FIDX(C, I, Jl, 0)) + hatw(bb - r, r) * ld(vz2, FIDX(C, I, Jl + 1, 0));
+        for (int ao = 0; ao < r; ao++) {
+            const int i = r * I + ao;
+            const long q = FIDX(F, i, j, nz1);
+            const double cc = ld(c33_1, q);
+            const double kf = ld(szz1, q) + ld(c13_1, q) * a.k1x * (ld(vx1, q + 1) - ld(vx1, q))
+                            + ld(c23_1, q) * a.k1y * (ld(vy1, q) - ld(vy1, q - F.P))
+                            + cc * a.k1z * (pv - 2 * ld(vz1, q - F.S2));
+            fs += wj * kf;
+        }
+    }
+    const long qc = FIDX(C, I, J, 0);
+    const double kc = ld(szz2, qc) + ld(c13_2, qc) * a.k2x * (ld(vx2, qc + 1) - ld(vx2, qc))
+                    + ld(c23_2, qc) * a.k2y * (ld(vy2, qc) - ld(vy2, qc - C.P))
+                    + ld(c33_2, qc) * a.k2z * ld(vz2, qc);
+    B[(long)(J - 1) * nx2 + I] = (double)(r * r * r) * kc - fs;
+}
+
+// Prefactored Thomas solve, one thread per coarse x-line (update_ghost_vz.cpp:
+// 211-220 with the time-invariant elimination factors computed once).
+// M: multipliers, Dp: eliminated diagonal, U: super-diagonal, all [q][I].
+template <typename T>
+__global__ __launch_bounds__(64) void k_gvz_solve(IfaceDev<T> a, double *__restrict__ B, const double *__restrict__ M,
+                                                  const double *__restrict__ Dp, const double *__restrict__ U)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx2 = a.nx2, n = a.ny2 - 1;
+    if (I >= nx2) return;
+    constexpr int CH = 16;
+    double prev = 0;
+    for (int q0 = 0; q0 < n; q0 += CH) {
+        double bq[CH], mq[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (q0 + t < n) {
+                bq[t] = B[(long)(q0 + t) * nx2 + I];
+                mq[t] = M[(long)(q0 + t) * nx2 + I];
+            }
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (q0 + t < n) {
+                const double v = (q0 + t == 0) ? bq[t] : bq[t] - mq[t] * prev;
+                B[(long)(q0 + t) * nx2 + I] = v;
+                prev = v;
+            }
+    }
+    T *vz2 = a.C.f[2];
+    double x = prev / Dp[(long)(n - 1) * nx2 + I];
+    vz2[FIDX(a.C, I, n, -1)] = (T)x;
+    for (int q1 = n - 2; q1 >= 0; q1 -= CH) {
+        double bq[CH], uq[CH], dq[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (q1 - t >= 0) {
+                bq[t] = B[(long)(q1 - t) * nx2 + I];
+                uq[t] = U[(long)(q1 - t) * nx2 + I];
+                dq[t] = Dp[(long)(q1 - t) * nx2 + I];
+            }
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (q1 - t >= 0) {
+                x = (bq[t] - uq[t] * x) / dq[t];
+                vz2[FIDX(a.C, I, q1 - t + 1, -1)] = (T)x;
+            }
+    }
+}
+
+// Fine ghosts vz1(., ., nz1): injection in x, hat in y (update_ghost_vz.cpp:
+// 231-248 generalised).  One thread per fine (i, j) with j in [1, ny1-1].
+template <typename T>
+__global__ __launch_bounds__(256) void k_gvz_fine(IfaceDev<T> a)
+{
+    const int r = a.r, nx1 = r * a.nx2, ny1 = r * a.ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx1 * (ny1 - 1)) return;
+    const int i = t % nx1;
+    const int j = t / nx1 + 1;
+    const int I = i / r, J = j / r, bo = j - r * J;
+    const T *vz2 = a.C.f[2];
+    T *vz1 = a.F.f[2];
+    const double t0 = ld(vz2, FIDX(a.C, I, J, 0)) + ld(vz2, FIDX(a.C, I, J, -1));
+    double tmp;
+    if (bo == 0) {
+        tmp = t0;
+    } else {
+        const double t1 = ld(vz2, FIDX(a.C, I, J + 1, 0)) + ld(vz2, FIDX(a.C, I, J + 1, -1));
+        tmp = hatw(bo, r) * t0 + hatw(bo - r, r) * t1;
+    }
+    const long q = FIDX(a.F, i, j, a.nz1);
+    vz1[q] = (T)(-ld(vz1, q - a.F.S2) + tmp);
+}
+
+// vx-sxz pair: right-hand side on interior coarse nodes (derived ratio-r form
+// of update_ghost_sxz.cpp:139-226) and per-workgroup sums of |B|
+// (AvgAbs, array.cpp:188-197).  B layout [J-1][I-1].
+template <typename T>
+__global__ __launch_bounds__(256) void k_gsxz_rhs(IfaceDev<T> a, double *__restrict__ B, double *__restrict__ abs_part)
+{
+    __shared__ double lds[4];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx2 = a.nx2, ny2 = a.ny2, r = a.r, nz1 = a.nz1;
+    const int n0 = nx2 - 1, n1 = ny2 - 1;
+    double v = 0;
+    if (t < n0 * n1) {
+        const int I = t % n0 + 1;
+        const int J = t / n0 + 1;
+        const GridDev<T> &F = a.F;
+        const GridDev<T> &C = a.C;
+        const T *vx1 = F.f[0], *sxx1 = F.f[3], *sxz1 = F.f[7], *sxy1 = F.f[8], *b1 = F.b;
+        const T *vx2 = C.f[0], *sxx2 = C.f[3], *sxz2 = C.f[7], *sxy2 = C.f[8], *b2 = C.b;
+        double fs = 0;
+        for (int bo = -(r - 1); bo <= r - 1; bo++)
+            for (int ao = -(r - 1); ao <= r - 1; ao++) {
+                const int i = r * I + ao, j = r * J + bo;
+                const double w = (double)((r - (ao < 0 ? -ao : ao)) * (r - (bo < 0 ? -bo : bo)));
+                const long q = FIDX(F, i, j, nz1);
+                const double bb = ld(b1, q);
+                double S = 0;
+                for (int dJ = -1; dJ <= 1; dJ++)
+                    for (int dI = -1; dI <= 1; dI++) {
+                        const int Ic = I + dI, Jc = J + dJ;
+                        const int ai = i - r * Ic, bj = j - r * Jc;
+                        if (ai >= r || ai <= -r || bj >= r || bj <= -r) continue;
+                        if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
+                        const double pw = hatw(ai, r) * hatw(bj, r);
+                        S += pw * ld(sxz2, FIDX(C, Ic, Jc, 0));
+                    }
+                const double kf = ld(vx1, q) + bb * (a.k1x * (ld(sxx1, q) - ld(sxx1, q - 1))
+                                                   + a.k1y * (ld(sxy1, q) - ld(sxy1, q - F.P))
+                                                   + a.k1z * (S - 2 * ld(sxz1, q - F.S2)));
+                fs += w * kf;
+            }
+        const long qc = FIDX(C, I, J, 0);
+        const double kc = ld(vx2, qc) + ld(b2, qc) * (a.k2x * (ld(sxx2, qc) - ld(sxx2, qc - 1))
+                                                     + a.k2y * (ld(sxy2, qc) - ld(sxy2, qc - C.P))
+                                                     + a.k2z * ld(sxz2, qc));
+        const double r2 = (double)(r * r);
+        v = (r2 * r2) * kc - fs;
+        B[t] = v;
+    }
+    // 1-D block of 256: reuse the 2-D helper with threadIdx.y == 0
+    double s = v < 0 ? -v : v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) abs_part[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+struct JacobiArgs {
+    const double *A;      // 9 bands, band k at A + k*nn, [J-1][I-1]
+    const double *B;
+    double *X0, *X1;      // iterates (nx2+1)(ny2+1), zero border; X0 holds the warm start
+    const double *abs_part;
+    int nabs;
+    double *part;         // 2 parities x 2 sums x gridDim.x
+    long long *stats;     // total, last, capped
+    double tol_cap, tol_fac;
+    double floor;         // rounding-floor factor of the stagnation exit
+    int cap;
+};
+
+// Sum of v[0..n-1] by one wavefront in a fixed order; all lanes get it.
+__device__ __forceinline__ double wave_sum_fixed(const double *v, int n)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0;
+    for (int i = lane; i < n; i += 64) s += v[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+// Jacobi2D + Norm2D (update_ghost_sxz.cpp:314-388) as one cooperative
+// persistent kernel: every iteration ends with a grid barrier, after which
+// every workgroup sums the per-workgroup norms in the same order, so all of
+// them take the same exit decision.  Then the scatter of the coarse ghosts
+// and the fine ghosts (update_ghost_sxz.cpp:247-295, generalised to ratio r).
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi(IfaceDev<T> a, JacobiArgs j)
+{
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    __shared__ double lds[4];
+    __shared__ double bcast;
+    const int nx2 = a.nx2, ny2 = a.ny2;
+    const int n0 = nx2 - 1, n1 = ny2 - 1;
+    const long nn = (long)n0 * n1;
+    const long w = nx2 + 1;
+    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gstride = (long)gridDim.x * blockDim.x;
+    if (threadIdx.x < 64) {
+        const double sb = wave_sum_fixed(j.abs_part, j.nabs);
+        if (threadIdx.x == 0) {
+            const double avg = sb / (double)nn;
+            const double t2 = avg * j.tol_fac;
+            bcast = (j.tol_cap < t2) ? j.tol_cap : t2;
+        }
+    }
+    __syncthreads();
+    const double tol = bcast;
+    const double *A = j.A;
+    double *Xo = j.X0, *Xn = j.X1;
+    int it = 0;
+    double prev = 0;
+    __shared__ double lds2[4];
+    __shared__ double bcast2;
+    while (true) {
+        double acc = 0, ax = 0;
+        for (long q = gtid; q < nn; q += gstride) {
+            const long i0 = q % n0 + 1, i1 = q / n0 + 1;
+            const double xn = (j.B[q]
+                               - A[0 * nn + q] * Xo[(i1 - 1) * w + (i0 - 1)]
+                               - A[1 * nn + q] * Xo[(i1) * w + (i0 - 1)]
+                               - A[2 * nn + q] * Xo[(i1 + 1) * w + (i0 - 1)]
+                               - A[3 * nn + q] * Xo[(i1 - 1) * w + (i0)]
+                               - A[5 * nn + q] * Xo[(i1 + 1) * w + (i0)]
+                               - A[6 * nn + q] * Xo[(i1 - 1) * w + (i0 + 1)]
+                               - A[7 * nn + q] * Xo[(i1) * w + (i0 + 1)]
+                               - A[8 * nn + q] * Xo[(i1 + 1) * w + (i0 + 1)]) / A[4 * nn + q];
+            const double d = xn - Xo[i1 * w + i0];
+            acc += d * d;
+            ax += xn * xn;
+            Xn[i1 * w + i0] = xn;
+        }
+        double s = acc, sx = ax;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            s += __shfl_xor(s, off, 64);
+            sx += __shfl_xor(sx, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            lds[threadIdx.x >> 6] = s;
+            lds2[threadIdx.x >> 6] = sx;
+        }
+        __syncthreads();
+        double *part = j.part + (it & 1) * 2 * gridDim.x;
+        if (threadIdx.x == 0) {
+            part[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+            part[gridDim.x + blockIdx.x] = ((lds2[0] + lds2[1]) + lds2[2]) + lds2[3];
+        }
+        grid.sync();
+        if (threadIdx.x < 64) {
+            const double n2 = wave_sum_fixed(part, gridDim.x);
+            const double x2 = wave_sum_fixed(part + gridDim.x, gridDim.x);
+            if (threadIdx.x == 0) { bcast = n2; bcast2 = x2; }
+        }
+        __syncthreads();
+        const double norm = sqrt(bcast);
+        const double xnorm = sqrt(bcast2);
+        it++;
+        double *tsw = Xo; Xo = Xn; Xn = tsw;
+        // the reference's exit, a rounding-floor stagnation exit, and a cap
+        if (!(norm > tol)) break;
+        if (it > 1 && norm >= prev && norm <= j.floor * xnorm) break;
+        if (it >= j.cap) break;
+        prev = norm;
+    }
+    // Xo holds the final iterate; keep it in X0 for the next warm start and
+    // write the coarse ghosts sxz2(I, J, -1).
+    T *sxz2 = a.C.f[7];
+    for (long q = gtid; q < nn; q += gstride) {
+        const long i0 = q % n0 + 1, i1 = q / n0 + 1;
+        const double x = Xo[i1 * w + i0];
+        if (Xo != j.X0) j.X0[i1 * w + i0] = x;
+        sxz2[FIDX(a.C, i0, i1, -1)] = (T)x;
+    }
+    grid.sync();
+    // fine ghosts on fine interior nodes
+    const int r = a.r, nx1 = r * nx2, ny1 = r * ny2;
+    T *sxz1 = a.F.f[7];
+    const long nf = (long)(nx1 - 1) * (ny1 - 1);
+    for (long q = gtid; q < nf; q += gstride) {
+        const int i = (int)(q % (nx1 - 1)) + 1, jj = (int)(q / (nx1 - 1)) + 1;
+        const int I0 = i / r, J0 = jj / r, ao = i - r * I0, bo = jj - r * J0;
+        double S = 0;
+        for (int dJ = 0; dJ <= 1; dJ++)
+            for (int dI = 0; dI <= 1; dI++) {
+                const int Ic = I0 + dI, Jc = J0 + dJ;
+                if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
+                const double pw = hatw(ao - r * dI, r) * hatw(bo - r * dJ, r);
+                if (pw == 0) continue;
+                S += pw * (ld(sxz2, FIDX(a.C, Ic, Jc, -1)) + ld(sxz2, FIDX(a.C, Ic, Jc, 0)));
+            }
+        const long qf = FIDX(a.F, i, jj, a.nz1);
+        sxz1[qf] = (T)(-ld(sxz1, qf - a.F.S2) + S);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        j.stats[0] += it;
+        j.stats[1] = it;
+        if (it >= j.cap) j.stats[2] = 1;
+    }
+}
+
+// vy-syz pair: pointwise coarse ghost and its fine children (derived ratio-r
+// form of update_ghost_syz.cpp:162-238).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gsyz(IfaceDev<T> a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx2 = a.nx2, ny2 = a.ny2, r = a.r, nz1 = a.nz1;
+    if (t >= nx2 * ny2) return;
+    const int I = t % nx2, J = t / nx2;
+    const GridDev<T> &F = a.F;
+    const GridDev<T> &C = a.C;
+    const T *syy1 = F.f[4], *sxy1 = F.f[8], *vy1 = F.f[1], *b1 = F.b;
+    T *syz1 = F.f[6];
+    const T *syy2 = C.f[4], *sxy2 = C.f[8], *vy2 = C.f[1], *b2 = C.b;
+    T *syz2 = C.f[6];
+    const long qc = FIDX(C, I, J, 0);
+    const double c0 = 0.25 * (((ld(b2, qc) + ld(b2, qc + 1)) + ld(b2, qc + C.P)) + ld(b2, qc + C.P + 1));
+    const double s2 = ld(syz2, qc);
+    const double r2 = (double)(r * r);
+    double num = r2 * (ld(vy2, qc) + c0 * (a.k2x * (ld(sxy2, qc + 1) - ld(sxy2, qc))
+                                         + a.k2y * (ld(syy2, qc + C.P) - ld(syy2, qc))
+                                         + a.k2z * s2));
+    double den = r2 * c0 * a.k2z;
+    for (int bo = 0; bo < r; bo++)
+        for (int ao = 0; ao < r; ao++) {
+            const long q = FIDX(F, r * I + ao, r * J + bo, nz1);
+            const double c = 0.25 * (((ld(b1, q) + ld(b1, q + 1)) + ld(b1, q + F.P)) + ld(b1, q + F.P + 1));
+            num -= ld(vy1, q) + c * (a.k1x * (ld(sxy1, q + 1) - ld(sxy1, q))
+                                    + a.k1y * (ld(syy1, q + F.P) - ld(syy1, q))
+                                    + a.k1z * (s2 - 2 * ld(syz1, q - F.S2)));
+            den += c * a.k1z;
+        }
+    const T g2 = (T)(num / den);
+    syz2[qc - C.S2] = g2;
+    const double tmp = (double)g2 + s2;
+    for (int bo = 0; bo < r; bo++)
+        for (int ao = 0; ao < r; ao++) {
+            const long q = FIDX(F, r * I + ao, r * J + bo, nz1);
+            syz1[q] = (T)(-ld(syz1, q - F.S2) + tmp);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Receivers, source, energy finalisation and the step counter.
+template <typename T>
+struct PointArgs {
+    T *vx[MAXG];
+    T *sxx[MAXG], *syy[MAXG], *szz[MAXG];
+    const T *b[MAXG];
+    long o[MAXG], P[MAXG], S2[MAXG];
+    int n[MAXG][3];
+    int nrec;
+    const int *rec;      // nrec x 4
+    T *traces;           // nrec x max_steps
+    int max_steps;
+    int *step;           // device step counter
+    int src_g, src_i, src_j, src_k, src_type, src_len;
+    const float *src_tr;
+    const T *src_old;    // pre-update vx at the source point (energy)
+    double *fix;         // kinetic-energy correction of the injected source
+    int energy;
+};
+
+// main.cpp:183-195: receivers sample vx after UpdateVel, then the source is added.
+template <typename T>
+__global__ __launch_bounds__(256) void k_recsrc(PointArgs<T> a)
+{
+    const int step = *a.step;
+    if (step < a.max_steps)
+        for (int q = threadIdx.x; q < a.nrec; q += blockDim.x) {
+            const int g = a.rec[4 * q], i = a.rec[4 * q + 1], jj = a.rec[4 * q + 2], k = a.rec[4 * q + 3];
+            a.traces[(long)q * a.max_steps + step] = a.vx[g][a.o[g] + (long)k * a.S2[g] + (long)jj * a.P[g] + i];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double fix = 0;
+        if (step < a.src_len) {
+            const int g = a.src_g;
+            const long q = a.o[g] + (long)a.src_k * a.S2[g] + (long)a.src_j * a.P[g] + a.src_i;
+            if (a.src_type == 0) {
+                const float add = 1000 * a.src_tr[step];
+                a.vx[g][q] = a.vx[g][q] + (T)add;
+                if (a.energy) {
+                    const int *n = a.n[g];
+                    const double m = ((a.src_k == 0 || a.src_k == n[2]) ? 0.5 : 1.0)
+                                   * ((a.src_j == 0 || a.src_j == n[1]) ? 0.5 : 1.0)
+                                   * ((a.src_i == 0 || a.src_i == n[0]) ? 0.5 : 1.0);
+                    fix = m * (1.0 / (double)a.b[g][q]) * (double)(T)add * (double)(*a.src_old);
+                }
+            } else {
+                const T add = (T)a.src_tr[step];
+                a.sxx[g][q] = a.sxx[g][q] + add;
+                a.syy[g][q] = a.syy[g][q] + add;
+                a.szz[g][q] = a.szz[g][q] + add;
+            }
+        }
+        *a.fix = fix;
+        if (!a.energy) *a.step = step + 1;
+    }
+}
+
+struct EnergyArgs {
+    const double *ke[MAXG];
+    const double *se[MAXG];
+    int nblk[MAXG];
+    double vol[MAXG];   // 0.5*hx*hy*hz
+    int ng;
+    int src_g;
+    const double *fix;
+    double *energy;
+    int *step;
+    int max_steps;
+};
+
+// compute_energy.cpp:34-54 summed over grids (main.cpp:219-224); ends the step.
+__global__ __launch_bounds__(64) void k_energy(EnergyArgs a)
+{
+    double e = 0;
+    for (int g = 0; g < a.ng; g++) {
+        double sk = wave_sum_fixed(a.ke[g], a.nblk[g]);
+        double ss = wave_sum_fixed(a.se[g], a.nblk[g]);
+        double kin = sk;
+        if (g == a.src_g) kin += *a.fix;
+        e += (kin + ss) * a.vol[g];
+    }
+    if (threadIdx.x == 0) {
+        const int step = *a.step;
+        if (step < a.max_steps) a.energy[step] = e;
+        *a.step = step + 1;
+    }
+}
+
+__global__ void k_end_step(int *step)
+{
+    if (threadIdx.x == 0) *step = *step + 1;
+}
+
+// compute_res.cpp:4-95 with r^2: fine-plane sum minus r^2 times the coarse
+// sum, one workgroup, fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_res(IfaceDev<T> a, int which, double *out)
+{
+    __shared__ double lds[4];
+    const int r = a.r, nx1 = r * a.nx2, ny1 = r * a.ny2, nx2 = a.nx2, ny2 = a.ny2, nz1 = a.nz1;
+    const T *vf, *sf, *vc, *sc;
+    int i0s, j0s, i0e, j0e;  // fine ranges [s, e)
+    if (which == 0) { vf = a.F.f[0]; sf = a.F.f[7]; vc = a.C.f[0]; sc = a.C.f[7]; i0s = 1; j0s = 1; i0e = nx1; j0e = ny1; }
+    else if (which == 1) { vf = a.F.f[1]; sf = a.F.f[6]; vc = a.C.f[1]; sc = a.C.f[6]; i0s = 0; j0s = 0; i0e = nx1; j0e = ny1; }
+    else { vf = a.F.f[5]; sf = a.F.f[2]; vc = a.C.f[5]; sc = a.C.f[2]; i0s = 0; j0s = 1; i0e = nx1; j0e = ny1; }
+    double s1 = 0, s2 = 0;
+    const long nfx = i0e - i0s, nf = nfx * (j0e - j0s);
+    for (long q = threadIdx.x; q < nf; q += blockDim.x) {
+        const int i = (int)(q % nfx) + i0s, jj = (int)(q / nfx) + j0s;
+        const long p = FIDX(a.F, i, jj, nz1);
+        s1 += ld(vf, p) * (ld(sf, p - a.F.S2) + ld(sf, p));
+    }
+    const int ics = i0s, jcs = j0s, ice = (which == 0) ? nx2 : nx2, jce = ny2;
+    const long ncx = ice - ics, nc = ncx * (jce - jcs);
+    const double rr = (double)(r * r);
+    for (long q = threadIdx.x; q < nc; q += blockDim.x) {
+        const int i = (int)(q % ncx) + ics, jj = (int)(q / ncx) + jcs;
+        const long p = FIDX(a.C, i, jj, 0);
+        s2 += rr * ld(vc, p) * (ld(sc, p - a.C.S2) + ld(sc, p));
+    }
+    double t = s1 - s2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+#undef FIDX
+
+}  // namespace ecsg

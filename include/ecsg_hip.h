@@ -1,0 +1,146 @@
+/*
+ * ecsg_hip.h -- C ABI of the MI355X (gfx950) ECSG time-stepping path.
+ *
+ * The reference (llodds/ecsg) is a C++/OpenMP code whose time loop in
+ * main.cpp:165-261 calls free functions over caller-owned Field/Array objects
+ * (sim_utils.hpp:128-161).  This library owns device copies of those objects
+ * behind an opaque handle and exposes one entry point per line of that loop,
+ * plus a fused, graph-captured multi-step entry point.  Every function returns
+ * an int status (ECSG_OK = 0, negative on error); ecsg_last_error() gives the
+ * message.  No exceptions and no exit() cross this boundary.
+ *
+ * Conventions
+ *   - Arrays are x-fastest, like the reference's Field (field.hpp:84-85).
+ *   - Host pointers are borrowed for the duration of a call only.
+ *   - One HIP stream per handle; a handle is used from one host thread at a
+ *     time.  Distinct handles may run concurrently.
+ *   - Precision: ECSG_F64 stores wavefields and parameters in double (the
+ *     reference's default build, init.hpp:16-18); ECSG_F32 stores them in
+ *     float.  Interface ghost solves, energy and residual reductions always
+ *     compute in double.
+ */
+#ifndef ECSG_HIP_H
+#define ECSG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ECSG_MAX_GRIDS 8
+
+enum {
+    ECSG_OK = 0,
+    ECSG_EINVAL = -1,   /* bad argument / unsupported configuration */
+    ECSG_EHIP = -2,     /* HIP runtime error */
+    ECSG_ENOMEM = -3,   /* device allocation failed */
+    ECSG_ENOCONV = -4,  /* Jacobi iteration cap reached */
+    ECSG_ESTATE = -5    /* call out of order (e.g. params not set) */
+};
+
+enum { ECSG_F32 = 0, ECSG_F64 = 1 };
+
+/* Field / parameter selectors for ecsg_get_field / ecsg_set_field.
+ * 0..8 follow sim[][] of main.cpp:86-94 (init_del_sim.cpp:161-173). */
+enum {
+    ECSG_VX = 0, ECSG_VY, ECSG_VZ, ECSG_SXX, ECSG_SYY, ECSG_SZZ, ECSG_SYZ, ECSG_SXZ, ECSG_SXY,
+    ECSG_BUOY = 9,          /* buoyancy at primary nodes */
+    ECSG_C11 = 10           /* 10..18: staged c11 c22 c33 c23 c13 c12 c44 c55 c66 */
+};
+
+typedef struct ecsg_ctx ecsg_ctx;
+
+typedef struct {
+    int num_grids;                      /* main.cpp:66-67 */
+    int n[ECSG_MAX_GRIDS][3];           /* cells per grid (Grid::size, grid.hpp:8) */
+    double h[ECSG_MAX_GRIDS][3];        /* grid steps (Grid::step) */
+    double dt;                          /* time step (main.cpp:51) */
+    int radius;                         /* stencil radius; 1 supported (main.cpp:48) */
+    int top_bc;                         /* 0 rigid, 1 free surface (main.cpp:60) */
+    int imode;                          /* 1 = energy-conserving interface (main.cpp:54) */
+    int check_energy;                   /* main.cpp:45 */
+    int precision;                      /* ECSG_F32 or ECSG_F64 */
+    int max_steps;                      /* capacity of the trace / energy buffers */
+    int vy_mirror_all;                  /* 0: the reference's ghost-velocity loop ranges, in which
+                                           vy is not mirrored across the y walls on plane iz = nz
+                                           (update_vel.cpp:211-218, 239-246); 1: mirror it there too,
+                                           which makes the composite energy exactly conserved */
+} ecsg_config;
+
+/* Creates the device state: wavefields (InitSim, init_del_sim.cpp:3-29),
+ * interface containers (InitGhostContainer, init_del_ghost_container.cpp:4-44),
+ * FD coefficients (InitFDcoeff, init_del_fdcoeff.cpp:7-22).  Fine grid k and
+ * coarse grid k+1 must have sizes in an integer ratio r >= 2 in x and y. */
+int ecsg_create(const ecsg_config *cfg, ecsg_ctx **out);
+void ecsg_destroy(ecsg_ctx *ctx);
+const char *ecsg_last_error(const ecsg_ctx *ctx);
+const char *ecsg_version(void);
+
+/* Parameter staging of grid g (InitParField, init_del_par_field.cpp:77-150 and
+ * ReshapeField, field.cpp:428-476).  buoy and stiff[0..8] are primary-node
+ * arrays of (nx+1)(ny+1)(nz+1) values in the order c11 c22 c33 c23 c13 c12
+ * c44 c55 c66, code convention c44 = 2*C44(Voigt).  iso != 0 selects the
+ * reference's input mode: only stiff[0] (s11) and stiff[4] (s13) are read and
+ * c44 = c55 = c66 = s11 - s13 (init_del_par_field.cpp:119).  Also builds the
+ * time-invariant interface matrices of the interfaces touching g. */
+int ecsg_set_params(ecsg_ctx *ctx, int g, const double *buoy, const double *const stiff[9], int iso);
+
+/* Source (InitSrc, init_src.cpp:7-45, resolved to grid indices by the caller).
+ * type 0: vx += 1000*trace[n] in float (main.cpp:194-195);
+ * type 1: sxx, syy, szz += trace[n] (main.cpp:197-202). */
+int ecsg_set_source(ecsg_ctx *ctx, int g, int ix, int iy, int iz, int type, const float *trace, int len);
+
+/* Receivers (InitRecTrace, init_del_rec_trace.cpp:16-55): loc[4*i..] =
+ * grid, ix, iy, iz; each records vx once per step. */
+int ecsg_set_receivers(ecsg_ctx *ctx, int n, const int *loc);
+
+/* --- one entry point per phase of the time loop, main.cpp:165-261 ------ */
+int ecsg_update_vel(ecsg_ctx *ctx, int g);            /* UpdateVel, update_vel.cpp:4-249; main.cpp:179-180 */
+int ecsg_record_receivers(ecsg_ctx *ctx);             /* main.cpp:183-184 */
+int ecsg_add_source(ecsg_ctx *ctx);                   /* main.cpp:194-195 */
+int ecsg_update_ghost_vz(ecsg_ctx *ctx, int k);       /* UpdateGhostVZ, update_ghost_vz.cpp:124-250; main.cpp:206-209 */
+int ecsg_compute_energy(ecsg_ctx *ctx, double *e);    /* ComputeEnergy, compute_energy.cpp:34-54; main.cpp:217-225 */
+int ecsg_update_stress(ecsg_ctx *ctx, int g);         /* UpdateStress, update_stress.cpp:4-311; main.cpp:228-229 */
+int ecsg_update_ghost_sxz(ecsg_ctx *ctx, int k);      /* UpdateGhostSXZ + Jacobi2D, update_ghost_sxz.cpp:86-350; main.cpp:242-246 */
+int ecsg_update_ghost_syz(ecsg_ctx *ctx, int k);      /* UpdateGhostSYZ, update_ghost_syz.cpp:151-240; main.cpp:247-250 */
+int ecsg_end_step(ecsg_ctx *ctx);                     /* niter++ of main.cpp:165 */
+
+/* Interface residuals (compute_res_*, compute_res.cpp:4-95) with the factor 4
+ * generalised to r^2: which 0 = vx-sxz, 1 = vy-syz, 2 = vz-szz; out has one
+ * value per interface. */
+int ecsg_compute_res(ecsg_ctx *ctx, int which, double *out);
+
+/* Runs nsteps full steps of main.cpp:165-261 on the device (receivers and
+ * energy recorded on the device), captured once into a hipGraph and replayed
+ * when use_graph != 0. */
+int ecsg_step_n(ecsg_ctx *ctx, int nsteps, int use_graph);
+
+int ecsg_steps_done(const ecsg_ctx *ctx);
+int ecsg_synchronize(ecsg_ctx *ctx);
+
+/* Field transfer in the reference's Field layout with halo 1 for wavefields
+ * (len = n+1+2 non-staggered, n+2 staggered, field.cpp:85-98) and no halo for
+ * parameters (sel >= ECSG_BUOY). */
+int ecsg_field_dims(const ecsg_ctx *ctx, int g, int sel, int len[3]);
+int ecsg_get_field(ecsg_ctx *ctx, int g, int sel, double *out);
+int ecsg_set_field(ecsg_ctx *ctx, int g, int sel, const double *in);
+
+/* Recorded data: traces[nrec][steps_done], energy[steps_done]. */
+int ecsg_get_traces(ecsg_ctx *ctx, double *out);
+int ecsg_get_energy(ecsg_ctx *ctx, double *out);
+
+/* Jacobi statistics of interface k: total iterations, iterations of the last
+ * solve, and whether the iteration cap was hit. */
+int ecsg_jacobi_stats(ecsg_ctx *ctx, int k, long long *total, int *last, int *capped);
+
+/* Measurement hook for the benchmark: runs nsteps steps without a graph and
+ * times each kernel launch with HIP events on the handle's stream.
+ * ms_out[kind] receives the average duration of kernel kind over the run and
+ * count_out[kind] the number of launches: kind 0 = velocity stencil of grid 0,
+ * 1 = stress stencil of grid 0, 2 = all velocity stencils, 3 = all stress
+ * stencils, 4 = interface kernels, 5 = receivers/source/energy kernels. */
+int ecsg_profile_kernels(ecsg_ctx *ctx, int nsteps, double ms_out[6], int count_out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ECSG_HIP_H */

@@ -1,0 +1,100 @@
+"""Small synthetic models shared by the CPU and GPU tests."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ecsg_amd.model import Grid, Model, cfl_dt, ricker
+
+
+def random_medium(n, rng, vp=(2000.0, 3000.0), vs_ratio=0.55, rho=(1800.0, 2400.0), pert=0.1):
+    """Orthorhombic medium at primary nodes with random per-node perturbation
+    of each stiffness (float32-representable values)."""
+    nx, ny, nz = n
+    shape = (nz + 1, ny + 1, nx + 1)
+    vpf = rng.uniform(*vp, size=shape)
+    vsf = vpf * vs_ratio
+    rh = rng.uniform(*rho, size=shape)
+    c11 = rh * vpf ** 2
+    c44 = rh * vsf ** 2
+    c12 = c11 - 2 * c44
+    base = [c11, c11, c11, c12, c12, c12, 2 * c44, 2 * c44, 2 * c44]
+    stiff = []
+    for v in base:
+        p = 1.0 + pert * rng.uniform(-1, 1, size=shape)
+        stiff.append((v * p).astype(np.float32).astype(np.float64))
+    buoy = (1.0 / rh).astype(np.float32).astype(np.float64)
+    return buoy, stiff
+
+
+def iso_medium(n, rng, vp=(2000.0, 3000.0), vs_ratio=0.55, rho=(1800.0, 2400.0)):
+    nx, ny, nz = n
+    shape = (nz + 1, ny + 1, nx + 1)
+    vpf = rng.uniform(*vp, size=shape)
+    rh = rng.uniform(*rho, size=shape)
+    s11 = (rh * vpf ** 2).astype(np.float32).astype(np.float64)
+    s13 = (s11 - 2 * rh * (vpf * vs_ratio) ** 2).astype(np.float32).astype(np.float64)
+    return (1.0 / rh).astype(np.float32).astype(np.float64), [s11, s13]
+
+
+def composite(ratios=(2,), coarse=(5, 4, 4), nz_fine=(4,), h0=10.0, nsteps=40, seed=0, iso=False,
+              top_bc=1, src_len=None, fpeak=None, src_type=0, cfl=0.9):
+    """Stack of grids, finest on top: grid k+1 is coarser by ratios[k].
+
+    coarse: cells of the bottom (coarsest) grid; nz_fine: z cells of the finer grids.
+    """
+    rng = np.random.default_rng(seed)
+    ng = len(ratios) + 1
+    cells = [None] * ng
+    steps = [None] * ng
+    cells[-1] = tuple(coarse)
+    hb = h0 * int(np.prod(ratios))
+    steps[-1] = hb
+    for k in range(ng - 2, -1, -1):
+        r = ratios[k]
+        cells[k] = (cells[k + 1][0] * r, cells[k + 1][1] * r, nz_fine[k])
+        steps[k] = steps[k + 1] / r
+    grids = []
+    z0 = 0.0
+    for k in range(ng):
+        if iso:
+            b, st = iso_medium(cells[k], rng)
+        else:
+            b, st = random_medium(cells[k], rng)
+        grids.append(Grid(cells[k], (steps[k],) * 3, b, st, origin_z=z0))
+        z0 += steps[k] * cells[k][2]
+    dt = cfl_dt(grids, iso, frac=cfl)
+    n0 = cells[0]
+    src = (n0[0] // 2, n0[1] // 2, max(1, n0[2] // 2))
+    if fpeak is None:
+        fpeak = 0.25 / (dt * 6)
+    L = src_len if src_len is not None else nsteps
+    tr = ricker(L, dt, fpeak)
+    rec = []
+    for k in range(ng):
+        n = cells[k]
+        rec.append([k, n[0] // 3, n[1] // 2, 1])
+        rec.append([k, (2 * n[0]) // 3, n[1] // 3, n[2] // 2])
+    return Model(grids=grids, dt=dt, nsteps=nsteps, iso=iso, top_bc=top_bc, src_grid=0, src_loc=src,
+                 src_type=src_type, src_trace=tr, rec_loc=np.array(rec, np.int32), name="test_composite")
+
+
+def single(n=(8, 7, 6), h=10.0, nsteps=40, seed=0, iso=False, top_bc=1, src_len=None, src_type=1, cfl=0.9):
+    rng = np.random.default_rng(seed)
+    b, st = iso_medium(n, rng) if iso else random_medium(n, rng)
+    G = Grid(tuple(n), (h, h, h), b, st)
+    dt = cfl_dt([G], iso, frac=cfl)
+    L = src_len if src_len is not None else nsteps
+    tr = ricker(L, dt, 0.25 / (dt * 6))
+    rec = np.array([[0, n[0] // 3, n[1] // 2, 1], [0, n[0] // 2, n[1] // 3, n[2] // 2]], np.int32)
+    return Model(grids=[G], dt=dt, nsteps=nsteps, iso=iso, top_bc=top_bc, src_grid=0,
+                 src_loc=(n[0] // 2, n[1] // 2, n[2] // 2), src_type=src_type, src_trace=tr, rec_loc=rec,
+                 name="test_single")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))

@@ -1,0 +1,61 @@
+"""The oracle against the reference's own golden vectors.
+
+The reference keeps the receiver traces of its composite test flow
+(tests/test1-cmp-eci-ii/eci/SConstruct with par1.py / par3.py: 2 grids at
+ratio 2, 385x385x73 + 193x193x60 nodes, 1172 steps, free surface, vx force
+Ricker) in its SVN pristine store; tests/golden/reference_traces.npz holds
+them, decoded by tests/golden/decode_reference_traces.py.
+
+Over the full 1172 steps the oracle reproduces the ECI traces to 1.44e-5
+(test1) and 1.34e-5 (test2) relative L2, against 1.38e-2 / 7.2e-2 for the
+II-mode traces of the same flows; the full runs take minutes on 8 cores and
+are marked slow.  The default suite checks prefixes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from ecsg_amd.model import reference_flow
+from helpers import rel
+from oracle_lib import Oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = np.load(os.path.join(HERE, "golden", "reference_traces.npz"))
+
+
+def test_golden_file_layout():
+    for t in (1, 2, 3):
+        for mode in ("eci", "ii"):
+            a = GOLD["rec_%s_test%d" % (mode, t)]
+            assert a.shape == (4, 1172) and a.dtype == np.float32
+            assert abs(float(GOLD["rec_%s_test%d_d1" % (mode, t)]) - 0.589378) < 1e-9
+    geo = GOLD["geo_eci_test1"]
+    assert np.allclose(geo[:, 0], [875.0, 958.3333, 1033.3334, 1116.6666], atol=1e-3)
+
+
+@pytest.mark.parametrize("test,steps,tol", [(1, 150, 2e-5), (3, 200, 1e-4)])
+def test_oracle_matches_golden_prefix(test, steps, tol):
+    m = reference_flow(test)
+    o = Oracle(m, eci_mode=0, check_energy=False)
+    o.run(steps)
+    tr = o.traces()
+    g = GOLD["rec_eci_test%d" % test][:, :steps]
+    err = rel(tr, g)
+    assert err <= tol, err
+    o.close()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("test", [1, 2, 3])
+def test_oracle_matches_golden_full(test):
+    m = reference_flow(test)
+    o = Oracle(m, eci_mode=0, check_energy=True)
+    o.run(m.nsteps)
+    tr = o.traces()
+    g = GOLD["rec_eci_test%d" % test]
+    err = rel(tr, g)
+    err_ii = rel(tr, GOLD["rec_ii_test%d" % test])
+    assert err <= 3e-5 and err_ii >= 100 * err
+    e = o.energy()
+    assert abs(e[-1] - e[701]) / e[701] <= 1e-6

@@ -1,0 +1,223 @@
+"""Mathematical pins of the oracle (CPU).
+
+These fix what the golden traces leave open: the ratio-3 interface systems
+(no reference output exists for them), the orthorhombic stiffness path, the
+interface solves themselves, and the discrete energy.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from ecsg_amd.model import Grid, Model, cfl_dt, ricker
+from helpers import composite, rel, single
+from oracle_lib import Oracle, fd_coeff
+
+
+def energy_spread(o, m, src_len):
+    e = o.energy()[src_len + 1:]
+    return (e.max() - e.min()) / abs(e).max()
+
+
+# ---------------------------------------------------------------------------
+# FD coefficients: staggered-grid order conditions fix them uniquely for
+# radius <= 4 (init_del_fdcoeff.cpp:38-85).
+@pytest.mark.parametrize("radius", [1, 2, 3, 4])
+def test_fd_coefficients_order_conditions(radius):
+    c = fd_coeff(radius)
+    # sum_k c_k (2k-1)^(2m+1) = delta_{m0} for m = 0..radius-1
+    for mm in range(radius):
+        s = sum(c[k] * (2 * k + 1) ** (2 * mm + 1) for k in range(radius))
+        assert abs(s - (1.0 if mm == 0 else 0.0)) <= 1e-14
+    # exact rationals from solving the Vandermonde system
+    A = [[Fraction(2 * k + 1) ** (2 * mm + 1) for k in range(radius)] for mm in range(radius)]
+    b = [Fraction(1)] + [Fraction(0)] * (radius - 1)
+    n = radius
+    for i in range(n):  # Gauss-Jordan in exact arithmetic
+        piv = A[i][i]
+        A[i] = [x / piv for x in A[i]]
+        b[i] = b[i] / piv
+        for j in range(n):
+            if j != i and A[j][i] != 0:
+                f = A[j][i]
+                A[j] = [x - f * y for x, y in zip(A[j], A[i])]
+                b[j] = b[j] - f * b[i]
+    assert np.allclose(c, [float(x) for x in b], rtol=1e-15, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# energy
+@pytest.mark.parametrize("top_bc", [0, 1])
+@pytest.mark.parametrize("iso", [False, True])
+def test_single_grid_energy_conserved(top_bc, iso):
+    m = single(top_bc=top_bc, iso=iso, nsteps=240, src_len=50)
+    o = Oracle(m)
+    o.run(m.nsteps)
+    assert energy_spread(o, m, 50) <= 5e-15
+
+
+@pytest.mark.parametrize("ratios,mode", [((2,), 0), ((2,), 1), ((3,), 1), ((2, 3), 1), ((3, 2), 1)])
+def test_composite_energy_conserved_with_full_vy_mirror(ratios, mode):
+    """With vy mirrored across the y walls on every plane, the composite energy
+    (sum of the grids' energies) is conserved to rounding after the source
+    stops -- for the reference's ratio-2 formulas and for the derived ratio-r
+    systems alike."""
+    m = composite(ratios, nz_fine=(4,) * len(ratios), nsteps=240, src_len=50, seed=sum(ratios))
+    o = Oracle(m, eci_mode=mode, vy_mirror_all=True)
+    o.run(m.nsteps)
+    assert energy_spread(o, m, 50) <= 5e-15
+
+
+def test_reference_loop_ranges_leave_a_small_drift():
+    """The reference mirrors vy across the y walls only on z planes
+    [0, nz-1] (update_vel.cpp:211-218, 239-246), so a fine grid's interface
+    plane keeps stale y-halo values of vy; the composite energy then drifts.
+    The oracle reproduces that (default), and the drift disappears with the
+    full mirror (test above)."""
+    m = composite((2,), nz_fine=(4,), nsteps=240, src_len=50, seed=2)
+    o = Oracle(m, eci_mode=0)
+    o.run(m.nsteps)
+    assert energy_spread(o, m, 50) >= 1e-8
+
+
+# ---------------------------------------------------------------------------
+# interface transmission conditions (the defining property of the ECI solves)
+def _interface_planes(o, m, k=0):
+    F, C = m.grids[k], m.grids[k + 1]
+    nz1 = F.n[2]
+    f = {c: o.get(k, c) for c in (0, 1, 5)}
+    c_ = {c: o.get(k + 1, c) for c in (0, 1, 5)}
+    return F, C, nz1, f, c_
+
+
+def _check_transmission(o, m, k=0):
+    F, C, nz1, f, c = _interface_planes(o, m, k)
+    r = F.n[0] // C.n[0]
+    nx2, ny2 = C.n[0], C.n[1]
+    hat = lambda a: (r - abs(a)) / r
+    # vx, node-centred in x and y: r^2 vx2 = sum P(a) P(b) vx1 over the hat support
+    v1 = f[0][nz1 + 1, 1:-1, 1:-1]
+    v2 = c[0][1, 1:-1, 1:-1]
+    worst = 0.0
+    for J in range(1, ny2):
+        for I in range(1, nx2):
+            s = sum(hat(a) * hat(b) * v1[r * J + b, r * I + a] for a in range(-r + 1, r) for b in range(-r + 1, r))
+            worst = max(worst, abs(r * r * v2[J, I] - s))
+    scale = np.abs(v2).max() * r * r
+    assert worst <= 1e-12 * scale, ("vx", worst, scale)
+    # vy, cell-centred: r^2 vy2 = sum of the r x r children
+    w1 = f[1][nz1 + 1, 1:-1, 1:-1]
+    w2 = c[1][1, 1:-1, 1:-1]
+    worst = max(abs(r * r * w2[J, I] - w1[r * J:r * J + r, r * I:r * I + r].sum()) for J in range(ny2) for I in range(nx2))
+    assert worst <= 1e-12 * np.abs(w2).max() * r * r, ("vy", worst)
+    # szz, cell-centred in x, node-centred in y
+    s1 = f[5][nz1 + 1, 1:-1, 1:-1]
+    s2 = c[5][1, 1:-1, 1:-1]
+    worst = 0.0
+    for J in range(1, ny2):
+        for I in range(nx2):
+            s = sum(hat(b) * s1[r * J + b, r * I + a] for a in range(r) for b in range(-r + 1, r))
+            worst = max(worst, abs(r * r * s2[J, I] - s))
+    assert worst <= 1e-12 * np.abs(s2).max() * r * r, ("szz", worst)
+
+
+@pytest.mark.parametrize("ratios,mode", [((2,), 0), ((2,), 1), ((3,), 1), ((4,), 1)])
+def test_transmission_conditions_hold(ratios, mode):
+    """After a step the coarse interface values are the full-weighting
+    restriction of the fine ones: the property the three interface systems
+    are solved for (vx through the sxz Jacobi system, szz through the vz
+    tridiagonal systems, vy through the pointwise syz formula)."""
+    m = composite(ratios, nz_fine=(4,), coarse=(5, 4, 4), nsteps=30, seed=3)
+    o = Oracle(m, eci_mode=mode)
+    o.run(m.nsteps)
+    _check_transmission(o, m)
+
+
+def test_generic_formulas_equal_reference_at_ratio2():
+    m = composite((2,), iso=True, nsteps=60, seed=9)
+    a = Oracle(m, eci_mode=0)
+    b = Oracle(m, eci_mode=1)
+    a.run(m.nsteps)
+    b.run(m.nsteps)
+    assert rel(b.traces(), a.traces()) <= 1e-12
+    for g in range(2):
+        for c in range(9):
+            x, y = a.get(g, c), b.get(g, c)
+            assert np.abs(x - y).max() <= 1e-11 * max(np.abs(x).max(), 1e-300)
+
+
+@pytest.mark.parametrize("r", [2, 3])
+def test_sxz_system_symmetric_dominant_and_solved(r):
+    m = composite((r,), coarse=(6, 5, 4), nz_fine=(4,), nsteps=12, seed=4)
+    o = Oracle(m, eci_mode=1)
+    o.run(m.nsteps)
+    A, B = o.sxz_system(0)
+    n1, n0 = B.shape
+    N = n0 * n1
+    rows, cols, vals = [], [], []
+    for k in range(9):
+        dI, dJ = k // 3 - 1, k % 3 - 1
+        for j in range(n1):
+            for i in range(n0):
+                ii, jj = i + dI, j + dJ
+                if 0 <= ii < n0 and 0 <= jj < n1 and A[k, j, i] != 0:
+                    rows.append(j * n0 + i)
+                    cols.append(jj * n0 + ii)
+                    vals.append(A[k, j, i])
+    M = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    assert abs(M - M.T).max() <= 1e-15 * abs(M).max()
+    d = M.diagonal()
+    off = np.asarray(abs(M).sum(axis=1)).ravel() - np.abs(d)
+    assert np.all(d > off)
+    x_direct = spla.spsolve(M.tocsc(), B.ravel())
+    X = o.jacobi_state(0)[1:-1, 1:-1].ravel()
+    assert np.abs(X - x_direct).max() <= 1e-9 * np.abs(x_direct).max()
+
+
+# ---------------------------------------------------------------------------
+def test_free_surface_szz_stays_zero():
+    m = single(top_bc=1, nsteps=40)
+    o = Oracle(m)
+    o.run(m.nsteps)
+    szz = o.get(0, 5)
+    assert np.all(szz[1] == 0.0)          # plane iz = 0
+    assert np.abs(szz[2]).max() > 0.0
+
+
+def _homogeneous(frac, nsteps):
+    n = (16, 16, 16)
+    shape = tuple(x + 1 for x in n[::-1])
+    rho, vp, vs = 2000.0, 3000.0, 1732.0
+    s11 = np.full(shape, rho * vp * vp)
+    s13 = np.full(shape, rho * vp * vp - 2 * rho * vs * vs)
+    G = Grid(n, (10.0, 10.0, 10.0), np.full(shape, 1 / rho), [s11, s13])
+    dt = frac * 10.0 / (vp * math.sqrt(3))
+    assert abs(cfl_dt([G], True, frac) - dt) <= 1e-15 * dt
+    return Model(grids=[G], dt=dt, nsteps=nsteps, iso=True, top_bc=0, src_grid=0, src_loc=(8, 8, 8), src_type=1,
+                 src_trace=ricker(30, dt, 1 / (12 * dt)), rec_loc=np.array([[0, 4, 4, 4]], np.int32))
+
+
+def test_cfl_bound_h_over_vp_sqrt3():
+    """Energy stays bounded just below h/(vp sqrt 3) and blows up above it."""
+    m = _homogeneous(0.99, 400)
+    o = Oracle(m)
+    o.run(m.nsteps)
+    e = o.energy()[31:]
+    assert (e.max() - e.min()) / abs(e).max() <= 1e-13
+    m = _homogeneous(1.08, 400)
+    o = Oracle(m)
+    o.run(m.nsteps)
+    tr = np.abs(o.traces())
+    assert not np.isfinite(tr[0, -1]) or tr[0, -1] > 1e6 * max(tr[0, :60].max(), 1e-30)
+
+
+def test_fp32_build_tracks_fp64():
+    m = composite((2,), iso=True, nsteps=80, seed=21)
+    a = Oracle(m, prec="f64", eci_mode=0)
+    b = Oracle(m, prec="f32", eci_mode=0)
+    a.run(m.nsteps)
+    b.run(m.nsteps)
+    assert rel(b.traces(), a.traces()) <= 1e-4
