@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the ECSG composite time step on one MI355X.
+
+Workload (default): the three-grid composite -- 1207x1207x81 nodes at
+h = 5 m, 403x403x101 at 15 m, 135x135x121 at 45 m, ratio 3 at both
+interfaces, 136.6 M nodes, layered orthorhombic medium (9 Cij + rho per node),
+FP32 wavefields and parameters, 2000 steps.  A step is one full composite
+time step of main.cpp:165-261 (velocity stencils, receivers, source,
+vz interface solves, stress stencils, sxz Jacobi and syz interface solves),
+with all inputs resident in HBM.
+
+Output: one JSON line.  value = grid-point updates per second (sum over grids
+of nodes x steps / wall time).  roofline: the dominant kernel (stress stencil
+of the 1207x1207x81 grid) timed with HIP events on its own stream over the
+timed region, at 96 algorithmic bytes per node (24 FP32 array accesses).
+cpu_baseline: the CPU restatement (oracle/, OpenMP) on the same workload for a
+bounded number of steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload three_grid]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpt-updates/s fwd & fwd+adjoint, 3-grid FP32 vs plain FP64 GPU; % HBM peak"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_STEP_F32 = 148       # algorithmic bytes per node and step (37 array accesses x 4 B)
+BYTES_STRESS_F32 = 96      # UpdateStress: 24 accesses x 4 B
+BYTES_VEL_F32 = 52         # UpdateVel: 13 accesses x 4 B
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_model(name: str, steps: int):
+    from ecsg_amd import model as M
+    if name == "three_grid":
+        return M.three_grid_workload(nsteps=steps)
+    if name == "two_grid":
+        return M.two_grid_workload(nsteps=steps)
+    if name == "uniform":
+        return M.uniform_model(nsteps=steps)
+    if name == "reference_test1":
+        return M.reference_flow(1)
+    raise SystemExit("unknown workload %s" % name)
+
+
+def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool):
+    import ecsg_amd
+    t0 = time.time()
+    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=warmup + steps + 2)
+    log("[bench] %s state ready in %.1fs" % (precision, time.time() - t0))
+    sim.step(warmup)
+    sim.sync()
+    log("[bench] %s warmup %d steps done" % (precision, warmup))
+    kern = None
+    t = time.perf_counter()
+    if profile:
+        ms, cnt = sim.profile(steps)   # same steps, each kernel bracketed by HIP events
+        kern = (ms, cnt)
+    else:
+        sim.step(steps)
+    sim.sync()
+    el = time.perf_counter() - t
+    stats = [sim.jacobi_stats(k) for k in range(m.num_grids - 1)]
+    tr = sim.traces()
+    sim.close()
+    return el, kern, stats, tr
+
+
+def cpu_baseline(m, steps: int):
+    """The oracle (CPU restatement, OpenMP) on the same model for `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_lib import Oracle, lib
+    L = lib("f32")
+    threads = int(L.orc_num_threads())
+    t0 = time.time()
+    o = Oracle(m, prec="f32", check_energy=False, max_steps=steps + 1)
+    log("[bench] cpu oracle ready in %.1fs (%d threads)" % (time.time() - t0, threads))
+    o.run(1)  # first touch
+    t = time.perf_counter()
+    o.run(steps)
+    el = time.perf_counter() - t
+    o.close()
+    return el, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="three_grid")
+    ap.add_argument("--fp64-steps", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
+              "(see the sharding section of the design notes), no N>1 figure is produced", file=sys.stderr)
+        sys.exit(2)
+
+    total_steps = args.warmup + max(args.steps, args.fp64_steps)
+    t0 = time.time()
+    m = build_model(args.workload, total_steps)
+    nodes = m.total_nodes
+    log("[bench] model %s: %d grids, %.1f M nodes, dt=%.4g, built in %.1fs"
+        % (m.name, m.num_grids, nodes / 1e6, m.dt, time.time() - t0))
+
+    el, kern, stats, _ = gpu_run(m, "f32", args.warmup, args.steps, profile=True)
+    gpts = nodes * args.steps / el / 1e9
+    ms_step = el / args.steps * 1e3
+    log("[bench] f32: %.3f s for %d steps -> %.2f Gpt/s, %.3f ms/step" % (el, args.steps, gpts, ms_step))
+    ms, cnt = kern
+    g0 = m.grids[0]
+    n0 = g0.nodes
+    t_stress = ms[1] * 1e-3
+    t_vel = ms[0] * 1e-3
+    achieved = BYTES_STRESS_F32 * n0 / t_stress
+    vel_bw = BYTES_VEL_F32 * n0 / t_vel
+    log("[bench] kernel avg ms: vel0 %.4f stress0 %.4f (launches %d/%d), interface %.4f (x%d), point %.4f"
+        % (ms[0], ms[1], cnt[0], cnt[1], ms[4], cnt[4], ms[5]))
+
+    fp64 = None
+    if not args.no_fp64 and args.fp64_steps > 0:
+        el64, kern64, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=True)
+        g64 = nodes * args.fp64_steps / el64 / 1e9
+        fp64 = {"value": round(g64, 3), "unit": "Gpt/s", "steps": args.fp64_steps,
+                "ms_per_step": round(el64 / args.fp64_steps * 1e3, 4),
+                "hbm_frac_step": round(2 * BYTES_STEP_F32 * g64 * 1e9 / HBM_PEAK, 4),
+                "stress0_ms": round(kern64[0][1], 5), "vel0_ms": round(kern64[0][0], 5)}
+        log("[bench] f64: %.2f Gpt/s" % g64)
+
+    cpu = None
+    if not args.no_cpu:
+        ce, threads = cpu_baseline(m, args.cpu_steps)
+        cpu = {"value": round(nodes * args.cpu_steps / ce / 1e9, 5), "unit": "Gpt/s", "cores": threads,
+               "kind": "port",
+               "sample": "the same %s model, FP32 restatement (oracle/ecsg_oracle.c, OpenMP), %d steps after one "
+                         "warm-up step, %.1f s wall" % (m.name, args.cpu_steps, ce)}
+        log("[bench] cpu: %s" % cpu)
+
+    out = {
+        "metric": METRIC,
+        "value": round(gpts, 3),
+        "unit": "Gpt/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": m.name, "grids": [list(g.n) for g in m.grids], "steps_h": [g.h[0] for g in m.grids],
+                   "nodes": nodes, "ratio": m.ratios(), "receivers": int(len(m.rec_loc)),
+                   "energy_check": False, "parallelism": "single GPU"},
+        "hbm_frac_step": round(BYTES_STEP_F32 * gpts * 1e9 / HBM_PEAK, 4),
+        "roofline": {"bound": "hbm", "kernel": "k_stress<float> on the %dx%dx%d-node grid" % tuple(x + 1 for x in g0.n),
+                     "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "bytes_per_launch": BYTES_STRESS_F32 * n0, "avg_launch_ms": round(ms[1], 5),
+                     "vel_kernel_GBps": round(vel_bw / 1e9, 1)},
+        "fp64": fp64,
+        "adjoint": None,
+        "jacobi_iters_per_step": [round(s[0] / (args.warmup + args.steps), 2) for s in stats],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -761,7 +761,9 @@ public:
         if (int rc = check_ready()) return rc;
         if (n < 0) return fail(ECSG_EINVAL, "negative step count");
         if (host_step + n > cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded");
-        if (use_graph && !graph_failed && !prof) {
+        // The interface Jacobi is a cooperative launch; its behaviour inside a
+        // captured graph is not relied on, so composite models run eagerly.
+        if (use_graph && !graph_failed && !prof && I.empty()) {
             if (!gexec) {
                 if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                     int rc = enqueue_step();

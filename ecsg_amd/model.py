@@ -370,3 +370,97 @@ def field_shape(n, pos, r):
     """(len2, len1, len0) of a Field with halo r (field.cpp:85-98)."""
     lens = [(n[d] + 2 * r) if pos[d] else (n[d] + 1 + 2 * r) for d in range(3)]
     return (lens[2], lens[1], lens[0])
+
+
+# ---------------------------------------------------------------------------
+# the throughput workloads
+
+def _layered_orthorhombic(sizes, steps_h, vp_fn, rng, pert, vs_over_vp=1 / 1.8):
+    """Per-grid orthorhombic media: layered vp(z), vs = vp*vs_over_vp,
+    rho = 1700 + 0.2 vp, each of the 9 stiffnesses and rho multiplied by an
+    independent seeded uniform factor in [1-pert, 1+pert] per node."""
+    grids = []
+    z0 = 0.0
+    for n, h in zip(sizes, steps_h):
+        nx, ny, nz = n
+        shape = (nz + 1, ny + 1, nx + 1)
+        z = z0 + h * np.arange(nz + 1)
+        vp = vp_fn(z)
+        vs = vp * vs_over_vp
+        rho = 1700.0 + 0.2 * vp
+        c11 = rho * vp * vp
+        c44 = rho * vs * vs
+        c12 = c11 - 2 * c44
+        base = [c11, c11, c11, c12, c12, c12, 2 * c44, 2 * c44, 2 * c44]
+
+        def noisy(prof):
+            f = rng.random(shape, dtype=np.float32)
+            f *= np.float32(2 * pert)
+            f += np.float32(1 - pert)
+            f *= prof.astype(np.float32)[:, None, None]
+            return f.astype(np.float64)
+
+        stiff = [noisy(b) for b in base]
+        rh = noisy(rho)
+        buoy = (1.0 / rh).astype(np.float32).astype(np.float64)
+        del rh
+        grids.append(Grid(tuple(n), (h, h, h), buoy, stiff, origin_z=z0))
+        z0 += h * nz
+    return grids
+
+
+def three_grid_workload(seed: int = 0, nsteps: int = 2000, nrec=(40, 30), fpeak: float = 15.0) -> Model:
+    """Three-grid composite throughput workload: 1207x1207x81 nodes at h=5 m,
+    403x403x101 at 15 m, 135x135x121 at 45 m (ratio 3 at each interface; the
+    sizes that make the three grids align, 136.6 M nodes), layered
+    orthorhombic medium with vp 1000 -> 5000 m/s linear in depth and +-10%
+    seeded per-node perturbation of the 9 Cij and rho, explosive Ricker at
+    50 m depth, nrec[0] x nrec[1] receivers on the surface grid."""
+    sizes = [(1206, 1206, 80), (402, 402, 100), (134, 134, 120)]
+    hs = [5.0, 15.0, 45.0]
+    ztot = sum(n[2] * h for n, h in zip(sizes, hs))
+    rng = np.random.default_rng(seed)
+    grids = _layered_orthorhombic(sizes, hs, lambda z: 1000.0 + 4000.0 * z / ztot, rng, 0.10)
+    dt = cfl_dt(grids, iso=False, frac=0.9)
+    g0 = grids[0]
+    src = map_location(grids, g0.n[0] // 2 * g0.h[0], g0.n[1] // 2 * g0.h[1], 50.0)
+    xs = np.linspace(0, g0.n[0], nrec[0] + 2)[1:-1].round().astype(int)
+    ys = np.linspace(0, g0.n[1], nrec[1] + 2)[1:-1].round().astype(int)
+    rec = np.array([[0, x, y, 1] for y in ys for x in xs], np.int32)
+    return Model(grids=grids, dt=dt, nsteps=nsteps, iso=False, top_bc=1, src_grid=src[0], src_loc=src[1:],
+                 src_type=1, src_trace=ricker(min(nsteps, 400), dt, fpeak), rec_loc=rec,
+                 name="three_grid_1207x1207x81_403x403x101_135x135x121")
+
+
+def two_grid_workload(seed: int = 0, nsteps: int = 1500) -> Model:
+    """Two-grid composite at ratio 3: 241x241x61 nodes at h=5 m over a
+    low-velocity top layer (vp 1200, vs 600, rho 1800) and 81x81x121 at
+    h=15 m below (vp 3500, vs 2000, rho 2300), orthorhombic stiffnesses with
+    +-5% seeded Gaussian-smoothed perturbation (correlation 50 m), Ricker
+    20 Hz at 30 m depth."""
+    rng = np.random.default_rng(seed)
+    sizes = [(240, 240, 60), (80, 80, 120)]
+    hs = [5.0, 15.0]
+    props = [(1200.0, 600.0, 1800.0), (3500.0, 2000.0, 2300.0)]
+    grids = []
+    z0 = 0.0
+    for n, h, (vp, vs, rho) in zip(sizes, hs, props):
+        nx, ny, nz = n
+        shape = (nz + 1, ny + 1, nx + 1)
+        c11 = rho * vp * vp
+        c44 = rho * vs * vs
+        c12 = c11 - 2 * c44
+        base = [c11, c11, c11, c12, c12, c12, 2 * c44, 2 * c44, 2 * c44]
+        corr = 50.0 / h
+        stiff = [(v * (1 + 0.05 * _smooth_noise(shape, corr, rng))).astype(np.float32).astype(np.float64) for v in base]
+        buoy = np.full(shape, np.float32(1.0 / rho), np.float64)
+        grids.append(Grid(tuple(n), (h, h, h), buoy, stiff, origin_z=z0))
+        z0 += h * nz
+    dt = cfl_dt(grids, iso=False, frac=0.9)
+    g0 = grids[0]
+    src = map_location(grids, g0.n[0] // 2 * h, g0.n[1] // 2 * h, 30.0)
+    src = map_location(grids, g0.n[0] // 2 * g0.h[0], g0.n[1] // 2 * g0.h[1], 30.0)
+    rec = np.array([[0, x, g0.n[1] // 2, 1] for x in range(0, g0.n[0] + 1, 4)], np.int32)
+    return Model(grids=grids, dt=dt, nsteps=nsteps, iso=False, top_bc=1, src_grid=src[0], src_loc=src[1:],
+                 src_type=0, src_trace=ricker(min(nsteps, 300), dt, 20.0), rec_loc=rec,
+                 name="two_grid_241x241x61_81x81x121")
