@@ -55,24 +55,25 @@ def build_model(name: str, steps: int):
     raise SystemExit("unknown workload %s" % name)
 
 
-def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool):
+def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_steps: int = 50):
     import ecsg_amd
     t0 = time.time()
-    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=warmup + steps + 2)
+    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=warmup + steps + prof_steps + 2)
     log("[bench] %s state ready in %.1fs" % (precision, time.time() - t0))
     sim.step(warmup)
     sim.sync()
     log("[bench] %s warmup %d steps done" % (precision, warmup))
-    kern = None
     t = time.perf_counter()
-    if profile:
-        ms, cnt = sim.profile(steps)   # same steps, each kernel bracketed by HIP events
-        kern = (ms, cnt)
-    else:
-        sim.step(steps)
+    sim.step(steps)
     sim.sync()
     el = time.perf_counter() - t
     stats = [sim.jacobi_stats(k) for k in range(m.num_grids - 1)]
+    kern = None
+    if profile:
+        # per-kernel durations: further steps with each launch bracketed by
+        # HIP events on the simulation's stream
+        kern = sim.profile(prof_steps)
+        sim.sync()
     tr = sim.traces()
     sim.close()
     return el, kern, stats, tr
@@ -95,6 +96,19 @@ def cpu_baseline(m, steps: int):
     return el, threads
 
 
+def copy_calibration():
+    """A plain 1 GiB device-to-device copy (torch) so that FETCH_SIZE /
+    WRITE_SIZE of a kernel with known traffic can scale the stencil counters."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,41 +119,50 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=50)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (scale factor for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
               "(see the sharding section of the design notes), no N>1 figure is produced", file=sys.stderr)
         sys.exit(2)
 
-    total_steps = args.warmup + max(args.steps, args.fp64_steps)
+    total_steps = args.warmup + max(args.steps, args.fp64_steps) + 60
     t0 = time.time()
     m = build_model(args.workload, total_steps)
     nodes = m.total_nodes
     log("[bench] model %s: %d grids, %.1f M nodes, dt=%.4g, built in %.1fs"
         % (m.name, m.num_grids, nodes / 1e6, m.dt, time.time() - t0))
 
-    el, kern, stats, _ = gpu_run(m, "f32", args.warmup, args.steps, profile=True)
+    el, kern, stats, _ = gpu_run(m, "f32", args.warmup, args.steps, profile=args.prof_steps > 0,
+                                 prof_steps=args.prof_steps)
     gpts = nodes * args.steps / el / 1e9
     ms_step = el / args.steps * 1e3
     log("[bench] f32: %.3f s for %d steps -> %.2f Gpt/s, %.3f ms/step" % (el, args.steps, gpts, ms_step))
-    ms, cnt = kern
+    ms, cnt = kern if kern else ([0.0] * 6, [0] * 6)
     g0 = m.grids[0]
     n0 = g0.nodes
     t_stress = ms[1] * 1e-3
     t_vel = ms[0] * 1e-3
-    achieved = BYTES_STRESS_F32 * n0 / t_stress
-    vel_bw = BYTES_VEL_F32 * n0 / t_vel
+    achieved = BYTES_STRESS_F32 * n0 / t_stress if t_stress > 0 else 0.0
+    vel_bw = BYTES_VEL_F32 * n0 / t_vel if t_vel > 0 else 0.0
     log("[bench] kernel avg ms: vel0 %.4f stress0 %.4f (launches %d/%d), interface %.4f (x%d), point %.4f"
         % (ms[0], ms[1], cnt[0], cnt[1], ms[4], cnt[4], ms[5]))
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
-        el64, kern64, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=True)
+        el64, kern64, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=args.prof_steps > 0,
+                                     prof_steps=args.prof_steps)
         g64 = nodes * args.fp64_steps / el64 / 1e9
         fp64 = {"value": round(g64, 3), "unit": "Gpt/s", "steps": args.fp64_steps,
                 "ms_per_step": round(el64 / args.fp64_steps * 1e3, 4),
                 "hbm_frac_step": round(2 * BYTES_STEP_F32 * g64 * 1e9 / HBM_PEAK, 4),
-                "stress0_ms": round(kern64[0][1], 5), "vel0_ms": round(kern64[0][0], 5)}
+                "stress0_ms": round(kern64[0][1], 5) if kern64 else None,
+                "vel0_ms": round(kern64[0][0], 5) if kern64 else None}
         log("[bench] f64: %.2f Gpt/s" % g64)
 
     cpu = None

@@ -477,21 +477,27 @@ public:
     }
 
     // ----- launches --------------------------------------------------------
+    // events come from a pool created before the measured steps, so the
+    // instrumented launches cost one hipEventRecord each
+    size_t ev_next = 0;
+    hipEvent_t next_event()
+    {
+        if (ev_next < evs.size()) return evs[ev_next++];
+        hipEvent_t e;
+        hipEventCreate(&e);
+        evs.push_back(e);
+        ev_next++;
+        return e;
+    }
     void ev_begin()
     {
         if (!prof) return;
-        hipEvent_t e;
-        hipEventCreate(&e);
-        hipEventRecord(e, s);
-        evs.push_back(e);
+        hipEventRecord(next_event(), s);
     }
     void ev_end(int kind)
     {
         if (!prof) return;
-        hipEvent_t e;
-        hipEventCreate(&e);
-        hipEventRecord(e, s);
-        evs.push_back(e);
+        hipEventRecord(next_event(), s);
         ev_kind.push_back(kind);
     }
 
@@ -897,6 +903,15 @@ public:
         if (int rc = check_ready()) return rc;
         if (host_step + nsteps > cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded");
         HIPCHK(hipStreamSynchronize(s));
+        // pre-create the events: launches per step times two
+        size_t per_step = 2 * (2 * G.size() + 2 + 3 * I.size() + (cfg.check_energy ? 1 : 0));
+        while (evs.size() < per_step * (size_t)nsteps) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            evs.push_back(e);
+        }
+        ev_next = 0;
+        ev_kind.clear();
         prof = true;
         for (int i = 0; i < nsteps; i++)
             if (int rc = enqueue_step()) { prof = false; return rc; }
@@ -913,9 +928,8 @@ public:
             if (kind == 1) { ms[3] += t; cnt[3]++; }
         }
         for (int k = 0; k < 6; k++) ms[k] = cnt[k] ? ms[k] / cnt[k] : 0;
-        for (auto e : evs) hipEventDestroy(e);
-        evs.clear();
         ev_kind.clear();
+        ev_next = 0;
         return ECSG_OK;
     }
 };
