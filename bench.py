@@ -55,10 +55,11 @@ def build_model(name: str, steps: int):
     raise SystemExit("unknown workload %s" % name)
 
 
-def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_steps: int = 50):
+def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_steps: int = 50, solver: str = "jacobi"):
     import ecsg_amd
     t0 = time.time()
-    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=warmup + steps + prof_steps + 2)
+    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=warmup + steps + prof_steps + 2,
+                              sxz_solver=solver)
     log("[bench] %s state ready in %.1fs" % (precision, time.time() - t0))
     sim.step(warmup)
     sim.sync()
@@ -120,6 +121,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=50)
+    ap.add_argument("--solver", default="jacobi", choices=["jacobi", "chebyshev"],
+                    help="vx-sxz interface solver (jacobi = the reference's sweeps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (scale factor for HBM byte counters)")
     args = ap.parse_args()
@@ -139,7 +142,7 @@ def main():
         % (m.name, m.num_grids, nodes / 1e6, m.dt, time.time() - t0))
 
     el, kern, stats, _ = gpu_run(m, "f32", args.warmup, args.steps, profile=args.prof_steps > 0,
-                                 prof_steps=args.prof_steps)
+                                 prof_steps=args.prof_steps, solver=args.solver)
     gpts = nodes * args.steps / el / 1e9
     ms_step = el / args.steps * 1e3
     log("[bench] f32: %.3f s for %d steps -> %.2f Gpt/s, %.3f ms/step" % (el, args.steps, gpts, ms_step))
@@ -156,7 +159,7 @@ def main():
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
         el64, kern64, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=args.prof_steps > 0,
-                                     prof_steps=args.prof_steps)
+                                     prof_steps=args.prof_steps, solver=args.solver)
         g64 = nodes * args.fp64_steps / el64 / 1e9
         fp64 = {"value": round(g64, 3), "unit": "Gpt/s", "steps": args.fp64_steps,
                 "ms_per_step": round(el64 / args.fp64_steps * 1e3, 4),
@@ -189,7 +192,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": m.name, "grids": [list(g.n) for g in m.grids], "steps_h": [g.h[0] for g in m.grids],
                    "nodes": nodes, "ratio": m.ratios(), "receivers": int(len(m.rec_loc)),
-                   "energy_check": False, "parallelism": "single GPU"},
+                   "energy_check": False, "sxz_solver": args.solver, "parallelism": "single GPU"},
         "hbm_frac_step": round(BYTES_STEP_F32 * gpts * 1e9 / HBM_PEAK, 4),
         "roofline": {"bound": "hbm", "kernel": "k_stress<float> on the %dx%dx%d-node grid" % tuple(x + 1 for x in g0.n),
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -46,6 +46,7 @@ class EcsgConfig(C.Structure):
         ("precision", C.c_int),
         ("max_steps", C.c_int),
         ("vy_mirror_all", C.c_int),
+        ("sxz_solver", C.c_int),
     ]
 
 
@@ -107,7 +108,7 @@ class Simulation:
     """
 
     def __init__(self, m, precision: str = "f64", check_energy: bool = True, max_steps: Optional[int] = None,
-                 vy_mirror_all: bool = False):
+                 vy_mirror_all: bool = False, sxz_solver: str = "jacobi"):
         self.L = load_library()
         self.model = m
         cfg = EcsgConfig()
@@ -124,6 +125,7 @@ class Simulation:
         cfg.precision = {"f32": F32, "f64": F64}[precision]
         cfg.max_steps = int(m.nsteps if max_steps is None else max_steps)
         cfg.vy_mirror_all = int(vy_mirror_all)
+        cfg.sxz_solver = {"jacobi": 0, "chebyshev": 1}[sxz_solver]
         self.max_steps = cfg.max_steps
         self.precision = precision
         self.h = C.c_void_p()

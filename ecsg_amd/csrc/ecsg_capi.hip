@@ -98,6 +98,7 @@ struct DevIface {
     double *X0 = nullptr, *X1 = nullptr, *jpart = nullptr;
     long long *stats = nullptr;
     int nabs = 0, coop = 0;
+    double rho = 0;  // Gershgorin bound of the Jacobi iteration matrix
     bool ready = false;
 };
 
@@ -119,6 +120,7 @@ public:
     int src_g = -1, src_loc[3] = {0, 0, 0}, src_type = 0, src_len = 0;
     int host_step = 0;
     double tol_cap = 0, tol_fac = 0;
+    int max_coop = 0;  // co-resident Jacobi workgroups
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     bool graph_failed = false;
@@ -233,13 +235,23 @@ public:
             int dev = 0, ncu = 0, per = 0;
             HIPCHK(hipGetDevice(&dev));
             HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_jacobi<T>, 256, 0));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_jacobi<T>, JT, 0));
             if (per < 1) return fail(ECSG_EHIP, "cooperative Jacobi kernel cannot be resident");
-            const long want = (nn + 255) / 256;
-            f.coop = (int)std::max(1L, std::min<long>(want, (long)per * ncu));
-            if (int rc = dalloc(&f.jpart, 4 * (size_t)f.coop)) return rc;
+            // every unknown must be owned by one thread for the whole solve
+            const long want = (nn + (long)JT * JPPT - 1) / ((long)JT * JPPT);
+            max_coop = per * ncu;
+            if (want > (long)max_coop)
+                return fail(ECSG_EINVAL, "interface " + std::to_string(k) + " too large for the resident Jacobi solve");
+            f.coop = (int)std::max(1L, want);
+            if (int rc = dalloc(&f.jpart, 4 * (size_t)f.coop + 1)) return rc;
             if (int rc = dalloc(&f.stats, 3)) return rc;
         }
+        for (int k0 = 0; k0 < (int)I.size(); k0 += MAXI) {
+            int tot = 0;
+            for (int k = k0; k < std::min((int)I.size(), k0 + MAXI); k++) tot += I[k].coop;
+            if (tot > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+        }
+        if (c.sxz_solver != 0 && c.sxz_solver != 1) return fail(ECSG_EINVAL, "sxz_solver must be 0 (Jacobi) or 1 (Chebyshev)");
         if (int rc = dalloc(&d_step, 1)) return rc;
         if (int rc = dalloc(&d_energy, cfg.max_steps)) return rc;
         if (int rc = dalloc(&d_fix, 1)) return rc;
@@ -428,6 +440,16 @@ public:
                 A[(size_t)4 * nn + q] += r4 * C.b_top[(long)J * pxC + Ii] * k2z;
             }
         HIPCHK(hipMemcpy(f.A, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice));
+        // Gershgorin bound of the Jacobi iteration matrix D^-1 (A - D): the
+        // Chebyshev recurrence needs an upper bound of its spectral radius
+        double rho = 0;
+        for (long q = 0; q < nn; q++) {
+            double off = 0;
+            for (int m = 0; m < 9; m++)
+                if (m != 4) off += std::fabs(A[(size_t)m * nn + q]);
+            rho = std::max(rho, off / A[(size_t)4 * nn + q]);
+        }
+        f.rho = rho;
         f.ready = true;
         return ECSG_OK;
     }
@@ -555,21 +577,53 @@ public:
         return ECSG_OK;
     }
 
-    int launch_ghost_sxz(int k)
+    // right-hand side of the vx-sxz system of interface k
+    int launch_sxz_rhs(int k)
     {
         DevIface &f = I[k];
         IfaceDev<T> a = iface_dev(k);
-        const long nn = (long)(f.nx2 - 1) * (f.ny2 - 1);
-        ev_begin();
         hipLaunchKernelGGL((k_gsxz_rhs<T>), dim3(f.nabs), dim3(256), 0, s, a, f.Bs, f.absp);
         HIPCHK(hipGetLastError());
-        JacobiArgs j{};
-        j.A = f.A; j.B = f.Bs; j.X0 = f.X0; j.X1 = f.X1; j.abs_part = f.absp; j.nabs = f.nabs;
-        j.part = f.jpart; j.stats = f.stats; j.tol_cap = tol_cap; j.tol_fac = tol_fac; j.cap = JACOBI_CAP;
-        j.floor = 1024 * 2.220446049250313e-16;
-        (void)nn;
-        void *args[] = {(void *)&a, (void *)&j};
-        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(f.coop), dim3(256), args, 0, s));
+        return ECSG_OK;
+    }
+
+    // Jacobi solves + ghost scatter of interfaces [k0, k0+n) in one launch
+    int launch_jacobi(int k0, int n)
+    {
+        JacBatch<T> jb{};
+        int blk = 0;
+        for (int q = 0; q < n; q++) {
+            DevIface &f = I[k0 + q];
+            JacItf<T> &J = jb.it[q];
+            J.A = f.A; J.B = f.Bs; J.X0 = f.X0; J.X1 = f.X1; J.abs_part = f.absp; J.nabs = f.nabs;
+            J.part = f.jpart; J.stats = f.stats;
+            J.sxz1 = G[k0 + q].dev.f[7]; J.sxz2 = G[k0 + q + 1].dev.f[7];
+            J.o1 = G[k0 + q].o; J.P1 = G[k0 + q].P; J.S21 = G[k0 + q].S2;
+            J.o2 = G[k0 + q + 1].o; J.P2 = G[k0 + q + 1].P; J.S22 = G[k0 + q + 1].S2;
+            J.nx2 = f.nx2; J.ny2 = f.ny2; J.nz1 = f.nz1; J.r = f.r;
+            J.blk0 = blk; J.nblk = f.coop; J.rho2 = f.rho * f.rho;
+            blk += f.coop;
+        }
+        jb.n = n;
+        jb.tol_cap = tol_cap; jb.tol_fac = tol_fac;
+        jb.floor = 1024 * 2.220446049250313e-16;
+        jb.cap = JACOBI_CAP;
+        // the Chebyshev weights need a spectral bound below one
+        bool cheb_ok = cfg.sxz_solver == 1;
+        for (int q = 0; q < n; q++)
+            if (!(I[k0 + q].rho < 1.0)) cheb_ok = false;
+        jb.cheby = cheb_ok ? 1 : 0;
+        if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+        void *args[] = {(void *)&jb};
+        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, 0, s));
+        return ECSG_OK;
+    }
+
+    int launch_ghost_sxz(int k)
+    {
+        ev_begin();
+        if (int rc = launch_sxz_rhs(k)) return rc;
+        if (int rc = launch_jacobi(k, 1)) return rc;
         ev_end(4);
         return ECSG_OK;
     }
@@ -647,10 +701,18 @@ public:
             if (int rc = launch_stress(g)) return rc;
         if (cfg.check_energy)
             if (int rc = launch_energy()) return rc;
-        for (int k = 0; k + 1 < ng; k++) {
-            if (int rc = launch_ghost_sxz(k)) return rc;
-            if (int rc = launch_ghost_syz(k)) return rc;
+        // UpdateGhostSXZ of every interface (main.cpp:242-246): right-hand
+        // sides, then the Jacobi solves of up to MAXI interfaces side by side
+        if (ng > 1) {
+            ev_begin();
+            for (int k = 0; k + 1 < ng; k++)
+                if (int rc = launch_sxz_rhs(k)) return rc;
+            for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
+                if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
+            ev_end(4);
         }
+        for (int k = 0; k + 1 < ng; k++)
+            if (int rc = launch_ghost_syz(k)) return rc;
         return ECSG_OK;
     }
 

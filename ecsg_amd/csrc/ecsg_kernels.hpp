@@ -555,104 +555,208 @@ __device__ __forceinline__ double wave_sum_fixed(const double *v, int n)
     return s;
 }
 
-// Jacobi2D + Norm2D (update_ghost_sxz.cpp:314-388) as one cooperative
-// persistent kernel: every iteration ends with a grid barrier, after which
-// every workgroup sums the per-workgroup norms in the same order, so all of
-// them take the same exit decision.  Then the scatter of the coarse ghosts
-// and the fine ghosts (update_ghost_sxz.cpp:247-295, generalised to ratio r).
+constexpr int JT = 1024;   // threads per Jacobi workgroup
+constexpr int JPPT = 2;    // unknowns owned per thread
+constexpr int MAXI = 4;    // interfaces solved by one launch
+
+// One interface of a batched Jacobi solve.
 template <typename T>
-__global__ __launch_bounds__(256) void k_jacobi(IfaceDev<T> a, JacobiArgs j)
+struct JacItf {
+    const double *A;        // 9 bands, band k at A + k*nn, [J-1][I-1]
+    const double *B;
+    double *X0, *X1;        // iterates (nx2+1)(ny2+1), zero border; X0 holds the warm start
+    const double *abs_part; // per-workgroup sums of |B|
+    int nabs;
+    double *part;           // 2 parities x 2 sums x nblk
+    long long *stats;       // total, last, capped
+    T *sxz1, *sxz2;
+    long o1, P1, S21, o2, P2, S22;
+    int nx2, ny2, nz1, r;
+    int blk0, nblk;         // workgroups [blk0, blk0+nblk) work on this interface
+    double rho2;            // square of the Jacobi spectral bound (Chebyshev)
+};
+
+template <typename T>
+struct JacBatch {
+    JacItf<T> it[MAXI];
+    int n;
+    double tol_cap, tol_fac;  // tol = min(tol_cap, tol_fac * AvgAbs(B))
+    double floor;             // rounding-floor factor of the stagnation exit
+    int cap;
+    int cheby;                // 1: Chebyshev acceleration of the Jacobi sweeps
+};
+
+// Jacobi2D + Norm2D (update_ghost_sxz.cpp:314-388) for all interfaces of a
+// composite model in one cooperative persistent kernel: the iterations of the
+// interfaces run side by side.  Each thread owns up to JPPT unknowns for the
+// whole solve and keeps their nine matrix bands and right-hand side in
+// registers, so a sweep only reads the (L2-resident) previous iterate.  Every
+// iteration ends with a grid barrier, after which every workgroup sums the
+// per-workgroup norms of every interface in the same order, so all take the
+// same decisions.  With cheby = 1 the sweeps are combined by the Chebyshev
+// semi-iterative recurrence (same fixed point and exit test).  Then the
+// scatter of the coarse ghosts and the fine ghosts (update_ghost_sxz.cpp:
+// 247-295, generalised to ratio r).
+template <typename T>
+__global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
 {
     namespace cg = cooperative_groups;
     cg::grid_group grid = cg::this_grid();
-    __shared__ double lds[4];
-    __shared__ double bcast;
-    const int nx2 = a.nx2, ny2 = a.ny2;
+    __shared__ double lds[JT / 64];
+    __shared__ double lds2[JT / 64];
+    __shared__ double s_tol[MAXI], s_n2[MAXI], s_x2[MAXI];
+    int me = 0;
+    for (int k = 1; k < jb.n; k++)
+        if ((int)blockIdx.x >= jb.it[k].blk0) me = k;
+    const JacItf<T> &J = jb.it[me];
+    const int nx2 = J.nx2, ny2 = J.ny2;
     const int n0 = nx2 - 1, n1 = ny2 - 1;
     const long nn = (long)n0 * n1;
     const long w = nx2 + 1;
-    const long gtid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long gstride = (long)gridDim.x * blockDim.x;
-    if (threadIdx.x < 64) {
-        const double sb = wave_sum_fixed(j.abs_part, j.nabs);
-        if (threadIdx.x == 0) {
-            const double avg = sb / (double)nn;
-            const double t2 = avg * j.tol_fac;
-            bcast = (j.tol_cap < t2) ? j.tol_cap : t2;
+    const int lb = blockIdx.x - J.blk0;
+    const long gtid = (long)lb * blockDim.x + threadIdx.x;
+    const long gstride = (long)J.nblk * blockDim.x;
+    const int nw = blockDim.x >> 6;
+    if (threadIdx.x < 64)
+        for (int k = 0; k < jb.n; k++) {
+            const JacItf<T> &K = jb.it[k];
+            const double sb = wave_sum_fixed(K.abs_part, K.nabs);
+            if (threadIdx.x == 0) {
+                const double avg = sb / (double)((long)(K.nx2 - 1) * (K.ny2 - 1));
+                const double t2 = avg * jb.tol_fac;
+                s_tol[k] = (jb.tol_cap < t2) ? jb.tol_cap : t2;
+            }
         }
-    }
     __syncthreads();
-    const double tol = bcast;
-    const double *A = j.A;
-    double *Xo = j.X0, *Xn = j.X1;
-    int it = 0;
-    double prev = 0;
-    __shared__ double lds2[4];
-    __shared__ double bcast2;
-    while (true) {
-        double acc = 0, ax = 0;
-        for (long q = gtid; q < nn; q += gstride) {
-            const long i0 = q % n0 + 1, i1 = q / n0 + 1;
-            const double xn = (j.B[q]
-                               - A[0 * nn + q] * Xo[(i1 - 1) * w + (i0 - 1)]
-                               - A[1 * nn + q] * Xo[(i1) * w + (i0 - 1)]
-                               - A[2 * nn + q] * Xo[(i1 + 1) * w + (i0 - 1)]
-                               - A[3 * nn + q] * Xo[(i1 - 1) * w + (i0)]
-                               - A[5 * nn + q] * Xo[(i1 + 1) * w + (i0)]
-                               - A[6 * nn + q] * Xo[(i1 - 1) * w + (i0 + 1)]
-                               - A[7 * nn + q] * Xo[(i1) * w + (i0 + 1)]
-                               - A[8 * nn + q] * Xo[(i1 + 1) * w + (i0 + 1)]) / A[4 * nn + q];
-            const double d = xn - Xo[i1 * w + i0];
-            acc += d * d;
-            ax += xn * xn;
-            Xn[i1 * w + i0] = xn;
-        }
-        double s = acc, sx = ax;
+    double Ar[JPPT][9], Br[JPPT], xprev[JPPT];
+    long xi[JPPT];
+    bool own[JPPT];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            s += __shfl_xor(s, off, 64);
-            sx += __shfl_xor(sx, off, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            lds[threadIdx.x >> 6] = s;
-            lds2[threadIdx.x >> 6] = sx;
-        }
-        __syncthreads();
-        double *part = j.part + (it & 1) * 2 * gridDim.x;
-        if (threadIdx.x == 0) {
-            part[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-            part[gridDim.x + blockIdx.x] = ((lds2[0] + lds2[1]) + lds2[2]) + lds2[3];
+    for (int k = 0; k < JPPT; k++) {
+        const long q = gtid + k * gstride;
+        own[k] = q < nn;
+        const long qq = own[k] ? q : 0;
+        const long i0 = qq % n0 + 1, i1 = qq / n0 + 1;
+        xi[k] = i1 * w + i0;
+#pragma unroll
+        for (int b = 0; b < 9; b++) Ar[k][b] = J.A[b * nn + qq];
+        Br[k] = J.B[qq];
+        xprev[k] = J.X0[xi[k]];
+    }
+    double *Xo = J.X0, *Xn = J.X1;
+    // per-interface iteration state, replicated identically in every workgroup
+    int it_k[MAXI];
+    bool done_k[MAXI];
+    double prev_k[MAXI];
+#pragma unroll
+    for (int k = 0; k < MAXI; k++) { it_k[k] = 0; done_k[k] = (k >= jb.n); prev_k[k] = 0; }
+    double omega = 1.0;
+    int my_it = 0;
+    while (true) {
+        bool active = false;
+#pragma unroll
+        for (int k = 0; k < MAXI; k++)
+            if (k == me) { active = !done_k[k]; my_it = it_k[k]; }
+        if (active) {
+            // Chebyshev weights: omega_1 = 1, omega_2 = 2/(2-rho^2),
+            // omega_{k+1} = 1/(1 - rho^2 omega_k / 4)
+            if (jb.cheby) omega = (my_it == 0) ? 1.0 : (my_it == 1 ? 2.0 / (2.0 - J.rho2) : 1.0 / (1.0 - 0.25 * J.rho2 * omega));
+            double acc = 0, ax = 0;
+#pragma unroll
+            for (int k = 0; k < JPPT; k++) {
+                if (!own[k]) continue;
+                const long c = xi[k];
+                const double xo = Xo[c];
+                double xn = (Br[k]
+                             - Ar[k][0] * Xo[c - w - 1]
+                             - Ar[k][1] * Xo[c - 1]
+                             - Ar[k][2] * Xo[c + w - 1]
+                             - Ar[k][3] * Xo[c - w]
+                             - Ar[k][5] * Xo[c + w]
+                             - Ar[k][6] * Xo[c - w + 1]
+                             - Ar[k][7] * Xo[c + 1]
+                             - Ar[k][8] * Xo[c + w + 1]) / Ar[k][4];
+                if (jb.cheby) {
+                    xn = omega * (xn - xprev[k]) + xprev[k];
+                    xprev[k] = xo;
+                }
+                const double d = xn - xo;
+                acc += d * d;
+                ax += xn * xn;
+                Xn[c] = xn;
+            }
+            double s = acc, sx = ax;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                s += __shfl_xor(s, off, 64);
+                sx += __shfl_xor(sx, off, 64);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                lds[threadIdx.x >> 6] = s;
+                lds2[threadIdx.x >> 6] = sx;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = 0, tx = 0;
+                for (int q = 0; q < nw; q++) { t += lds[q]; tx += lds2[q]; }
+                double *part = J.part + (my_it & 1) * 2 * J.nblk;
+                part[lb] = t;
+                part[J.nblk + lb] = tx;
+            }
+            double *tsw = Xo; Xo = Xn; Xn = tsw;
         }
         grid.sync();
-        if (threadIdx.x < 64) {
-            const double n2 = wave_sum_fixed(part, gridDim.x);
-            const double x2 = wave_sum_fixed(part + gridDim.x, gridDim.x);
-            if (threadIdx.x == 0) { bcast = n2; bcast2 = x2; }
-        }
+        // every workgroup evaluates every active interface's exit test from
+        // the same partial sums in the same order
+        if (threadIdx.x < 64)
+            for (int k = 0; k < jb.n; k++) {
+                bool dk = false;
+                int ik = 0;
+#pragma unroll
+                for (int kk = 0; kk < MAXI; kk++)
+                    if (kk == k) { dk = done_k[kk]; ik = it_k[kk]; }
+                if (dk) continue;
+                const JacItf<T> &K = jb.it[k];
+                const double *part = K.part + (ik & 1) * 2 * K.nblk;
+                const double n2 = wave_sum_fixed(part, K.nblk);
+                const double x2 = wave_sum_fixed(part + K.nblk, K.nblk);
+                if (threadIdx.x == 0) { s_n2[k] = n2; s_x2[k] = x2; }
+            }
         __syncthreads();
-        const double norm = sqrt(bcast);
-        const double xnorm = sqrt(bcast2);
-        it++;
-        double *tsw = Xo; Xo = Xn; Xn = tsw;
-        // the reference's exit, a rounding-floor stagnation exit, and a cap
-        if (!(norm > tol)) break;
-        if (it > 1 && norm >= prev && norm <= j.floor * xnorm) break;
-        if (it >= j.cap) break;
-        prev = norm;
+        bool all_done = true;
+#pragma unroll
+        for (int k = 0; k < MAXI; k++) {
+            if (done_k[k]) continue;
+            const double norm = sqrt(s_n2[k]);
+            const double xnorm = sqrt(s_x2[k]);
+            it_k[k]++;
+            // the reference's exit, a rounding-floor stagnation exit, and a cap
+            if (!(norm > s_tol[k])) done_k[k] = true;
+            else if (it_k[k] > 1 && norm >= prev_k[k] && norm <= jb.floor * xnorm) done_k[k] = true;
+            else if (it_k[k] >= jb.cap) done_k[k] = true;
+            prev_k[k] = norm;
+            if (!done_k[k]) all_done = false;
+        }
+        if (all_done) break;
     }
+    int iters = 0;
+#pragma unroll
+    for (int k = 0; k < MAXI; k++)
+        if (k == me) iters = it_k[k];
     // Xo holds the final iterate; keep it in X0 for the next warm start and
     // write the coarse ghosts sxz2(I, J, -1).
-    T *sxz2 = a.C.f[7];
-    for (long q = gtid; q < nn; q += gstride) {
-        const long i0 = q % n0 + 1, i1 = q / n0 + 1;
-        const double x = Xo[i1 * w + i0];
-        if (Xo != j.X0) j.X0[i1 * w + i0] = x;
-        sxz2[FIDX(a.C, i0, i1, -1)] = (T)x;
+#pragma unroll
+    for (int k = 0; k < JPPT; k++) {
+        if (!own[k]) continue;
+        const long c = xi[k];
+        const double x = Xo[c];
+        if (Xo != J.X0) J.X0[c] = x;
+        const long i1 = c / w, i0 = c - i1 * w;
+        J.sxz2[J.o2 + (long)(-1) * J.S22 + i1 * J.P2 + i0] = (T)x;
     }
     grid.sync();
     // fine ghosts on fine interior nodes
-    const int r = a.r, nx1 = r * nx2, ny1 = r * ny2;
-    T *sxz1 = a.F.f[7];
+    const int r = J.r, nx1 = r * nx2, ny1 = r * ny2;
     const long nf = (long)(nx1 - 1) * (ny1 - 1);
     for (long q = gtid; q < nf; q += gstride) {
         const int i = (int)(q % (nx1 - 1)) + 1, jj = (int)(q / (nx1 - 1)) + 1;
@@ -664,15 +768,16 @@ __global__ __launch_bounds__(256) void k_jacobi(IfaceDev<T> a, JacobiArgs j)
                 if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
                 const double pw = hatw(ao - r * dI, r) * hatw(bo - r * dJ, r);
                 if (pw == 0) continue;
-                S += pw * (ld(sxz2, FIDX(a.C, Ic, Jc, -1)) + ld(sxz2, FIDX(a.C, Ic, Jc, 0)));
+                const long q2 = J.o2 + (long)Jc * J.P2 + Ic;
+                S += pw * ((double)J.sxz2[q2 - J.S22] + (double)J.sxz2[q2]);
             }
-        const long qf = FIDX(a.F, i, jj, a.nz1);
-        sxz1[qf] = (T)(-ld(sxz1, qf - a.F.S2) + S);
+        const long qf = J.o1 + (long)J.nz1 * J.S21 + (long)jj * J.P1 + i;
+        J.sxz1[qf] = (T)(-(double)J.sxz1[qf - J.S21] + S);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        j.stats[0] += it;
-        j.stats[1] = it;
-        if (it >= j.cap) j.stats[2] = 1;
+    if (lb == 0 && threadIdx.x == 0) {
+        J.stats[0] += iters;
+        J.stats[1] = iters;
+        if (iters >= jb.cap) J.stats[2] = 1;
     }
 }
 

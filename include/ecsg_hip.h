@@ -64,6 +64,10 @@ typedef struct {
                                            vy is not mirrored across the y walls on plane iz = nz
                                            (update_vel.cpp:211-218, 239-246); 1: mirror it there too,
                                            which makes the composite energy exactly conserved */
+    int sxz_solver;                     /* vx-sxz interface system: 0 = Jacobi sweeps as the
+                                           reference (update_ghost_sxz.cpp:314-350); 1 = the same
+                                           sweeps with Chebyshev acceleration (same fixed point and
+                                           exit test, fewer sweeps) */
 } ecsg_config;
 
 /* Creates the device state: wavefields (InitSim, init_del_sim.cpp:3-29),

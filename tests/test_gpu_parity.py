@@ -128,6 +128,26 @@ def test_reference_formulas_ratio2_fp64():
 
 
 @pytest.mark.parametrize("ratios", [(2,), (3,), (2, 3)])
+def test_chebyshev_solver_matches_jacobi_oracle(ratios):
+    """Chebyshev-accelerated sweeps converge to the same interface solution
+    as the reference's Jacobi (same exit test), so results agree to the
+    solver tolerance."""
+    m = composite(ratios, nz_fine=(4,) * len(ratios), nsteps=80, seed=19)
+    o = Oracle(m, eci_mode=1)
+    o.run(m.nsteps)
+    s = ecsg_amd.Simulation(m, precision="f64", sxz_solver="chebyshev")
+    s.step(m.nsteps)
+    s.sync()
+    assert rel(s.traces(), o.traces()) <= 1e-9
+    assert np.allclose(s.energy(), o.energy(), rtol=1e-9)
+    it_cheb = s.jacobi_stats(0)[0]
+    s2 = ecsg_amd.Simulation(m, precision="f64")
+    s2.step(m.nsteps)
+    s2.sync()
+    assert it_cheb < s2.jacobi_stats(0)[0]
+
+
+@pytest.mark.parametrize("ratios", [(2,), (3,), (2, 3)])
 def test_fp32_against_fp64_oracle(ratios):
     m = composite(ratios, nz_fine=(4,) * len(ratios), nsteps=120, seed=11)
     o = Oracle(m, eci_mode=1)

@@ -75,6 +75,6 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_layout_matches_header():
-    # ecsg_config: int, int[8][3], double[8][3], double, 6 ints, int
+    # ecsg_config: int, int[8][3], (pad), double[8][3], double, 8 ints
     import ctypes as C
-    assert C.sizeof(ecsg_amd.EcsgConfig) == 4 + 96 + 4 + 192 + 8 + 7 * 4 + 4
+    assert C.sizeof(ecsg_amd.EcsgConfig) == 4 + 96 + 4 + 192 + 8 + 8 * 4
