@@ -25,7 +25,7 @@ const int SIM_POS[9][3] = {{0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 0, 0}, {1, 0, 0}
 const int ST_POS[9][3] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0},
                           {1, 0, 0}, {1, 1, 1}, {0, 0, 1}, {0, 1, 0}};
 
-constexpr int ZCHUNK = 16;
+constexpr int ZCHUNK = 8;
 constexpr int JACOBI_CAP = 100000;
 
 struct Status {
@@ -85,6 +85,7 @@ struct DevGrid {
     bool params = false;
     int iso = 0;
     double *ke = nullptr, *se = nullptr;  // energy partials, nblk each
+    int by = 4;                           // workgroup height of the stencil tiles
     // host copies (double of the T-rounded values) of the planes the
     // interface matrices need: buoyancy and c33 on iz = 0 and iz = nz
     std::vector<double> b_top, b_bot, c33_top, c33_bot;
@@ -97,7 +98,7 @@ struct DevIface {
     double *Bs = nullptr, *A = nullptr, *absp = nullptr;             // sxz system
     double *X0 = nullptr, *X1 = nullptr, *jpart = nullptr;
     long long *stats = nullptr;
-    int nabs = 0, coop = 0;
+    int nabs = 0, coop = 0, tiles = 0;
     double rho = 0;  // Gershgorin bound of the Jacobi iteration matrix
     bool ready = false;
 };
@@ -120,7 +121,8 @@ public:
     int src_g = -1, src_loc[3] = {0, 0, 0}, src_type = 0, src_len = 0;
     int host_step = 0;
     double tol_cap = 0, tol_fac = 0;
-    int max_coop = 0;  // co-resident Jacobi workgroups
+    int max_coop = 0;  // co-resident workgroups of the register Jacobi kernel
+    int max_tiled = 0; // co-resident workgroups of the tiled Jacobi kernel
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     bool graph_failed = false;
@@ -169,6 +171,10 @@ public:
         if (!(c.dt > 0)) return fail(ECSG_EINVAL, "dt must be positive");
         if (c.max_steps < 1) return fail(ECSG_EINVAL, "max_steps must be >= 1");
         if (sizeof(T) == 8) { tol_cap = 1e-12; tol_fac = 1e-5; } else { tol_cap = 1e-3; tol_fac = 1e-4; }
+        if (const char *e = getenv("ECSG_TILE_BY")) tile_by = atoi(e);
+        if (const char *e = getenv("ECSG_TILE_ZC")) tile_zc = atoi(e);
+        if (tile_by != 1 && tile_by != 2 && tile_by != 4 && tile_by != 8) return fail(ECSG_EINVAL, "tile height must be 1, 2, 4 or 8");
+        if (tile_zc < 1) return fail(ECSG_EINVAL, "z chunk must be >= 1");
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         G.resize(c.num_grids);
         for (int g = 0; g < c.num_grids; g++) {
@@ -203,13 +209,11 @@ public:
             v.iso = 0;
             // vy y-wall mirror planes: [0, nz-1] as the reference loops, or all
             v.zy = c.vy_mirror_all ? d.n[2] : d.n[2] - 1;
-            v.zc = ZCHUNK;
-            v.gx = (d.n[0] + 1 + BX - 1) / BX;
-            v.gy = (d.n[1] + 1 + BY - 1) / BY;
-            const int gz = (d.n[2] + 1 + ZCHUNK - 1) / ZCHUNK;
-            v.nblk = v.gx * v.gy * gz;
-            if (int rc = dalloc(&d.ke, v.nblk)) return rc;
-            if (int rc = dalloc(&d.se, v.nblk)) return rc;
+            set_tiling(d, tile_by, tile_zc);
+            // energy partials sized for the largest tiling (BY = 1, ZC = 1)
+            const size_t maxblk = (size_t)((d.n[0] + BX) / BX) * (d.n[1] + 1) * (d.n[2] + 1);
+            if (int rc = dalloc(&d.ke, maxblk)) return rc;
+            if (int rc = dalloc(&d.se, maxblk)) return rc;
         }
         I.resize(c.num_grids - 1);
         for (int k = 0; k + 1 < c.num_grids; k++) {
@@ -243,13 +247,20 @@ public:
             if (want > (long)max_coop)
                 return fail(ECSG_EINVAL, "interface " + std::to_string(k) + " too large for the resident Jacobi solve");
             f.coop = (int)std::max(1L, want);
-            if (int rc = dalloc(&f.jpart, 4 * (size_t)f.coop + 1)) return rc;
+            int pert = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pert, (const void *)k_jacobi_tiled<T>, JT, 0));
+            max_tiled = pert * ncu;
+            f.tiles = ((f.nx2 - 1 + TJ - 1) / TJ) * ((f.ny2 - 1 + TJ - 1) / TJ);
+            const size_t np = std::max(4 * (size_t)f.coop + 1, (size_t)2 * f.tiles * 2 * SJ);
+            if (int rc = dalloc(&f.jpart, np)) return rc;
             if (int rc = dalloc(&f.stats, 3)) return rc;
         }
         for (int k0 = 0; k0 < (int)I.size(); k0 += MAXI) {
             int tot = 0;
-            for (int k = k0; k < std::min((int)I.size(), k0 + MAXI); k++) tot += I[k].coop;
-            if (tot > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+            int tott = 0;
+            for (int k = k0; k < std::min((int)I.size(), k0 + MAXI); k++) { tot += I[k].coop; tott += I[k].tiles; }
+            if (tot > max_coop || (c.sxz_solver == 0 && tott > max_tiled))
+                return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
         }
         if (c.sxz_solver != 0 && c.sxz_solver != 1) return fail(ECSG_EINVAL, "sxz_solver must be 0 (Jacobi) or 1 (Chebyshev)");
         if (int rc = dalloc(&d_step, 1)) return rc;
@@ -261,6 +272,19 @@ public:
         if (int rc = dalloc(&d_rec, 4)) return rc;
         if (int rc = dalloc(&d_traces, 1)) return rc;
         return ECSG_OK;
+    }
+
+    // stencil tiling: BX x by threads per workgroup, zc planes per thread
+    int tile_by = 4, tile_zc = ZCHUNK;
+    static void set_tiling(DevGrid<T> &d, int by, int zc)
+    {
+        GridDev<T> &v = d.dev;
+        v.zc = zc;
+        v.gx = (d.n[0] + 1 + BX - 1) / BX;
+        v.gy = (d.n[1] + 1 + by - 1) / by;
+        const int gz = (d.n[2] + 1 + zc - 1) / zc;
+        v.nblk = v.gx * v.gy * gz;
+        d.by = by;
     }
 
     bool all_params() const
@@ -523,16 +547,35 @@ public:
         ev_kind.push_back(kind);
     }
 
+    template <int BYv>
+    void vel_launch(DevGrid<T> &d, bool en, const int *srcp)
+    {
+        if (en)
+            hipLaunchKernelGGL((k_vel<T, true, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.ke, srcp, d_src_old);
+        else
+            hipLaunchKernelGGL((k_vel<T, false, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.ke, srcp, d_src_old);
+    }
+    template <int BYv>
+    void stress_launch(DevGrid<T> &d, bool en)
+    {
+        if (en)
+            hipLaunchKernelGGL((k_stress<T, true, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.se);
+        else
+            hipLaunchKernelGGL((k_stress<T, false, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.se);
+    }
+
     int launch_vel(int g)
     {
         DevGrid<T> &d = G[g];
         const bool en = cfg.check_energy != 0;
         const int *srcp = (en && src_type == 0 && g == src_g && src_len > 0) ? d_src_loc : nullptr;
         ev_begin();
-        if (en)
-            hipLaunchKernelGGL((k_vel<T, true>), dim3(d.dev.nblk), dim3(BX, BY), 0, s, d.dev, d.ke, srcp, d_src_old);
-        else
-            hipLaunchKernelGGL((k_vel<T, false>), dim3(d.dev.nblk), dim3(BX, BY), 0, s, d.dev, d.ke, srcp, d_src_old);
+        switch (d.by) {
+        case 1: vel_launch<1>(d, en, srcp); break;
+        case 2: vel_launch<2>(d, en, srcp); break;
+        case 8: vel_launch<8>(d, en, srcp); break;
+        default: vel_launch<4>(d, en, srcp); break;
+        }
         ev_end(g == 0 ? 0 : 2);
         HIPCHK(hipGetLastError());
         return ECSG_OK;
@@ -542,10 +585,12 @@ public:
     {
         DevGrid<T> &d = G[g];
         ev_begin();
-        if (cfg.check_energy)
-            hipLaunchKernelGGL((k_stress<T, true>), dim3(d.dev.nblk), dim3(BX, BY), 0, s, d.dev, d.se);
-        else
-            hipLaunchKernelGGL((k_stress<T, false>), dim3(d.dev.nblk), dim3(BX, BY), 0, s, d.dev, d.se);
+        switch (d.by) {
+        case 1: stress_launch<1>(d, cfg.check_energy != 0); break;
+        case 2: stress_launch<2>(d, cfg.check_energy != 0); break;
+        case 8: stress_launch<8>(d, cfg.check_energy != 0); break;
+        default: stress_launch<4>(d, cfg.check_energy != 0); break;
+        }
         ev_end(g == 0 ? 1 : 3);
         HIPCHK(hipGetLastError());
         return ECSG_OK;
@@ -601,8 +646,9 @@ public:
             J.o1 = G[k0 + q].o; J.P1 = G[k0 + q].P; J.S21 = G[k0 + q].S2;
             J.o2 = G[k0 + q + 1].o; J.P2 = G[k0 + q + 1].P; J.S22 = G[k0 + q + 1].S2;
             J.nx2 = f.nx2; J.ny2 = f.ny2; J.nz1 = f.nz1; J.r = f.r;
-            J.blk0 = blk; J.nblk = f.coop; J.rho2 = f.rho * f.rho;
-            blk += f.coop;
+            const bool tiled = cfg.sxz_solver == 0;
+            J.blk0 = blk; J.nblk = tiled ? f.tiles : f.coop; J.rho2 = f.rho * f.rho;
+            blk += J.nblk;
         }
         jb.n = n;
         jb.tol_cap = tol_cap; jb.tol_fac = tol_fac;
@@ -613,9 +659,15 @@ public:
         for (int q = 0; q < n; q++)
             if (!(I[k0 + q].rho < 1.0)) cheb_ok = false;
         jb.cheby = cheb_ok ? 1 : 0;
-        if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
         void *args[] = {(void *)&jb};
-        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, 0, s));
+        if (cfg.sxz_solver == 0) {
+            // the reference's Jacobi sweeps, SJ per grid barrier in LDS tiles
+            if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JT), args, 0, s));
+        } else {
+            if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, 0, s));
+        }
         return ECSG_OK;
     }
 
@@ -783,7 +835,12 @@ public:
             T *save = nullptr;
             if (int rc = dalloc(&save, (size_t)d.astride * 6)) return rc;
             HIPCHK(hipMemcpyAsync(save, d.arr(3), sizeof(T) * d.astride * 6, hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL((k_stress<T, true>), dim3(d.dev.nblk), dim3(BX, BY), 0, s, d.dev, d.se);
+            switch (d.by) {
+            case 1: stress_launch<1>(d, true); break;
+            case 2: stress_launch<2>(d, true); break;
+            case 8: stress_launch<8>(d, true); break;
+            default: stress_launch<4>(d, true); break;
+            }
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(d.arr(3), save, sizeof(T) * d.astride * 6, hipMemcpyDeviceToDevice, s));
             HIPCHK(hipStreamSynchronize(s));

@@ -51,8 +51,9 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
     return x * q + (x < rem ? x : rem) + k;
 }
 
-// Fixed-order workgroup reduction of one double per thread (256 threads).
-__device__ __forceinline__ double block_sum_256(double v, double *lds)
+// Fixed-order workgroup reduction of one double per thread (NW wavefronts).
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double *lds)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -60,7 +61,8 @@ __device__ __forceinline__ double block_sum_256(double v, double *lds)
     if ((lane & 63) == 0) lds[lane >> 6] = v;
     __syncthreads();
     double r = 0.0;
-    if (lane == 0) r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    if (lane == 0)
+        for (int q = 0; q < NW; q++) r += lds[q];
     return r;  // valid in lane 0
 }
 
@@ -68,18 +70,18 @@ __device__ __forceinline__ double block_sum_256(double v, double *lds)
 // UpdateVel (update_vel.cpp:4-249) with its radius-1 ghost mirrors fused.
 // EN: also the kinetic part of ComputeEnergy (compute_energy.cpp:38-39,
 // TriMulFields field.cpp:482-518) from the pre- and post-update velocities.
-template <typename T, bool EN>
-__global__ __launch_bounds__(NTHR) void k_vel(GridDev<T> g, double *__restrict__ part,
-                                              const int *__restrict__ src, T *__restrict__ src_old)
+template <typename T, bool EN, int BYv>
+__global__ __launch_bounds__(BX * BYv) void k_vel(GridDev<T> g, double *__restrict__ part,
+                                                  const int *__restrict__ src, T *__restrict__ src_old)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[BYv];
     const int tile = xcd_tile(blockIdx.x, g.nblk);
     const int bx = tile % g.gx;
     const int rest = tile / g.gx;
     const int by = rest % g.gy;
     const int bz = rest / g.gy;
     const int ix = bx * BX + threadIdx.x;
-    const int iy = by * BY + threadIdx.y;
+    const int iy = by * BYv + threadIdx.y;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int z0 = bz * g.zc;
     const int z1 = min(z0 + g.zc, nz + 1);
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(NTHR) void k_vel(GridDev<T> g, double *__restrict__
         }
     }
     if (EN) {
-        const double s = block_sum_256(acc, lds);
+        const double s = block_sum<BYv>(acc, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = s;
     }
 }
@@ -223,17 +225,17 @@ __device__ __forceinline__ void compliance(const GridDev<T> &g, T c11, T c22, T 
 // UpdateStress (update_stress.cpp:4-311), free surface and its radius-1 ghost
 // stresses fused.  EN: strain part of ComputeEnergy (compute_energy.cpp:40-49)
 // from the pre-update stresses.
-template <typename T, bool EN>
-__global__ __launch_bounds__(NTHR) void k_stress(GridDev<T> g, double *__restrict__ part)
+template <typename T, bool EN, int BYv>
+__global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__restrict__ part)
 {
-    __shared__ double lds[4];
+    __shared__ double lds[BYv];
     const int tile = xcd_tile(blockIdx.x, g.nblk);
     const int bx = tile % g.gx;
     const int rest = tile / g.gx;
     const int by = rest % g.gy;
     const int bz = rest / g.gy;
     const int ix = bx * BX + threadIdx.x;
-    const int iy = by * BY + threadIdx.y;
+    const int iy = by * BYv + threadIdx.y;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int z0 = bz * g.zc;
     const int z1 = min(z0 + g.zc, nz + 1);
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(NTHR) void k_stress(GridDev<T> g, double *__restric
         }
     }
     if (EN) {
-        const double s = block_sum_256(acc, lds);
+        const double s = block_sum<BYv>(acc, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) part[tile] = s;
     }
 }
@@ -775,6 +777,243 @@ __global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
         J.sxz1[qf] = (T)(-(double)J.sxz1[qf - J.S21] + S);
     }
     if (lb == 0 && threadIdx.x == 0) {
+        J.stats[0] += iters;
+        J.stats[1] = iters;
+        if (iters >= jb.cap) J.stats[2] = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled s-step form of the same Jacobi solve.  Workgroup = one TJ x TJ tile of
+// unknowns of one interface.  Per grid barrier it loads the current iterate on
+// the tile plus a halo of width SJ into LDS and performs SJ Jacobi sweeps
+// there, the updated region shrinking by one ring per sweep; the values it
+// produces on the tile are those of SJ global Jacobi sweeps, operation for
+// operation.  Norm partials are kept per sweep and the tile after each sweep
+// is stashed in LDS, so the exit happens at exactly the sweep where the
+// reference's test (and the stagnation test) first holds.
+constexpr int TJ = 32;                   // tile edge (unknowns)
+constexpr int SJ = 6;                    // sweeps per grid barrier
+constexpr int EJ = TJ + 2 * SJ;          // LDS edge of tile + halo
+constexpr int RJ = TJ + 2 * (SJ - 1);    // edge of the first sweep's region
+constexpr int KJ = (RJ * RJ + JT - 1) / JT;  // region points per thread
+
+template <typename T>
+__global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
+{
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    __shared__ double xb[2][EJ * EJ];
+    __shared__ double stash[SJ][TJ * TJ];
+    __shared__ double red[JT / 64][2 * SJ];
+    __shared__ double s_tol[MAXI];
+    __shared__ double s_n2[MAXI][SJ], s_x2[MAXI][SJ];
+    int me = 0;
+    for (int k = 1; k < jb.n; k++)
+        if ((int)blockIdx.x >= jb.it[k].blk0) me = k;
+    const JacItf<T> &J = jb.it[me];
+    const int nx2 = J.nx2, ny2 = J.ny2;
+    const int n0 = nx2 - 1, n1 = ny2 - 1;
+    const long w = nx2 + 1;
+    const int lb = blockIdx.x - J.blk0;
+    const int ntx = (n0 + TJ - 1) / TJ;
+    const int I0 = 1 + (lb % ntx) * TJ, J0 = 1 + (lb / ntx) * TJ;  // tile origin (unknown indices)
+    const int tid = threadIdx.x;
+    if (tid < 64)
+        for (int k = 0; k < jb.n; k++) {
+            const JacItf<T> &K = jb.it[k];
+            const double sb = wave_sum_fixed(K.abs_part, K.nabs);
+            if (tid == 0) {
+                const double avg = sb / (double)((long)(K.nx2 - 1) * (K.ny2 - 1));
+                const double t2 = avg * jb.tol_fac;
+                s_tol[k] = (jb.tol_cap < t2) ? jb.tol_cap : t2;
+            }
+        }
+    // this thread's points of the first sweep's region, bands and rhs in registers
+    const long nn = (long)n0 * n1;
+    double Ar[KJ][9], Br[KJ];
+    int li[KJ];          // LDS index, -1 when not an unknown
+    int sh[KJ];          // ring distance outside the tile (0 inside)
+    int ti[KJ];          // tile index, -1 outside the tile
+#pragma unroll
+    for (int k = 0; k < KJ; k++) {
+        const int p = tid + k * JT;
+        li[k] = -1; sh[k] = 0; ti[k] = -1;
+        if (p < RJ * RJ) {
+            const int ri = p % RJ, rj = p / RJ;
+            const int I = I0 - (SJ - 1) + ri, Jg = J0 - (SJ - 1) + rj;
+            const int di = max(max(I0 - I, I - (I0 + TJ - 1)), 0), dj = max(max(J0 - Jg, Jg - (J0 + TJ - 1)), 0);
+            sh[k] = max(di, dj);
+            if (I >= 1 && I <= n0 && Jg >= 1 && Jg <= n1) {
+                li[k] = (rj + 1) * EJ + (ri + 1);
+                if (sh[k] == 0) ti[k] = (Jg - J0) * TJ + (I - I0);
+                const long q = (long)(Jg - 1) * n0 + (I - 1);
+#pragma unroll
+                for (int b = 0; b < 9; b++) Ar[k][b] = J.A[b * nn + q];
+                Br[k] = J.B[q];
+            }
+        }
+    }
+    double *Xo = J.X0, *Xn = J.X1;
+    int it_k[MAXI];
+    bool done_k[MAXI];
+    double prev_k[MAXI];
+    int stop_k[MAXI];    // sweep (1..SJ) of the last batch at which interface k stopped
+#pragma unroll
+    for (int k = 0; k < MAXI; k++) { it_k[k] = 0; done_k[k] = (k >= jb.n); prev_k[k] = 0; stop_k[k] = SJ; }
+    int batch = 0;
+    while (true) {
+        bool active = false;
+#pragma unroll
+        for (int k = 0; k < MAXI; k++)
+            if (k == me) active = !done_k[k];
+        if (active) {
+            // load tile + halo of the current iterate (zero outside the unknowns)
+            for (int p = tid; p < EJ * EJ; p += JT) {
+                const int ei = p % EJ, ej = p / EJ;
+                const int I = I0 - SJ + ei, Jg = J0 - SJ + ej;
+                double v = 0;
+                if (I >= 1 && I <= n0 && Jg >= 1 && Jg <= n1) v = Xo[(long)Jg * w + I];
+                xb[0][p] = v;
+                xb[1][p] = v;
+            }
+            __syncthreads();
+            double acc[SJ], ax[SJ];
+#pragma unroll
+            for (int t = 0; t < SJ; t++) { acc[t] = 0; ax[t] = 0; }
+#pragma unroll
+            for (int t = 1; t <= SJ; t++) {
+                const double *xo = xb[(t - 1) & 1];
+                double *xn = xb[t & 1];
+#pragma unroll
+                for (int k = 0; k < KJ; k++) {
+                    const int c = li[k];
+                    if (c < 0 || sh[k] > SJ - t) continue;
+                    const double v = (Br[k]
+                                      - Ar[k][0] * xo[c - EJ - 1]
+                                      - Ar[k][1] * xo[c - 1]
+                                      - Ar[k][2] * xo[c + EJ - 1]
+                                      - Ar[k][3] * xo[c - EJ]
+                                      - Ar[k][5] * xo[c + EJ]
+                                      - Ar[k][6] * xo[c - EJ + 1]
+                                      - Ar[k][7] * xo[c + 1]
+                                      - Ar[k][8] * xo[c + EJ + 1]) / Ar[k][4];
+                    xn[c] = v;
+                    if (ti[k] >= 0) {
+                        const double d = v - xo[c];
+                        acc[t - 1] += d * d;
+                        ax[t - 1] += v * v;
+                        stash[t - 1][ti[k]] = v;
+                    }
+                }
+                __syncthreads();
+            }
+            // per-sweep workgroup sums, fixed order
+#pragma unroll
+            for (int t = 0; t < SJ; t++) {
+                double a = acc[t], b = ax[t];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    a += __shfl_xor(a, off, 64);
+                    b += __shfl_xor(b, off, 64);
+                }
+                if ((tid & 63) == 0) { red[tid >> 6][2 * t] = a; red[tid >> 6][2 * t + 1] = b; }
+            }
+            __syncthreads();
+            if (tid < 2 * SJ) {
+                double v = 0;
+                for (int q = 0; q < JT / 64; q++) v += red[q][tid];
+                J.part[((long)(batch & 1) * J.nblk + lb) * 2 * SJ + tid] = v;
+            }
+            // the tile after SJ sweeps is the next batch's starting iterate
+            for (int p = tid; p < TJ * TJ; p += JT) {
+                const int I = I0 + p % TJ, Jg = J0 + p / TJ;
+                if (I <= n0 && Jg <= n1) Xn[(long)Jg * w + I] = stash[SJ - 1][p];
+            }
+        }
+        grid.sync();
+        // every workgroup replays every active interface's per-sweep exit test
+        if (tid < 64)
+            for (int k = 0; k < jb.n; k++) {
+                bool dk = false;
+#pragma unroll
+                for (int kk = 0; kk < MAXI; kk++)
+                    if (kk == k) dk = done_k[kk];
+                if (dk) continue;
+                const JacItf<T> &K = jb.it[k];
+                const double *part = K.part + (long)(batch & 1) * K.nblk * 2 * SJ;
+                for (int t = 0; t < SJ; t++) {
+                    const int lane = tid & 63;
+                    double a = 0, b = 0;
+                    for (int q = lane; q < K.nblk; q += 64) {
+                        a += part[(long)q * 2 * SJ + 2 * t];
+                        b += part[(long)q * 2 * SJ + 2 * t + 1];
+                    }
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) {
+                        a += __shfl_xor(a, off, 64);
+                        b += __shfl_xor(b, off, 64);
+                    }
+                    if (tid == 0) { s_n2[k][t] = a; s_x2[k][t] = b; }
+                }
+            }
+        __syncthreads();
+        bool all_done = true;
+#pragma unroll
+        for (int k = 0; k < MAXI; k++) {
+            if (done_k[k]) continue;
+            for (int t = 0; t < SJ && !done_k[k]; t++) {
+                const double norm = sqrt(s_n2[k][t]);
+                const double xnorm = sqrt(s_x2[k][t]);
+                it_k[k]++;
+                if (!(norm > s_tol[k])) done_k[k] = true;
+                else if (it_k[k] > 1 && norm >= prev_k[k] && norm <= jb.floor * xnorm) done_k[k] = true;
+                else if (it_k[k] >= jb.cap) done_k[k] = true;
+                prev_k[k] = norm;
+                if (done_k[k]) stop_k[k] = t + 1;
+            }
+            if (!done_k[k]) all_done = false;
+        }
+        if (active) { double *tsw = Xo; Xo = Xn; Xn = tsw; }
+        batch++;
+        if (all_done) break;
+    }
+    int iters = 0, stop = SJ;
+#pragma unroll
+    for (int k = 0; k < MAXI; k++)
+        if (k == me) { iters = it_k[k]; stop = stop_k[k]; }
+    // the tile at the exit sweep: next warm start and coarse ghosts sxz2(I,J,-1)
+    for (int p = tid; p < TJ * TJ; p += JT) {
+        const int I = I0 + p % TJ, Jg = J0 + p / TJ;
+        if (I <= n0 && Jg <= n1) {
+            const double x = stash[stop - 1][p];
+            J.X0[(long)Jg * w + I] = x;
+            J.sxz2[J.o2 - J.S22 + (long)Jg * J.P2 + I] = (T)x;
+        }
+    }
+    grid.sync();
+    // fine ghosts on fine interior nodes
+    const int r = J.r, nx1 = r * nx2, ny1 = r * ny2;
+    const long nf = (long)(nx1 - 1) * (ny1 - 1);
+    const long gtid = (long)lb * blockDim.x + tid;
+    const long gstride = (long)J.nblk * blockDim.x;
+    for (long q = gtid; q < nf; q += gstride) {
+        const int i = (int)(q % (nx1 - 1)) + 1, jj = (int)(q / (nx1 - 1)) + 1;
+        const int Ia = i / r, Ja = jj / r, ao = i - r * Ia, bo = jj - r * Ja;
+        double S = 0;
+        for (int dJ = 0; dJ <= 1; dJ++)
+            for (int dI = 0; dI <= 1; dI++) {
+                const int Ic = Ia + dI, Jc = Ja + dJ;
+                if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
+                const double pw = hatw(ao - r * dI, r) * hatw(bo - r * dJ, r);
+                if (pw == 0) continue;
+                const long q2 = J.o2 + (long)Jc * J.P2 + Ic;
+                S += pw * ((double)J.sxz2[q2 - J.S22] + (double)J.sxz2[q2]);
+            }
+        const long qf = J.o1 + (long)J.nz1 * J.S21 + (long)jj * J.P1 + i;
+        J.sxz1[qf] = (T)(-(double)J.sxz1[qf - J.S21] + S);
+    }
+    if (lb == 0 && tid == 0) {
         J.stats[0] += iters;
         J.stats[1] = iters;
         if (iters >= jb.cap) J.stats[2] = 1;
