@@ -97,6 +97,19 @@ def cpu_baseline(m, steps: int):
     return el, threads
 
 
+def profiled_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE (scaled by a copy of known size) and committed
+    under profiles/; None when no profile of this workload exists."""
+    path = os.path.join(ROOT, "profiles", "stencil_kernels_%s.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def copy_calibration():
     """A plain 1 GiB device-to-device copy (torch) so that FETCH_SIZE /
     WRITE_SIZE of a kernel with known traffic can scale the stencil counters."""
@@ -196,7 +209,7 @@ def main():
         "hbm_frac_step": round(BYTES_STEP_F32 * gpts * 1e9 / HBM_PEAK, 4),
         "roofline": {"bound": "hbm", "kernel": "k_stress<float> on the %dx%dx%d-node grid" % tuple(x + 1 for x in g0.n),
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": profiled_traffic(args.workload),
                      "bytes_per_launch": BYTES_STRESS_F32 * n0, "avg_launch_ms": round(ms[1], 5),
                      "vel_kernel_GBps": round(vel_bw / 1e9, 1)},
         "fp64": fp64,
