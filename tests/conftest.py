@@ -6,6 +6,8 @@ Markers:
 Everything else runs on the CPU: the oracle (CPU restatement) against the
 reference's golden vectors and mathematical invariants, the host logic, and
 that the C-ABI library loads and exports its symbols.
+  slow -- full-length CPU runs of the golden flows (about 3 minutes each);
+          skipped unless ECSG_RUN_SLOW=1.
 """
 import os
 import sys
@@ -25,6 +27,9 @@ def pytest_configure(config):
 
 def pytest_collection_modifyitems(config, items):
     have_gpu = os.path.exists("/dev/kfd")
+    run_slow = os.environ.get("ECSG_RUN_SLOW", "") not in ("", "0")
     for it in items:
         if "gpu" in it.keywords and not have_gpu:
             it.add_marker(pytest.mark.skip(reason="no GPU device (/dev/kfd) on this machine"))
+        if "slow" in it.keywords and not run_slow:
+            it.add_marker(pytest.mark.skip(reason="long CPU run; set ECSG_RUN_SLOW=1"))
