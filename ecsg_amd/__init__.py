@@ -30,7 +30,11 @@ ABI_SYMBOLS = (
     "ecsg_compute_energy", "ecsg_update_stress", "ecsg_update_ghost_sxz", "ecsg_update_ghost_syz", "ecsg_end_step",
     "ecsg_compute_res", "ecsg_step_n", "ecsg_steps_done", "ecsg_synchronize", "ecsg_field_dims", "ecsg_get_field",
     "ecsg_set_field", "ecsg_get_traces", "ecsg_get_energy", "ecsg_jacobi_stats", "ecsg_profile_kernels",
+    "ecsg_reset", "ecsg_set_checkpointing", "ecsg_gradient", "ecsg_get_gradient", "ecsg_adjoint_step",
+    "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats",
 )
+
+CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
 
 
 class EcsgConfig(C.Structure):
@@ -91,6 +95,14 @@ def load_library(path: Optional[str] = None):
     L.ecsg_get_energy.argtypes = [vp, P(d)]
     L.ecsg_jacobi_stats.argtypes = [vp, i, P(C.c_longlong), P(i), P(i)]
     L.ecsg_profile_kernels.argtypes = [vp, i, P(d), P(i)]
+    L.ecsg_reset.argtypes = [vp]
+    L.ecsg_set_checkpointing.argtypes = [vp, i, i]
+    L.ecsg_gradient.argtypes = [vp, i, P(d), P(d)]
+    L.ecsg_get_gradient.argtypes = [vp, i, i, P(d)]
+    L.ecsg_adjoint_step.argtypes = [vp]
+    L.ecsg_get_adjoint_field.argtypes = [vp, i, i, P(d)]
+    L.ecsg_set_adjoint_field.argtypes = [vp, i, i, P(d)]
+    L.ecsg_adjoint_stats.argtypes = [vp, P(C.c_longlong)]
     if path is None:
         _LIB = L
     return L
@@ -235,3 +247,44 @@ class Simulation:
         cnt = (C.c_int * 6)()
         self._chk(self.L.ecsg_profile_kernels(self.h, int(nsteps), ms, cnt))
         return list(ms), list(cnt)
+
+    # --- adjoint ------------------------------------------------------------
+    def reset(self):
+        """Zero wavefields, interface warm starts, traces and the step counter."""
+        self._chk(self.L.ecsg_reset(self.h))
+
+    def set_checkpointing(self, every: int = 50, fmt: str = "native"):
+        self._chk(self.L.ecsg_set_checkpointing(self.h, int(every), CKPT_FORMATS[fmt]))
+
+    def gradient(self, nsteps: int, observed: np.ndarray) -> float:
+        """Misfit 1/2 sum (traces - observed)^2 over nsteps steps from rest and
+        its gradient (read with grad()).  observed: nrec x nsteps."""
+        obs = np.ascontiguousarray(observed, np.float64).reshape(self.nrec, nsteps) if self.nrec else np.zeros(1)
+        J = C.c_double()
+        self._chk(self.L.ecsg_gradient(self.h, int(nsteps), _dp(obs), C.byref(J)))
+        return J.value
+
+    def grad(self, g: int, sel: int) -> np.ndarray:
+        """dJ/d(input) at primary nodes, (nz+1, ny+1, nx+1): sel 9 = buoyancy,
+        10 + m = stiff[m] of set_params."""
+        G = self.model.grids[g]
+        out = np.empty((G.n[2] + 1, G.n[1] + 1, G.n[0] + 1), np.float64)
+        self._chk(self.L.ecsg_get_gradient(self.h, g, sel, _dp(out)))
+        return out
+
+    def adjoint_step(self):
+        self._chk(self.L.ecsg_adjoint_step(self.h))
+
+    def get_adjoint(self, g, sel) -> np.ndarray:
+        out = np.empty(self.dims(g, sel), np.float64)
+        self._chk(self.L.ecsg_get_adjoint_field(self.h, g, sel, _dp(out)))
+        return out
+
+    def set_adjoint(self, g, sel, a):
+        a = np.ascontiguousarray(a, np.float64).reshape(self.dims(g, sel))
+        self._chk(self.L.ecsg_set_adjoint_field(self.h, g, sel, _dp(a)))
+
+    def adjoint_sweeps(self) -> int:
+        n = C.c_longlong()
+        self._chk(self.L.ecsg_adjoint_stats(self.h, C.byref(n)))
+        return n.value

@@ -3,6 +3,7 @@
 // capture, and the C ABI declared in include/ecsg_hip.h.
 #include "ecsg_hip.h"
 #include "ecsg_kernels.hpp"
+#include "ecsg_adjoint.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -67,6 +68,13 @@ public:
     virtual int get_energy(double *out) = 0;
     virtual int jacobi_stats(int k, long long *tot, int *last, int *capped) = 0;
     virtual int profile(int nsteps, double ms[6], int cnt[6]) = 0;
+    virtual int reset() = 0;
+    virtual int set_checkpointing(int every, int format) = 0;
+    virtual int gradient(int nsteps, const double *obs, double *misfit) = 0;
+    virtual int get_gradient(int g, int sel, double *out) = 0;
+    virtual int adjoint_step() = 0;
+    virtual int adj_field(int g, int sel, double *out, const double *in) = 0;
+    virtual int adjoint_stats(long long *sweeps, double *ms) = 0;
 };
 
 #define HIPCHK(x)                                                                                  \
@@ -147,6 +155,7 @@ public:
         }
         hipFree(d_rec); hipFree(d_traces); hipFree(d_step); hipFree(d_energy); hipFree(d_fix);
         hipFree(d_src_old); hipFree(d_src_loc); hipFree(d_src_tr);
+        adj_free();
         for (auto e : evs) hipEventDestroy(e);
         if (s) hipStreamDestroy(s);
     }
@@ -1051,6 +1060,925 @@ public:
         ev_next = 0;
         return ECSG_OK;
     }
+
+    // ======================================================================
+    // Adjoint-state gradient of the receiver misfit
+    //   J = 1/2 sum_n sum_r (vx_r(n) - obs_r(n))^2
+    // with respect to the parameters given to set_params.  Forward pass with
+    // checkpoints every ck_every steps; then, segment by segment from the
+    // last, the forward steps are recomputed from the checkpoint into a ring
+    // of saved states and the transposed steps run backwards over them.
+    // ======================================================================
+    struct DevList {            // CSR on the device: distinct keys and their terms
+        long *ukey = nullptr;
+        int *start = nullptr;
+        int *row = nullptr;
+        long *tkey = nullptr;
+        double *cst = nullptr;
+        int n = 0;
+        long nt = 0;
+    };
+    struct DevOp {              // one interface map
+        DevList fine, prop, grad;
+        long *zkey = nullptr;   // fine-ghost outputs (zeroed after their transpose)
+        int nz = 0;
+        long *xkey = nullptr;   // entries written from the solution
+        int *xrow = nullptr;    // their right-hand-side rows
+        int nx = 0;
+    };
+    struct AdjIface {
+        DevOp op[3];            // 0 vz-szz, 1 vx-sxz, 2 vy-syz
+        double *mu_vz = nullptr, *mu_syz = nullptr, *den = nullptr;
+        double *Ba = nullptr, *X0a = nullptr, *X1a = nullptr, *jpart = nullptr, *absz = nullptr;
+        long long *stats = nullptr;
+    };
+    struct MirrorDev {
+        long *src = nullptr;
+        int *start = nullptr;
+        long *halo = nullptr;
+        int n = 0, nh = 0;
+    };
+    struct AdjGrid {
+        T *lam = nullptr;       // 9 adjoint wavefields
+        T *grad = nullptr;      // 12 gradient accumulators (9 staged stiffness, b node, b xy mean, b xz mean)
+        MirrorDev vA, vB, sm;   // transposed ghost mirrors: vy y walls, other velocity mirrors, free surface
+    };
+    std::vector<AdjIface> AI;
+    std::vector<AdjGrid> AG;
+    bool adj_ready = false;
+    long *d_rloc = nullptr;
+    int *d_rstart = nullptr, *d_rrec = nullptr;
+    int nrloc = 0;
+    T *d_resid = nullptr;
+    int ck_every = 50, ck_fmt = 0;
+    int ck_alloc = 0, ck_alloc_fmt = -1, ring_alloc = 0;
+    std::vector<T *> ck_nat;              // [ck][grid]
+    std::vector<unsigned short *> ck_red; // [ck][grid]
+    std::vector<double *> ck_x0;          // [ck][iface]
+    float *ck_scale = nullptr;            // [ck][grid][9]
+    float *ck_part = nullptr;
+    std::vector<T *> ring;                // [slot][grid]
+    std::vector<std::vector<double>> gprim;  // [grid*10 + a]: 0 buoyancy, 1..9 stiffness inputs
+    double adj_rel_tol = 1e-11;
+    static constexpr int CKP = 512;       // partial maxima per array
+
+    template <typename P>
+    static void dfree(P *&p) { if (p) hipFree((void *)p); p = nullptr; }
+    static void list_free(DevList &L) { dfree(L.ukey); dfree(L.start); dfree(L.row); dfree(L.tkey); dfree(L.cst); L.n = 0; L.nt = 0; }
+    void ck_free()
+    {
+        for (auto p : ck_nat) dfree(p);
+        for (auto p : ck_red) dfree(p);
+        for (auto p : ck_x0) dfree(p);
+        for (auto p : ring) dfree(p);
+        ck_nat.clear(); ck_red.clear(); ck_x0.clear(); ring.clear();
+        dfree(ck_scale); dfree(ck_part);
+        ck_alloc = 0; ck_alloc_fmt = -1; ring_alloc = 0;
+    }
+    void adj_free()
+    {
+        for (auto &a : AI) {
+            for (auto &o : a.op) { list_free(o.fine); list_free(o.prop); list_free(o.grad); dfree(o.zkey); dfree(o.xkey); dfree(o.xrow); }
+            dfree(a.mu_vz); dfree(a.mu_syz); dfree(a.den); dfree(a.Ba); dfree(a.X0a); dfree(a.X1a); dfree(a.jpart); dfree(a.absz);
+            dfree(a.stats);
+        }
+        for (auto &g : AG) {
+            dfree(g.lam); dfree(g.grad);
+            for (MirrorDev *m : {&g.vA, &g.vB, &g.sm}) { dfree(m->src); dfree(m->start); dfree(m->halo); }
+        }
+        AI.clear(); AG.clear();
+        dfree(d_rloc); dfree(d_rstart); dfree(d_rrec); dfree(d_resid);
+        ck_free();
+        adj_ready = false;
+    }
+
+    template <typename P>
+    int upload(P **dst, const std::vector<P> &v)
+    {
+        if (int rc = dalloc(dst, std::max<size_t>(v.size(), 1))) return rc;
+        if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), sizeof(P) * v.size(), hipMemcpyHostToDevice));
+        return ECSG_OK;
+    }
+
+    static long key(int side, int a, long idx) { return ((long)(side * 16 + a) << KSH) | idx; }
+    struct GT { long key; int row; long tkey; double c; };
+
+    int upload_csr(std::vector<GT> &v, DevList &L)
+    {
+        std::stable_sort(v.begin(), v.end(), [](const GT &a, const GT &b) { return a.key < b.key; });
+        std::vector<long> uk, tk;
+        std::vector<int> st, rw;
+        std::vector<double> cs;
+        uk.reserve(v.size()); tk.reserve(v.size()); rw.reserve(v.size()); cs.reserve(v.size());
+        for (size_t i = 0; i < v.size(); i++) {
+            if (i == 0 || v[i].key != v[i - 1].key) { uk.push_back(v[i].key); st.push_back((int)i); }
+            rw.push_back(v[i].row); tk.push_back(v[i].tkey); cs.push_back(v[i].c);
+        }
+        st.push_back((int)v.size());
+        L.n = (int)uk.size();
+        L.nt = (long)v.size();
+        if (int rc = upload(&L.ukey, uk)) return rc;
+        if (int rc = upload(&L.start, st)) return rc;
+        if (int rc = upload(&L.row, rw)) return rc;
+        if (int rc = upload(&L.tkey, tk)) return rc;
+        if (int rc = upload(&L.cst, cs)) return rc;
+        return ECSG_OK;
+    }
+
+    // Host description of one interface map, mirroring the forward kernel:
+    //   B terms: rhs[row] += c * param(p) * u(f)  (p < 0: no parameter)
+    //   A terms: A[row][col] += c * param(p), stored with f = the entry that
+    //            holds x[col] (used by the gradient only)
+    //   fine terms: out = sum c * in (the fine ghosts written after the solve)
+    //   xs: the entry written from x[row] and the rhs row it is read back into
+    struct OpBuild {
+        std::vector<GT> b, a, fine;
+        std::vector<long> xkey;
+        std::vector<int> xrow;
+        void B(long f, long p, int row, double c) { b.push_back({f, row, p, c}); }
+        void A(long x, long p, int row, double c) { a.push_back({x, row, p, c}); }
+        void F(long in, long out, double c) { fine.push_back({in, 0, out, c}); }
+    };
+
+    int upload_op(OpBuild &ob, DevOp &op)
+    {
+        // prop: grouped by wavefield entry, term key = parameter
+        std::vector<GT> pv = ob.b;
+        if (int rc = upload_csr(pv, op.prop)) return rc;
+        // grad: grouped by parameter, term key = wavefield entry; A terms negated
+        std::vector<GT> gv;
+        gv.reserve(ob.b.size() + ob.a.size());
+        for (auto &t : ob.b)
+            if (t.tkey >= 0) gv.push_back({t.tkey, t.row, t.key, t.c});
+        for (auto &t : ob.a) gv.push_back({t.tkey, t.row, t.key, -t.c});
+        if (int rc = upload_csr(gv, op.grad)) return rc;
+        // fine: grouped by input entry, term key = output entry
+        std::vector<GT> fv = ob.fine;
+        std::vector<long> outs;
+        for (auto &t : fv) outs.push_back(t.tkey);
+        if (int rc = upload_csr(fv, op.fine)) return rc;
+        std::sort(outs.begin(), outs.end());
+        outs.erase(std::unique(outs.begin(), outs.end()), outs.end());
+        op.nz = (int)outs.size();
+        if (int rc = upload(&op.zkey, outs)) return rc;
+        op.nx = (int)ob.xkey.size();
+        if (int rc = upload(&op.xkey, ob.xkey)) return rc;
+        if (int rc = upload(&op.xrow, ob.xrow)) return rc;
+        return ECSG_OK;
+    }
+
+    // term lists of interface k (same loops and coefficients as k_gvz_rhs,
+    // k_gvz_fine, k_gsxz_rhs, the Jacobi scatter, k_gsyz and build_iface)
+    int build_adj_iface(int k)
+    {
+        const DevGrid<T> &F = G[k], &C = G[k + 1];
+        const DevIface &f = I[k];
+        AdjIface &ai = AI[k];
+        const int r = f.r, nx2 = f.nx2, ny2 = f.ny2, nz1 = f.nz1;
+        const int nx1 = r * nx2, ny1 = r * ny2;
+        const double k1x = (double)F.dev.cx, k1y = (double)F.dev.cy, k1z = (double)F.dev.cz;
+        const double k2x = (double)C.dev.cx, k2y = (double)C.dev.cy, k2z = (double)C.dev.cz;
+        auto iF = [&](int i, int j, int z) { return F.o + (long)z * F.S2 + (long)j * F.P + i; };
+        auto iC = [&](int i, int j, int z) { return C.o + (long)z * C.S2 + (long)j * C.P + i; };
+        enum { VX = 0, VY, VZ, SXX, SYY, SZZ, SYZ, SXZ, SXY };
+        enum { PC33 = 2, PC23 = 3, PC13 = 4, PBN = 9, PBY = 10 };
+        const int SF = 0, SC = 1;
+        const double r2 = (double)(r * r), r3 = r2 * r, r4 = r2 * r2;
+        // ---- vz-szz: rows (J-1)*nx2 + I, tridiagonal in J
+        {
+            OpBuild ob;
+            for (int J = 1; J <= ny2 - 1; J++)
+                for (int Ii = 0; Ii < nx2; Ii++) {
+                    const int q = (J - 1) * nx2 + Ii;
+                    const long qc = iC(Ii, J, 0);
+                    ob.B(key(SC, SZZ, qc), -1, q, r3);
+                    ob.B(key(SC, VX, qc + 1), key(SC, PC13, qc), q, r3 * k2x);
+                    ob.B(key(SC, VX, qc), key(SC, PC13, qc), q, -r3 * k2x);
+                    ob.B(key(SC, VY, qc), key(SC, PC23, qc), q, r3 * k2y);
+                    ob.B(key(SC, VY, qc - C.P), key(SC, PC23, qc), q, -r3 * k2y);
+                    ob.B(key(SC, VZ, qc), key(SC, PC33, qc), q, r3 * k2z);
+                    ob.A(key(SC, VZ, iC(Ii, J, -1)), key(SC, PC33, qc), q, r3 * k2z);
+                    for (int bo = -(r - 1); bo <= r - 1; bo++) {
+                        const int j = r * J + bo;
+                        const double wj = (double)(r - std::abs(bo));
+                        const int Jl = (bo >= 0) ? J : J - 1;
+                        const int bb = j - r * Jl;
+                        for (int ao = 0; ao < r; ao++) {
+                            const int i = r * Ii + ao;
+                            const long qf = iF(i, j, nz1);
+                            const long pc = key(SF, PC33, qf);
+                            ob.B(key(SF, SZZ, qf), -1, q, -wj);
+                            ob.B(key(SF, VX, qf + 1), key(SF, PC13, qf), q, -wj * k1x);
+                            ob.B(key(SF, VX, qf), key(SF, PC13, qf), q, wj * k1x);
+                            ob.B(key(SF, VY, qf), key(SF, PC23, qf), q, -wj * k1y);
+                            ob.B(key(SF, VY, qf - F.P), key(SF, PC23, qf), q, wj * k1y);
+                            ob.B(key(SC, VZ, iC(Ii, Jl, 0)), pc, q, -wj * k1z * hat(bb, r));
+                            ob.B(key(SC, VZ, iC(Ii, Jl + 1, 0)), pc, q, -wj * k1z * hat(bb - r, r));
+                            ob.B(key(SF, VZ, qf - F.S2), pc, q, 2 * wj * k1z);
+                            ob.A(key(SC, VZ, iC(Ii, J, -1)), pc, q, k1z * wj * hat(bo, r));
+                            if (bo < 0 && J - 1 >= 1) ob.A(key(SC, VZ, iC(Ii, J - 1, -1)), pc, q, k1z * wj * hat(bo + r, r));
+                            if (bo > 0 && J + 1 <= ny2 - 1) ob.A(key(SC, VZ, iC(Ii, J + 1, -1)), pc, q, k1z * wj * hat(bo - r, r));
+                        }
+                    }
+                    ob.xkey.push_back(key(SC, VZ, iC(Ii, J, -1)));
+                    ob.xrow.push_back(q);
+                }
+            for (int j = 1; j <= ny1 - 1; j++)
+                for (int i = 0; i < nx1; i++) {
+                    const int Ii = i / r, J = j / r, bo = j - r * J;
+                    const long out = key(SF, VZ, iF(i, j, nz1));
+                    ob.F(key(SF, VZ, iF(i, j, nz1 - 1)), out, -1.0);
+                    const double c0 = bo == 0 ? 1.0 : hat(bo, r);
+                    ob.F(key(SC, VZ, iC(Ii, J, 0)), out, c0);
+                    ob.F(key(SC, VZ, iC(Ii, J, -1)), out, c0);
+                    if (bo != 0) {
+                        ob.F(key(SC, VZ, iC(Ii, J + 1, 0)), out, hat(bo - r, r));
+                        ob.F(key(SC, VZ, iC(Ii, J + 1, -1)), out, hat(bo - r, r));
+                    }
+                }
+            if (int rc = upload_op(ob, ai.op[0])) return rc;
+        }
+        // ---- vx-sxz: rows J*(nx2+1) + I of the iterate, rhs rows (J-1)*(nx2-1) + I-1
+        {
+            OpBuild ob;
+            const long w = nx2 + 1;
+            const int n0 = nx2 - 1;
+            for (int J = 1; J <= ny2 - 1; J++)
+                for (int Ii = 1; Ii <= nx2 - 1; Ii++) {
+                    const int row = (int)(J * w + Ii);
+                    const long qc = iC(Ii, J, 0);
+                    const long pb = key(SC, PBN, qc);
+                    ob.B(key(SC, VX, qc), -1, row, r4);
+                    ob.B(key(SC, SXX, qc), pb, row, r4 * k2x);
+                    ob.B(key(SC, SXX, qc - 1), pb, row, -r4 * k2x);
+                    ob.B(key(SC, SXY, qc), pb, row, r4 * k2y);
+                    ob.B(key(SC, SXY, qc - C.P), pb, row, -r4 * k2y);
+                    ob.B(key(SC, SXZ, qc), pb, row, r4 * k2z);
+                    ob.A(key(SC, SXZ, qc - C.S2), pb, row, r4 * k2z);
+                    for (int bo = -(r - 1); bo <= r - 1; bo++)
+                        for (int ao = -(r - 1); ao <= r - 1; ao++) {
+                            const int i = r * Ii + ao, j = r * J + bo;
+                            const double wgt = (double)((r - std::abs(ao)) * (r - std::abs(bo)));
+                            const long qf = iF(i, j, nz1);
+                            const long pf = key(SF, PBN, qf);
+                            ob.B(key(SF, VX, qf), -1, row, -wgt);
+                            ob.B(key(SF, SXX, qf), pf, row, -wgt * k1x);
+                            ob.B(key(SF, SXX, qf - 1), pf, row, wgt * k1x);
+                            ob.B(key(SF, SXY, qf), pf, row, -wgt * k1y);
+                            ob.B(key(SF, SXY, qf - F.P), pf, row, wgt * k1y);
+                            ob.B(key(SF, SXZ, qf - F.S2), pf, row, 2 * wgt * k1z);
+                            for (int dJ = -1; dJ <= 1; dJ++)
+                                for (int dI = -1; dI <= 1; dI++) {
+                                    const int Ic = Ii + dI, Jc = J + dJ;
+                                    const int ai2 = i - r * Ic, bj = j - r * Jc;
+                                    if (std::abs(ai2) >= r || std::abs(bj) >= r) continue;
+                                    if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
+                                    const double p = hat(ai2, r) * hat(bj, r);
+                                    ob.B(key(SC, SXZ, iC(Ic, Jc, 0)), pf, row, -wgt * k1z * p);
+                                    ob.A(key(SC, SXZ, iC(Ic, Jc, -1)), pf, row, k1z * wgt * p);
+                                }
+                        }
+                    ob.xkey.push_back(key(SC, SXZ, qc - C.S2));
+                    ob.xrow.push_back((J - 1) * n0 + (Ii - 1));
+                }
+            for (int j = 1; j < ny1; j++)
+                for (int i = 1; i < nx1; i++) {
+                    const int I0 = i / r, J0 = j / r, ao = i - r * I0, bo = j - r * J0;
+                    const long out = key(SF, SXZ, iF(i, j, nz1));
+                    ob.F(key(SF, SXZ, iF(i, j, nz1 - 1)), out, -1.0);
+                    for (int dJ = 0; dJ <= 1; dJ++)
+                        for (int dI = 0; dI <= 1; dI++) {
+                            const int Ic = I0 + dI, Jc = J0 + dJ;
+                            if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
+                            const double pw = hat(ao - r * dI, r) * hat(bo - r * dJ, r);
+                            if (pw == 0) continue;
+                            ob.F(key(SC, SXZ, iC(Ic, Jc, -1)), out, pw);
+                            ob.F(key(SC, SXZ, iC(Ic, Jc, 0)), out, pw);
+                        }
+                }
+            if (int rc = upload_op(ob, ai.op[1])) return rc;
+        }
+        // ---- vy-syz: pointwise, rows J*nx2 + I
+        {
+            OpBuild ob;
+            for (int J = 0; J < ny2; J++)
+                for (int Ii = 0; Ii < nx2; Ii++) {
+                    const int q = J * nx2 + Ii;
+                    const long qc = iC(Ii, J, 0);
+                    const long pc = key(SC, PBY, qc);
+                    ob.B(key(SC, VY, qc), -1, q, r2);
+                    ob.B(key(SC, SXY, qc + 1), pc, q, r2 * k2x);
+                    ob.B(key(SC, SXY, qc), pc, q, -r2 * k2x);
+                    ob.B(key(SC, SYY, qc + C.P), pc, q, r2 * k2y);
+                    ob.B(key(SC, SYY, qc), pc, q, -r2 * k2y);
+                    ob.B(key(SC, SYZ, qc), pc, q, r2 * k2z);
+                    ob.A(key(SC, SYZ, qc - C.S2), pc, q, r2 * k2z);
+                    for (int bo = 0; bo < r; bo++)
+                        for (int ao = 0; ao < r; ao++) {
+                            const long qf = iF(r * Ii + ao, r * J + bo, nz1);
+                            const long pf = key(SF, PBY, qf);
+                            ob.B(key(SF, VY, qf), -1, q, -1.0);
+                            ob.B(key(SF, SXY, qf + 1), pf, q, -k1x);
+                            ob.B(key(SF, SXY, qf), pf, q, k1x);
+                            ob.B(key(SF, SYY, qf + F.P), pf, q, -k1y);
+                            ob.B(key(SF, SYY, qf), pf, q, k1y);
+                            ob.B(key(SC, SYZ, qc), pf, q, -k1z);
+                            ob.B(key(SF, SYZ, qf - F.S2), pf, q, 2 * k1z);
+                            ob.A(key(SC, SYZ, qc - C.S2), pf, q, k1z);
+                            const long out = key(SF, SYZ, qf);
+                            ob.F(key(SF, SYZ, qf - F.S2), out, -1.0);
+                            ob.F(key(SC, SYZ, qc - C.S2), out, 1.0);
+                            ob.F(key(SC, SYZ, qc), out, 1.0);
+                        }
+                    ob.xkey.push_back(key(SC, SYZ, qc - C.S2));
+                    ob.xrow.push_back(q);
+                }
+            if (int rc = upload_op(ob, ai.op[2])) return rc;
+        }
+        const long nl = (long)nx2 * (ny2 - 1), nn = (long)(nx2 - 1) * (ny2 - 1);
+        const long nxw = (long)(nx2 + 1) * (ny2 + 1);
+        if (int rc = dalloc(&ai.mu_vz, nl)) return rc;
+        if (int rc = dalloc(&ai.mu_syz, (long)nx2 * ny2)) return rc;
+        if (int rc = dalloc(&ai.den, (long)nx2 * ny2)) return rc;
+        if (int rc = dalloc(&ai.Ba, nn)) return rc;
+        if (int rc = dalloc(&ai.X0a, nxw)) return rc;
+        if (int rc = dalloc(&ai.X1a, nxw)) return rc;
+        if (int rc = dalloc(&ai.absz, f.nabs)) return rc;
+        if (int rc = dalloc(&ai.jpart, (size_t)2 * f.tiles * 2 * SJ + 1)) return rc;
+        if (int rc = dalloc(&ai.stats, 3)) return rc;
+        return ECSG_OK;
+    }
+
+    // transposed ghost mirrors of grid g (the mirror loops of k_vel and k_stress)
+    int upload_mirror(std::vector<std::pair<long, long>> &v, MirrorDev &m)
+    {
+        std::stable_sort(v.begin(), v.end(), [](const std::pair<long, long> &a, const std::pair<long, long> &b) { return a.first < b.first; });
+        std::vector<long> src, halo;
+        std::vector<int> st;
+        for (size_t i = 0; i < v.size(); i++) {
+            if (i == 0 || v[i].first != v[i - 1].first) { src.push_back(v[i].first); st.push_back((int)i); }
+            halo.push_back(v[i].second);
+        }
+        st.push_back((int)v.size());
+        m.n = (int)src.size();
+        m.nh = (int)halo.size();
+        if (int rc = upload(&m.src, src)) return rc;
+        if (int rc = upload(&m.start, st)) return rc;
+        if (int rc = upload(&m.halo, halo)) return rc;
+        return ECSG_OK;
+    }
+
+    int build_adj_grid(int g)
+    {
+        DevGrid<T> &d = G[g];
+        AdjGrid &a = AG[g];
+        const int nx = d.n[0], ny = d.n[1], nz = d.n[2];
+        const long as = d.astride;
+        auto ix = [&](int f, int i, int j, int k) { return (long)f * as + d.o + (long)k * d.S2 + (long)j * d.P + i; };
+        enum { VY = 1, VZ = 2, SYZ = 6, SXZ = 7 };
+        std::vector<std::pair<long, long>> A, B, S;  // (source, halo)
+        for (int k = 0; k <= d.dev.zy; k++)
+            for (int i = 0; i <= nx - 1; i++) {
+                A.push_back({ix(VY, i, 0, k), ix(VY, i, -1, k)});
+                A.push_back({ix(VY, i, ny - 1, k), ix(VY, i, ny, k)});
+            }
+        if (d.dev.top_eff)
+            for (int j = 0; j <= ny; j++)
+                for (int i = 0; i <= nx - 1; i++) B.push_back({ix(VZ, i, j, 0), ix(VZ, i, j, -1)});
+        if (d.dev.bottom)
+            for (int j = 0; j <= ny; j++)
+                for (int i = 0; i <= nx - 1; i++) B.push_back({ix(VZ, i, j, nz - 1), ix(VZ, i, j, nz)});
+        for (int k = 0; k <= nz - 1; k++)
+            for (int j = 0; j <= ny; j++) {
+                B.push_back({ix(VZ, 0, j, k), ix(VZ, -1, j, k)});
+                B.push_back({ix(VZ, nx - 1, j, k), ix(VZ, nx, j, k)});
+            }
+        for (int k = 0; k <= nz; k++)
+            for (int j = 0; j <= ny; j++) {
+                B.push_back({ix(VY, 0, j, k), ix(VY, -1, j, k)});
+                B.push_back({ix(VY, nx - 1, j, k), ix(VY, nx, j, k)});
+            }
+        if (d.dev.free_top) {
+            for (int j = 0; j <= ny - 1; j++)
+                for (int i = 0; i <= nx - 1; i++) S.push_back({ix(SYZ, i, j, 0), ix(SYZ, i, j, -1)});
+            for (int j = 1; j <= ny - 1; j++)
+                for (int i = 1; i <= nx - 1; i++) S.push_back({ix(SXZ, i, j, 0), ix(SXZ, i, j, -1)});
+        }
+        if (int rc = upload_mirror(A, a.vA)) return rc;
+        if (int rc = upload_mirror(B, a.vB)) return rc;
+        if (int rc = upload_mirror(S, a.sm)) return rc;
+        if (int rc = dalloc(&a.lam, (size_t)9 * as)) return rc;
+        if (int rc = dalloc(&a.grad, (size_t)12 * as)) return rc;
+        return ECSG_OK;
+    }
+
+    int build_adj_receivers()
+    {
+        dfree(d_rloc); dfree(d_rstart); dfree(d_rrec);
+        std::vector<int> loc4((size_t)4 * std::max(nrec, 1));
+        if (nrec) HIPCHK(hipMemcpy(loc4.data(), d_rec, sizeof(int) * 4 * nrec, hipMemcpyDeviceToHost));
+        std::vector<std::pair<long, int>> v;
+        for (int q = 0; q < nrec; q++) {
+            const int g = loc4[4 * q];
+            const DevGrid<T> &d = G[g];
+            const long idx = d.o + (long)loc4[4 * q + 3] * d.S2 + (long)loc4[4 * q + 2] * d.P + loc4[4 * q + 1];
+            v.push_back({((long)g << 40) | idx, q});
+        }
+        std::stable_sort(v.begin(), v.end(), [](const std::pair<long, int> &a, const std::pair<long, int> &b) { return a.first < b.first; });
+        std::vector<long> uk;
+        std::vector<int> st, rc2;
+        for (size_t i = 0; i < v.size(); i++) {
+            if (i == 0 || v[i].first != v[i - 1].first) { uk.push_back(v[i].first); st.push_back((int)i); }
+            rc2.push_back(v[i].second);
+        }
+        st.push_back((int)v.size());
+        nrloc = (int)uk.size();
+        if (int rc = upload(&d_rloc, uk)) return rc;
+        if (int rc = upload(&d_rstart, st)) return rc;
+        if (int rc = upload(&d_rrec, rc2)) return rc;
+        return ECSG_OK;
+    }
+
+    int adj_build()
+    {
+        if (adj_ready) return ECSG_OK;
+        if (int rc = check_ready()) return rc;
+        adj_free();
+        AG.resize(G.size());
+        AI.resize(I.size());
+        for (int g = 0; g < (int)G.size(); g++)
+            if (int rc = build_adj_grid(g)) return rc;
+        for (int k = 0; k < (int)I.size(); k++)
+            if (int rc = build_adj_iface(k)) return rc;
+        adj_ready = true;
+        return ECSG_OK;
+    }
+
+    const T *fview(int slot, int g, int f) const
+    {
+        return slot < 0 ? (const T *)G[g].arr(f) : ring[(size_t)slot * G.size() + g] + (long)f * G[g].astride;
+    }
+
+    TLDev<T> tl_dev(int k, int slot_v, int slot_s)
+    {
+        TLDev<T> d{};
+        for (int side = 0; side < 2; side++) {
+            const int g = k + side;
+            d.lam[side] = AG[g].lam;
+            d.astr[side] = G[g].astride;
+            for (int f = 0; f < 9; f++) d.fwd[side][f] = fview(f < 3 ? slot_v : slot_s, g, f);
+            d.par[side] = G[g].arr(9);
+            d.grad[side] = AG[g].grad;
+            d.P[side] = G[g].P;
+            d.S2[side] = G[g].S2;
+        }
+        return d;
+    }
+
+    AdjGridDev<T> adj_dev(int g, int slot_v, int slot_s)
+    {
+        const DevGrid<T> &d = G[g];
+        AdjGridDev<T> a{};
+        for (int f = 0; f < 9; f++) a.lam[f] = AG[g].lam + (long)f * d.astride;
+        for (int f = 0; f < 3; f++) a.fv[f] = fview(slot_v, g, f);
+        for (int f = 0; f < 6; f++) a.fs[f] = fview(slot_s, g, 3 + f);
+        for (int m = 0; m < 9; m++) a.gc[m] = AG[g].grad + (long)m * d.astride;
+        for (int m = 0; m < 3; m++) a.gb[m] = AG[g].grad + (long)(9 + m) * d.astride;
+        for (int m = 0; m < 9; m++) a.c[m] = d.dev.c[m];
+        a.b = d.dev.b;
+        a.nx = d.n[0]; a.ny = d.n[1]; a.nz = d.n[2];
+        a.P = d.P; a.S2 = d.S2; a.o = d.o;
+        a.cx = d.dev.cx; a.cy = d.dev.cy; a.cz = d.dev.cz;
+        a.top_eff = d.dev.top_eff; a.bottom = d.dev.bottom; a.free_top = d.dev.free_top;
+        a.gx = (d.n[0] + 2 + BX - 1) / BX;
+        a.gy = (d.n[1] + 2 + ABY - 1) / ABY;
+        a.gz = (d.n[2] + 2 + AZC - 1) / AZC;
+        return a;
+    }
+
+    static int nb(long n) { return (int)((n + 255) / 256); }
+
+    void launch_mirror(int g, const MirrorDev &m)
+    {
+        if (!m.n) return;
+        hipLaunchKernelGGL((k_mirror_T<T>), dim3(nb(m.n)), dim3(256), 0, s, AG[g].lam, m.src, m.start, m.halo, m.n);
+        hipLaunchKernelGGL((k_zero_list<T>), dim3(nb(m.nh)), dim3(256), 0, s, AG[g].lam, m.halo, m.nh);
+    }
+
+    void launch_op_pre(DevOp &op, const TLDev<T> &d, double *rhs, const double *den)
+    {
+        if (op.fine.n)
+            hipLaunchKernelGGL((k_tl_fine<T>), dim3(nb(op.fine.n)), dim3(256), 0, s, d, op.fine.ukey, op.fine.start, op.fine.tkey,
+                               op.fine.cst, op.fine.n);
+        if (op.nz) hipLaunchKernelGGL((k_tl_zero<T>), dim3(nb(op.nz)), dim3(256), 0, s, d, op.zkey, op.nz);
+        if (op.nx) hipLaunchKernelGGL((k_tl_readx<T>), dim3(nb(op.nx)), dim3(256), 0, s, d, op.xkey, op.xrow, den, rhs, op.nx);
+    }
+
+    void launch_op_post(DevOp &op, const TLDev<T> &d, const double *mu)
+    {
+        if (op.prop.n)
+            hipLaunchKernelGGL((k_tl_prop<T>), dim3(nb(op.prop.n)), dim3(256), 0, s, d, op.prop.ukey, op.prop.start, op.prop.row,
+                               op.prop.tkey, op.prop.cst, mu, op.prop.n);
+        if (op.grad.n)
+            hipLaunchKernelGGL((k_tl_grad<T>), dim3(nb(op.grad.n)), dim3(256), 0, s, d, op.grad.ukey, op.grad.start, op.grad.row,
+                               op.grad.tkey, op.grad.cst, mu, op.grad.n);
+    }
+
+    int launch_jacobi_adj(int k0, int n)
+    {
+        JacBatch<T> jb{};
+        int blk = 0;
+        for (int q = 0; q < n; q++) {
+            DevIface &f = I[k0 + q];
+            AdjIface &a = AI[k0 + q];
+            JacItf<T> &J = jb.it[q];
+            J.A = f.A; J.B = a.Ba; J.X0 = a.X0a; J.X1 = a.X1a; J.abs_part = a.absz; J.nabs = f.nabs;
+            J.part = a.jpart; J.stats = a.stats;
+            J.sxz1 = nullptr; J.sxz2 = nullptr;
+            J.nx2 = f.nx2; J.ny2 = f.ny2; J.nz1 = f.nz1; J.r = f.r;
+            J.blk0 = blk; J.nblk = f.tiles; J.rho2 = 0;
+            blk += J.nblk;
+        }
+        jb.n = n;
+        jb.tol_cap = 0; jb.tol_fac = 0;
+        jb.floor = 1024 * 2.220446049250313e-16;
+        jb.rel_tol = adj_rel_tol;
+        jb.cap = JACOBI_CAP;
+        jb.cheby = 0;
+        jb.solve_only = 1;
+        if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+        void *args[] = {(void *)&jb};
+        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JT), args, 0, s));
+        return ECSG_OK;
+    }
+
+    // One transposed step: forward step n read from ring slot slot_f (state
+    // after the step) and slot_p (state before it); -1 = the live state.
+    int adj_step_impl(int n, int slot_f, int slot_p, bool inject)
+    {
+        const int ng = (int)G.size(), ni = (int)I.size();
+        for (int k = ni - 1; k >= 0; k--) {
+            TLDev<T> d = tl_dev(k, slot_f, slot_f);
+            launch_op_pre(AI[k].op[2], d, AI[k].mu_syz, AI[k].den);
+            launch_op_post(AI[k].op[2], d, AI[k].mu_syz);
+        }
+        if (ni) {
+            for (int k = ni - 1; k >= 0; k--) launch_op_pre(AI[k].op[1], tl_dev(k, slot_f, slot_f), AI[k].Ba, nullptr);
+            for (int k0 = 0; k0 < ni; k0 += MAXI)
+                if (int rc = launch_jacobi_adj(k0, std::min(MAXI, ni - k0))) return rc;
+            for (int k = ni - 1; k >= 0; k--) launch_op_post(AI[k].op[1], tl_dev(k, slot_f, slot_f), AI[k].X0a);
+        }
+        for (int g = ng - 1; g >= 0; g--) {
+            launch_mirror(g, AG[g].sm);
+            AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
+            hipLaunchKernelGGL((k_adj_stress<T>), dim3(a.gx * a.gy * a.gz), dim3(BX, ABY), 0, s, a);
+        }
+        for (int k = ni - 1; k >= 0; k--) {
+            TLDev<T> d = tl_dev(k, slot_f, slot_p);
+            launch_op_pre(AI[k].op[0], d, AI[k].mu_vz, nullptr);
+            hipLaunchKernelGGL(k_tri_solve, dim3((I[k].nx2 + 63) / 64), dim3(64), 0, s, AI[k].mu_vz, I[k].M, I[k].Dp, I[k].U,
+                               I[k].nx2, I[k].ny2 - 1);
+            launch_op_post(AI[k].op[0], d, AI[k].mu_vz);
+        }
+        if (inject && nrloc) {
+            RecAdj<T> ra{};
+            for (int g = 0; g < ng; g++) ra.lamvx[g] = AG[g].lam;
+            ra.loc = d_rloc; ra.start = d_rstart; ra.rec = d_rrec; ra.resid = d_resid; ra.nloc = nrloc; ra.nsteps = resid_steps;
+            hipLaunchKernelGGL((k_adj_rec<T>), dim3(nb(nrloc)), dim3(256), 0, s, ra, n);
+        }
+        for (int g = ng - 1; g >= 0; g--) {
+            launch_mirror(g, AG[g].vA);
+            launch_mirror(g, AG[g].vB);
+            AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
+            hipLaunchKernelGGL((k_adj_vel<T>), dim3(a.gx * a.gy * a.gz), dim3(BX, ABY), 0, s, a);
+        }
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+    int resid_steps = 0;
+
+    int adj_zero()
+    {
+        for (int g = 0; g < (int)G.size(); g++) {
+            HIPCHK(hipMemsetAsync(AG[g].lam, 0, sizeof(T) * 9 * G[g].astride, s));
+            HIPCHK(hipMemsetAsync(AG[g].grad, 0, sizeof(T) * 12 * G[g].astride, s));
+        }
+        for (int k = 0; k < (int)I.size(); k++) {
+            const long nxw = (long)(I[k].nx2 + 1) * (I[k].ny2 + 1);
+            HIPCHK(hipMemsetAsync(AI[k].X0a, 0, sizeof(double) * nxw, s));
+            HIPCHK(hipMemsetAsync(AI[k].X1a, 0, sizeof(double) * nxw, s));
+            HIPCHK(hipMemsetAsync(AI[k].stats, 0, sizeof(long long) * 3, s));
+            const long nc = (long)I[k].nx2 * I[k].ny2;
+            hipLaunchKernelGGL((k_gsyz_den<T>), dim3(nb(nc)), dim3(256), 0, s, iface_dev(k), AI[k].den);
+        }
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
+    int reset() override
+    {
+        for (auto &d : G) HIPCHK(hipMemsetAsync(d.base, 0, sizeof(T) * 9 * d.astride, s));
+        for (auto &f : I) {
+            const long nxw = (long)(f.nx2 + 1) * (f.ny2 + 1);
+            HIPCHK(hipMemsetAsync(f.X0, 0, sizeof(double) * nxw, s));
+            HIPCHK(hipMemsetAsync(f.X1, 0, sizeof(double) * nxw, s));
+            HIPCHK(hipMemsetAsync(f.stats, 0, sizeof(long long) * 3, s));
+        }
+        if (d_traces) HIPCHK(hipMemsetAsync(d_traces, 0, sizeof(T) * std::max(nrec, 1) * cfg.max_steps, s));
+        HIPCHK(hipMemsetAsync(d_energy, 0, sizeof(double) * cfg.max_steps, s));
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, d_step, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        host_step = 0;
+        return ECSG_OK;
+    }
+
+    int set_checkpointing(int every, int format) override
+    {
+        if (every < 1) return fail(ECSG_EINVAL, "checkpoint interval must be >= 1");
+        if (format < 0 || format > 2) return fail(ECSG_EINVAL, "checkpoint format must be 0 (native), 1 (fp16) or 2 (bf16)");
+        ck_every = every;
+        ck_fmt = format;
+        return ECSG_OK;
+    }
+
+    int ck_prepare(int nck)
+    {
+        const int ng = (int)G.size(), ni = (int)I.size();
+        if (nck > ck_alloc || ck_fmt != ck_alloc_fmt || ck_every + 1 > ring_alloc) {
+            ck_free();
+            for (int j = 0; j < nck; j++) {
+                for (int g = 0; g < ng; g++) {
+                    if (ck_fmt == 0) {
+                        T *p = nullptr;
+                        if (int rc = dalloc(&p, (size_t)9 * G[g].astride)) return rc;
+                        ck_nat.push_back(p);
+                    } else {
+                        unsigned short *p = nullptr;
+                        if (int rc = dalloc(&p, (size_t)9 * G[g].astride)) return rc;
+                        ck_red.push_back(p);
+                    }
+                }
+                for (int k = 0; k < ni; k++) {
+                    double *p = nullptr;
+                    if (int rc = dalloc(&p, (size_t)(I[k].nx2 + 1) * (I[k].ny2 + 1))) return rc;
+                    ck_x0.push_back(p);
+                }
+            }
+            if (int rc = dalloc(&ck_scale, (size_t)nck * ng * 9)) return rc;
+            if (int rc = dalloc(&ck_part, (size_t)CKP)) return rc;
+            for (int q = 0; q <= ck_every; q++)
+                for (int g = 0; g < ng; g++) {
+                    T *p = nullptr;
+                    if (int rc = dalloc(&p, (size_t)9 * G[g].astride)) return rc;
+                    ring.push_back(p);
+                }
+            ck_alloc = nck;
+            ck_alloc_fmt = ck_fmt;
+            ring_alloc = ck_every + 1;
+        }
+        return ECSG_OK;
+    }
+
+    int ck_save(int j)
+    {
+        const int ng = (int)G.size(), ni = (int)I.size();
+        for (int g = 0; g < ng; g++) {
+            const long as = G[g].astride;
+            if (ck_fmt == 0) {
+                HIPCHK(hipMemcpyAsync(ck_nat[(size_t)j * ng + g], G[g].base, sizeof(T) * 9 * as, hipMemcpyDeviceToDevice, s));
+            } else {
+                for (int f = 0; f < 9; f++) {
+                    const T *x = G[g].arr(f);
+                    float *sc = ck_scale + ((size_t)j * ng + g) * 9 + f;
+                    hipLaunchKernelGGL((k_maxabs<T>), dim3(CKP), dim3(256), 0, s, x, as, ck_part);
+                    hipLaunchKernelGGL(k_ck_scale, dim3(1), dim3(64), 0, s, (const float *)ck_part, CKP, sc);
+                    hipLaunchKernelGGL((k_ck_pack<T>), dim3(1024), dim3(256), 0, s, x, as, ck_red[(size_t)j * ng + g] + (long)f * as,
+                                       (const float *)sc, ck_fmt);
+                }
+            }
+        }
+        for (int k = 0; k < ni; k++)
+            HIPCHK(hipMemcpyAsync(ck_x0[(size_t)j * ni + k], I[k].X0, sizeof(double) * (I[k].nx2 + 1) * (I[k].ny2 + 1),
+                                  hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
+    int ck_restore(int j, int step)
+    {
+        const int ng = (int)G.size(), ni = (int)I.size();
+        for (int g = 0; g < ng; g++) {
+            const long as = G[g].astride;
+            if (ck_fmt == 0) {
+                HIPCHK(hipMemcpyAsync(G[g].base, ck_nat[(size_t)j * ng + g], sizeof(T) * 9 * as, hipMemcpyDeviceToDevice, s));
+            } else {
+                for (int f = 0; f < 9; f++)
+                    hipLaunchKernelGGL((k_ck_unpack<T>), dim3(1024), dim3(256), 0, s, G[g].arr(f), as,
+                                       (const unsigned short *)(ck_red[(size_t)j * ng + g] + (long)f * as),
+                                       (const float *)(ck_scale + ((size_t)j * ng + g) * 9 + f), ck_fmt);
+            }
+        }
+        for (int k = 0; k < ni; k++)
+            HIPCHK(hipMemcpyAsync(I[k].X0, ck_x0[(size_t)j * ni + k], sizeof(double) * (I[k].nx2 + 1) * (I[k].ny2 + 1),
+                                  hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, d_step, step);
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
+    int ring_save(int slot)
+    {
+        const int ng = (int)G.size();
+        for (int g = 0; g < ng; g++)
+            HIPCHK(hipMemcpyAsync(ring[(size_t)slot * ng + g], G[g].base, sizeof(T) * 9 * G[g].astride, hipMemcpyDeviceToDevice, s));
+        return ECSG_OK;
+    }
+
+    int gradient(int N, const double *obs, double *misfit) override
+    {
+        if (int rc = check_ready()) return rc;
+        if (N < 1 || N > cfg.max_steps) return fail(ECSG_EINVAL, "gradient: step count out of range");
+        if (nrec > 0 && !obs) return fail(ECSG_EINVAL, "gradient: null observed traces");
+        if (int rc = adj_build()) return rc;
+        if (int rc = build_adj_receivers()) return rc;
+        const int K = ck_every;
+        const int nseg = (N + K - 1) / K;
+        if (int rc = ck_prepare(nseg)) return rc;
+        if (int rc = reset()) return rc;
+        // forward pass with checkpoints
+        for (int j = 0; j < nseg; j++) {
+            if (int rc = ck_save(j)) return rc;
+            const int len = std::min(K, N - j * K);
+            for (int t = 0; t < len; t++)
+                if (int rc = enqueue_step()) return rc;
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        // residuals and misfit
+        std::vector<T> tr((size_t)std::max(nrec, 1) * cfg.max_steps);
+        if (nrec) HIPCHK(hipMemcpy(tr.data(), d_traces, sizeof(T) * nrec * cfg.max_steps, hipMemcpyDeviceToHost));
+        std::vector<T> res((size_t)std::max(nrec, 1) * N, (T)0);
+        double J = 0;
+        for (int q = 0; q < nrec; q++)
+            for (int t = 0; t < N; t++) {
+                const double rr = (double)tr[(size_t)q * cfg.max_steps + t] - obs[(size_t)q * N + t];
+                res[(size_t)q * N + t] = (T)rr;
+                J += 0.5 * rr * rr;
+            }
+        if (misfit) *misfit = J;
+        dfree(d_resid);
+        if (int rc = upload(&d_resid, res)) return rc;
+        resid_steps = N;
+        if (int rc = adj_zero()) return rc;
+        // backward over the segments
+        for (int j = nseg - 1; j >= 0; j--) {
+            const int len = std::min(K, N - j * K);
+            if (int rc = ck_restore(j, j * K)) return rc;
+            if (int rc = ring_save(0)) return rc;
+            for (int t = 0; t < len; t++) {
+                if (int rc = enqueue_step()) return rc;
+                if (int rc = ring_save(t + 1)) return rc;
+            }
+            for (int t = len - 1; t >= 0; t--)
+                if (int rc = adj_step_impl(j * K + t, t + 1, t, true)) return rc;
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        host_step = std::min(K, N);
+        for (int k = 0; k < (int)I.size(); k++) {
+            long long st3[3];
+            HIPCHK(hipMemcpy(st3, AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
+            if (st3[2]) return fail(ECSG_ENOCONV, "adjoint Jacobi iteration cap reached on interface " + std::to_string(k));
+        }
+        return finalize_gradient();
+    }
+
+    // transposed parameter staging: gradients at the staged positions ->
+    // gradients w.r.t. the primary-node inputs of set_params
+    int finalize_gradient()
+    {
+        const int ng = (int)G.size();
+        gprim.assign((size_t)ng * 10, std::vector<double>());
+        for (int g = 0; g < ng; g++) {
+            const DevGrid<T> &d = G[g];
+            const long as = d.astride;
+            std::vector<T> hg((size_t)12 * as);
+            HIPCHK(hipMemcpy(hg.data(), AG[g].grad, sizeof(T) * 12 * as, hipMemcpyDeviceToHost));
+            const int n0 = d.n[0], n1 = d.n[1], n2 = d.n[2];
+            const long px = n0 + 1, py = n1 + 1;
+            const long np = px * py * (n2 + 1);
+            auto PI = [&](int i, int j, int k) { return ((long)k * py + j) * px + i; };
+            auto DI = [&](int i, int j, int k) { return d.o + (long)k * d.S2 + (long)j * d.P + i; };
+            std::vector<std::vector<double>> gm(9, std::vector<double>((size_t)np, 0.0));
+            for (int m = 0; m < 9; m++) {
+                const int *pos = ST_POS[m];
+                const int e0 = pos[0] ? n0 - 1 : n0, e1 = pos[1] ? n1 - 1 : n1, e2 = pos[2] ? n2 - 1 : n2;
+                std::vector<double> &o = gm[m];
+                for (int k = 0; k <= e2; k++)
+                    for (int j = 0; j <= e1; j++)
+                        for (int i = 0; i <= e0; i++) {
+                            const double v = (double)hg[(size_t)m * as + DI(i, j, k)] / 8;
+                            if (v == 0) continue;
+                            for (int c = 0; c < 8; c++)
+                                o[PI(i + ((c >> 2) & 1) * pos[0], j + ((c >> 1) & 1) * pos[1], k + (c & 1) * pos[2])] += v;
+                        }
+            }
+            if (d.iso) {
+                std::vector<double> s11((size_t)np, 0.0), s13((size_t)np, 0.0);
+                for (long q = 0; q < np; q++) {
+                    s11[q] = gm[0][q] + gm[1][q] + gm[2][q] + gm[6][q] + gm[7][q] + gm[8][q];
+                    s13[q] = gm[3][q] + gm[4][q] + gm[5][q] - gm[6][q] - gm[7][q] - gm[8][q];
+                }
+                for (int m = 0; m < 9; m++) gprim[(size_t)g * 10 + 1 + m].assign((size_t)np, 0.0);
+                gprim[(size_t)g * 10 + 1 + 0] = s11;
+                gprim[(size_t)g * 10 + 1 + 4] = s13;
+            } else {
+                for (int m = 0; m < 9; m++) gprim[(size_t)g * 10 + 1 + m] = gm[m];
+            }
+            std::vector<double> gb((size_t)np, 0.0);
+            for (int k = 0; k <= n2; k++)
+                for (int j = 0; j <= n1; j++)
+                    for (int i = 0; i <= n0; i++) gb[PI(i, j, k)] += (double)hg[(size_t)9 * as + DI(i, j, k)];
+            for (int k = 0; k <= n2; k++)
+                for (int j = 0; j <= n1 - 1; j++)
+                    for (int i = 0; i <= n0 - 1; i++) {
+                        const double v = 0.25 * (double)hg[(size_t)10 * as + DI(i, j, k)];
+                        if (v == 0) continue;
+                        gb[PI(i, j, k)] += v; gb[PI(i + 1, j, k)] += v; gb[PI(i, j + 1, k)] += v; gb[PI(i + 1, j + 1, k)] += v;
+                    }
+            for (int k = 0; k <= n2 - 1; k++)
+                for (int j = 0; j <= n1; j++)
+                    for (int i = 0; i <= n0 - 1; i++) {
+                        const double v = 0.25 * (double)hg[(size_t)11 * as + DI(i, j, k)];
+                        if (v == 0) continue;
+                        gb[PI(i, j, k)] += v; gb[PI(i, j, k + 1)] += v; gb[PI(i + 1, j, k)] += v; gb[PI(i + 1, j, k + 1)] += v;
+                    }
+            gprim[(size_t)g * 10] = gb;
+        }
+        return ECSG_OK;
+    }
+
+    int get_gradient(int g, int sel, double *out) override
+    {
+        if (g < 0 || g >= (int)G.size()) return fail(ECSG_EINVAL, "grid index out of range");
+        if (sel < ECSG_BUOY || sel > ECSG_C11 + 8) return fail(ECSG_EINVAL, "selector must be ECSG_BUOY or ECSG_C11..ECSG_C11+8");
+        if (gprim.size() != G.size() * 10) return fail(ECSG_ESTATE, "no gradient computed yet");
+        const std::vector<double> &v = gprim[(size_t)g * 10 + (sel == ECSG_BUOY ? 0 : 1 + sel - ECSG_C11)];
+        std::copy(v.begin(), v.end(), out);
+        return ECSG_OK;
+    }
+
+    int adjoint_step() override
+    {
+        if (int rc = adj_build()) return rc;
+        for (int k = 0; k < (int)I.size(); k++) {
+            const long nc = (long)I[k].nx2 * I[k].ny2;
+            hipLaunchKernelGGL((k_gsyz_den<T>), dim3(nb(nc)), dim3(256), 0, s, iface_dev(k), AI[k].den);
+        }
+        if (int rc = adj_step_impl(0, -1, -1, false)) return rc;
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < (int)I.size(); k++) {
+            long long st3[3];
+            HIPCHK(hipMemcpy(st3, AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
+            if (st3[2]) return fail(ECSG_ENOCONV, "adjoint Jacobi iteration cap reached on interface " + std::to_string(k));
+        }
+        return ECSG_OK;
+    }
+
+    int adj_field(int g, int sel, double *out, const double *in) override
+    {
+        if (int rc = adj_build()) return rc;
+        int len[3];
+        const int r = dims(g, sel, len);
+        if (r < 0 || sel >= 9) return fail(ECSG_EINVAL, "adjoint fields are wavefields (0..8)");
+        DevGrid<T> &d = G[g];
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<T> buf((size_t)d.astride);
+        T *dev = AG[g].lam + (long)sel * d.astride;
+        HIPCHK(hipMemcpy(buf.data(), dev, sizeof(T) * d.astride, hipMemcpyDeviceToHost));
+        long q = 0;
+        for (int k = -r; k < len[2] - r; k++)
+            for (int j = -r; j < len[1] - r; j++)
+                for (int i = -r; i < len[0] - r; i++, q++) {
+                    T &x = buf[d.o + (long)k * d.S2 + (long)j * d.P + i];
+                    if (in) x = (T)in[q];
+                    if (out) out[q] = (double)x;
+                }
+        if (in) HIPCHK(hipMemcpy(dev, buf.data(), sizeof(T) * d.astride, hipMemcpyHostToDevice));
+        return ECSG_OK;
+    }
+
+    int adjoint_stats(long long *sweeps, double *unused) override
+    {
+        long long tot = 0;
+        for (int k = 0; k < (int)AI.size(); k++) {
+            long long st3[3];
+            HIPCHK(hipMemcpy(st3, AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
+            tot += st3[0];
+        }
+        if (sweeps) *sweeps = tot;
+        if (unused) *unused = 0;
+        return ECSG_OK;
+    }
 };
 
 }  // namespace
@@ -1136,6 +2064,42 @@ int ecsg_profile_kernels(ecsg_ctx *ctx, int nsteps, double ms_out[6], int count_
 {
     CTX_CHECK(ctx);
     return ret(ctx, ctx->e->profile(nsteps, ms_out, count_out));
+}
+
+int ecsg_reset(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->reset()); }
+int ecsg_set_checkpointing(ecsg_ctx *ctx, int every, int format)
+{
+    CTX_CHECK(ctx);
+    return ret(ctx, ctx->e->set_checkpointing(every, format));
+}
+int ecsg_gradient(ecsg_ctx *ctx, int nsteps, const double *observed, double *misfit)
+{
+    CTX_CHECK(ctx);
+    return ret(ctx, ctx->e->gradient(nsteps, observed, misfit));
+}
+int ecsg_get_gradient(ecsg_ctx *ctx, int g, int sel, double *out)
+{
+    CTX_CHECK(ctx);
+    if (!out) return ECSG_EINVAL;
+    return ret(ctx, ctx->e->get_gradient(g, sel, out));
+}
+int ecsg_adjoint_step(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->adjoint_step()); }
+int ecsg_get_adjoint_field(ecsg_ctx *ctx, int g, int sel, double *out)
+{
+    CTX_CHECK(ctx);
+    if (!out) return ECSG_EINVAL;
+    return ret(ctx, ctx->e->adj_field(g, sel, out, nullptr));
+}
+int ecsg_set_adjoint_field(ecsg_ctx *ctx, int g, int sel, const double *in)
+{
+    CTX_CHECK(ctx);
+    if (!in) return ECSG_EINVAL;
+    return ret(ctx, ctx->e->adj_field(g, sel, nullptr, in));
+}
+int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps)
+{
+    CTX_CHECK(ctx);
+    return ret(ctx, ctx->e->adjoint_stats(sweeps, nullptr));
 }
 
 }  // extern "C"

@@ -584,8 +584,10 @@ struct JacBatch {
     int n;
     double tol_cap, tol_fac;  // tol = min(tol_cap, tol_fac * AvgAbs(B))
     double floor;             // rounding-floor factor of the stagnation exit
+    double rel_tol;           // also stop when the update norm <= rel_tol * ||X|| (0: off)
     int cap;
     int cheby;                // 1: Chebyshev acceleration of the Jacobi sweeps
+    int solve_only;           // 1: leave the solution in X0 only (no ghost scatter)
 };
 
 // Jacobi2D + Norm2D (update_ghost_sxz.cpp:314-388) for all interfaces of a
@@ -968,6 +970,7 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
                 it_k[k]++;
                 if (!(norm > s_tol[k])) done_k[k] = true;
                 else if (it_k[k] > 1 && norm >= prev_k[k] && norm <= jb.floor * xnorm) done_k[k] = true;
+                else if (norm <= jb.rel_tol * xnorm) done_k[k] = true;
                 else if (it_k[k] >= jb.cap) done_k[k] = true;
                 prev_k[k] = norm;
                 if (done_k[k]) stop_k[k] = t + 1;
@@ -988,8 +991,16 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
         if (I <= n0 && Jg <= n1) {
             const double x = stash[stop - 1][p];
             J.X0[(long)Jg * w + I] = x;
-            J.sxz2[J.o2 - J.S22 + (long)Jg * J.P2 + I] = (T)x;
+            if (!jb.solve_only) J.sxz2[J.o2 - J.S22 + (long)Jg * J.P2 + I] = (T)x;
         }
+    }
+    if (jb.solve_only) {
+        if (lb == 0 && tid == 0) {
+            J.stats[0] += iters;
+            J.stats[1] = iters;
+            if (iters >= jb.cap) J.stats[2] = 1;
+        }
+        return;
     }
     grid.sync();
     // fine ghosts on fine interior nodes

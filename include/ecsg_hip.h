@@ -144,6 +144,46 @@ int ecsg_jacobi_stats(ecsg_ctx *ctx, int k, long long *total, int *last, int *ca
  * stencils, 4 = interface kernels, 5 = receivers/source/energy kernels. */
 int ecsg_profile_kernels(ecsg_ctx *ctx, int nsteps, double ms_out[6], int count_out[6]);
 
+/* --- Adjoint-state gradient.  No counterpart in the reference, which has no
+ * inversion path; built on the same time step as the entry points above. ---
+ *
+ * Misfit J = 1/2 sum_n sum_r (vx_r(n) - observed[r*nsteps + n])^2 over the
+ * first nsteps steps from the zero state, vx_r(n) as recorded by the
+ * receivers.  The gradient is taken w.r.t. the arrays given to
+ * ecsg_set_params (buoyancy and stiff[0..8] at primary nodes; for iso grids
+ * w.r.t. stiff[0] = s11 and stiff[4] = s13).  The forward pass stores a
+ * checkpoint of the whole state every `every` steps; the backward pass
+ * recomputes each segment from its checkpoint and runs the transposed steps
+ * (time-reversed transposed stencils, transposed interface solves, misfit
+ * injection at the receivers) over it. */
+enum { ECSG_CKPT_NATIVE = 0, ECSG_CKPT_FP16 = 1, ECSG_CKPT_BF16 = 2 };
+
+/* Zero every wavefield, interface warm start, trace and the step counter. */
+int ecsg_reset(ecsg_ctx *ctx);
+
+/* Checkpoint interval (default 50) and storage: native precision, or IEEE
+ * half / bfloat16 of each array divided by its maximum magnitude. */
+int ecsg_set_checkpointing(ecsg_ctx *ctx, int every, int format);
+
+/* Runs the forward pass and the adjoint pass; misfit receives J.  After the
+ * call the forward state is that of an intermediate step: ecsg_reset before
+ * stepping again. */
+int ecsg_gradient(ecsg_ctx *ctx, int nsteps, const double *observed, double *misfit);
+
+/* dJ/d(input) of the last ecsg_gradient: sel = ECSG_BUOY for buoyancy or
+ * ECSG_C11 + m for stiff[m]; out has (nx+1)(ny+1)(nz+1) values. */
+int ecsg_get_gradient(ecsg_ctx *ctx, int g, int sel, double *out);
+
+/* Test hooks: one transposed time step applied to the adjoint wavefields
+ * (no misfit injection; gradient terms use the current forward state), and
+ * transfer of the adjoint wavefields in the layout of ecsg_get_field. */
+int ecsg_adjoint_step(ecsg_ctx *ctx);
+int ecsg_get_adjoint_field(ecsg_ctx *ctx, int g, int sel, double *out);
+int ecsg_set_adjoint_field(ecsg_ctx *ctx, int g, int sel, const double *in);
+
+/* Total Jacobi sweeps of the transposed vx-sxz solves since the last gradient. */
+int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps);
+
 #ifdef __cplusplus
 }
 #endif
