@@ -80,6 +80,28 @@ def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_step
     return el, kern, stats, tr
 
 
+def adjoint_run(m, steps: int, every: int, fmt: str, precision: str = "f32"):
+    """Forward with checkpoints + adjoint (gradient of 1/2 sum traces^2, i.e.
+    zero observed data) over `steps` steps; the first call builds the
+    interface term lists and allocates, the second is timed."""
+    import ecsg_amd
+    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=steps)
+    sim.set_checkpointing(every, fmt)
+    obs = np.zeros((len(m.rec_loc), steps))
+    t0 = time.time()
+    sim.gradient(steps, obs)
+    sim.sync()
+    log("[bench] adjoint set-up and first gradient (%d steps) in %.1fs" % (steps, time.time() - t0))
+    s0 = sim.adjoint_sweeps()
+    t = time.perf_counter()
+    J = sim.gradient(steps, obs)
+    sim.sync()
+    el = time.perf_counter() - t
+    sweeps = sim.adjoint_sweeps()
+    sim.close()
+    return el, J, sweeps
+
+
 def cpu_baseline(m, steps: int):
     """The oracle (CPU restatement, OpenMP) on the same model for `steps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -136,6 +158,10 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--solver", default="jacobi", choices=["jacobi", "chebyshev"],
                     help="vx-sxz interface solver (jacobi = the reference's sweeps)")
+    ap.add_argument("--adjoint-steps", type=int, default=100,
+                    help="steps of the timed forward+adjoint gradient (0: skip)")
+    ap.add_argument("--ckpt-every", type=int, default=10)
+    ap.add_argument("--ckpt-format", default="native", choices=["native", "fp16", "bf16"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (scale factor for HBM byte counters)")
     args = ap.parse_args()
@@ -147,7 +173,7 @@ def main():
               "(see the sharding section of the design notes), no N>1 figure is produced", file=sys.stderr)
         sys.exit(2)
 
-    total_steps = args.warmup + max(args.steps, args.fp64_steps) + 60
+    total_steps = args.warmup + max(args.steps, args.fp64_steps, args.adjoint_steps) + 60
     t0 = time.time()
     m = build_model(args.workload, total_steps)
     nodes = m.total_nodes
@@ -181,6 +207,21 @@ def main():
                 "vel0_ms": round(kern64[0][0], 5) if kern64 else None}
         log("[bench] f64: %.2f Gpt/s" % g64)
 
+    adjoint = None
+    if args.adjoint_steps > 0:
+        ea, Ja, sweeps = adjoint_run(m, args.adjoint_steps, args.ckpt_every, args.ckpt_format)
+        ga = nodes * args.adjoint_steps / ea / 1e9
+        ms_a = ea / args.adjoint_steps * 1e3
+        adjoint = {"value": round(ga, 3), "unit": "Gpt/s", "steps": args.adjoint_steps,
+                   "what": "forward with checkpoints + recomputation + adjoint step + gradient of the 9 Cij and "
+                           "buoyancy, FP32, misfit 1/2 sum vx_rec^2",
+                   "checkpoint_every": args.ckpt_every, "checkpoint_format": args.ckpt_format,
+                   "ms_per_step": round(ms_a, 4),
+                   "cost_vs_forward_f32": round(ms_a / ms_step, 3),
+                   "cost_vs_forward_f64": round(ms_a / fp64["ms_per_step"], 3) if fp64 else None,
+                   "adjoint_jacobi_sweeps_per_step": round(sweeps / args.adjoint_steps / max(1, m.num_grids - 1), 2)}
+        log("[bench] fwd+adjoint: %.2f Gpt/s, %.3f ms/step (%.2fx the FP32 forward)" % (ga, ms_a, ms_a / ms_step))
+
     cpu = None
     if not args.no_cpu:
         ce, threads = cpu_baseline(m, args.cpu_steps)
@@ -213,7 +254,7 @@ def main():
                      "bytes_per_launch": BYTES_STRESS_F32 * n0, "avg_launch_ms": round(ms[1], 5),
                      "vel_kernel_GBps": round(vel_bw / 1e9, 1)},
         "fp64": fp64,
-        "adjoint": None,
+        "adjoint": adjoint,
         "jacobi_iters_per_step": [round(s[0] / (args.warmup + args.steps), 2) for s in stats],
         "cpu_baseline": cpu,
     }

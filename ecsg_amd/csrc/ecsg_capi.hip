@@ -382,6 +382,7 @@ public:
         d.iso = iso ? 1 : 0;
         d.dev.iso = d.iso;
         d.params = true;
+        grad_ready = false;
         for (int k = 0; k < (int)I.size(); k++)
             if ((k == g || k + 1 == g) && G[k].params && G[k + 1].params)
                 if (int rc = build_iface(k)) return rc;
@@ -1705,8 +1706,23 @@ public:
     int ck_prepare(int nck)
     {
         const int ng = (int)G.size(), ni = (int)I.size();
-        if (nck > ck_alloc || ck_fmt != ck_alloc_fmt || ck_every + 1 > ring_alloc) {
-            ck_free();
+        if (ck_every + 1 > ring_alloc) {
+            for (auto p : ring) dfree(p);
+            ring.clear();
+            for (int q = 0; q <= ck_every; q++)
+                for (int g = 0; g < ng; g++) {
+                    T *p = nullptr;
+                    if (int rc = dalloc(&p, (size_t)9 * G[g].astride)) return rc;
+                    ring.push_back(p);
+                }
+            ring_alloc = ck_every + 1;
+        }
+        if (nck > ck_alloc || ck_fmt != ck_alloc_fmt) {
+            for (auto p : ck_nat) dfree(p);
+            for (auto p : ck_red) dfree(p);
+            for (auto p : ck_x0) dfree(p);
+            ck_nat.clear(); ck_red.clear(); ck_x0.clear();
+            dfree(ck_scale); dfree(ck_part);
             for (int j = 0; j < nck; j++) {
                 for (int g = 0; g < ng; g++) {
                     if (ck_fmt == 0) {
@@ -1727,15 +1743,8 @@ public:
             }
             if (int rc = dalloc(&ck_scale, (size_t)nck * ng * 9)) return rc;
             if (int rc = dalloc(&ck_part, (size_t)CKP)) return rc;
-            for (int q = 0; q <= ck_every; q++)
-                for (int g = 0; g < ng; g++) {
-                    T *p = nullptr;
-                    if (int rc = dalloc(&p, (size_t)9 * G[g].astride)) return rc;
-                    ring.push_back(p);
-                }
             ck_alloc = nck;
             ck_alloc_fmt = ck_fmt;
-            ring_alloc = ck_every + 1;
         }
         return ECSG_OK;
     }
@@ -1849,8 +1858,11 @@ public:
             HIPCHK(hipMemcpy(st3, AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
             if (st3[2]) return fail(ECSG_ENOCONV, "adjoint Jacobi iteration cap reached on interface " + std::to_string(k));
         }
-        return finalize_gradient();
+        gprim.clear();
+        grad_ready = true;
+        return ECSG_OK;
     }
+    bool grad_ready = false;
 
     // transposed parameter staging: gradients at the staged positions ->
     // gradients w.r.t. the primary-node inputs of set_params
@@ -1921,7 +1933,9 @@ public:
     {
         if (g < 0 || g >= (int)G.size()) return fail(ECSG_EINVAL, "grid index out of range");
         if (sel < ECSG_BUOY || sel > ECSG_C11 + 8) return fail(ECSG_EINVAL, "selector must be ECSG_BUOY or ECSG_C11..ECSG_C11+8");
-        if (gprim.size() != G.size() * 10) return fail(ECSG_ESTATE, "no gradient computed yet");
+        if (!grad_ready) return fail(ECSG_ESTATE, "no gradient computed yet");
+        if (gprim.size() != G.size() * 10)
+            if (int rc = finalize_gradient()) return rc;
         const std::vector<double> &v = gprim[(size_t)g * 10 + (sel == ECSG_BUOY ? 0 : 1 + sel - ECSG_C11)];
         std::copy(v.begin(), v.end(), out);
         return ECSG_OK;
