@@ -9,7 +9,8 @@
 //   UpdateGhostSYZ (pointwise + fine ghosts).
 // The adjoint step applies the exact transposes in reverse order.  Every
 // transpose is written in gather form (each output entry summed by one
-// thread in a fixed order), so results are deterministic.  Gradient terms of
+// thread in a fixed order), so results are deterministic; the interface maps
+// are transposed through their restriction structure (see IfAdj below).  Gradient terms of
 // each map with respect to the staged parameters are accumulated in the same
 // kernels, using the forward state saved for that step.
 //
@@ -253,120 +254,6 @@ __global__ __launch_bounds__(256) void k_zero_list(T *lam, const long *__restric
     if (t < n) lam[idx[t]] = T(0);
 }
 
-// ---------------------------------------------------------------------------
-// Interface maps as term lists.  A key names one entry of one array of the
-// fine (side 0) or coarse (side 1) grid: key = ((side*16 + a) << KSH) | idx,
-// a = wavefield 0..8, or for parameters 0..8 staged stiffness, 9 buoyancy at
-// the node, 10 its xy mean, 11 its xz mean.
-constexpr int KSH = 36;
-
-template <typename T>
-struct TLDev {
-    T *lam[2];            // adjoint wavefield blocks (9 arrays each)
-    long astr[2];         // array stride
-    const T *fwd[2][9];   // forward state read by the gradient terms
-    const T *par[2];      // staged stiffness arrays 0..8, buoyancy as array 9
-    T *grad[2];           // 12 gradient arrays
-    long P[2], S2[2];
-};
-
-__device__ __forceinline__ int tl_side(long k) { return (int)(k >> (KSH + 4)); }
-__device__ __forceinline__ int tl_arr(long k) { return (int)((k >> KSH) & 15); }
-__device__ __forceinline__ long tl_idx(long k) { return k & ((1L << KSH) - 1); }
-
-template <typename T>
-__device__ __forceinline__ T &tl_lam(const TLDev<T> &d, long k)
-{
-    const int s = tl_side(k);
-    return d.lam[s][(long)tl_arr(k) * d.astr[s] + tl_idx(k)];
-}
-
-template <typename T>
-__device__ __forceinline__ double tl_param(const TLDev<T> &d, long k)
-{
-    const int s = tl_side(k), a = tl_arr(k);
-    const long i = tl_idx(k);
-    const T *base = d.par[s];
-    const long as = d.astr[s];
-    if (a < 9) return (double)base[(long)a * as + i];
-    const T *b = base + 9 * as;
-    if (a == 9) return (double)b[i];
-    if (a == 10) return 0.25 * ((((double)b[i] + (double)b[i + 1]) + (double)b[i + d.P[s]]) + (double)b[i + d.P[s] + 1]);
-    return 0.25 * ((((double)b[i] + (double)b[i + d.S2[s]]) + (double)b[i + 1]) + (double)b[i + d.S2[s] + 1]);
-}
-
-// Transposed fine-ghost assignments out = sum cst * in: lam[in] += sum cst * lam[out]
-template <typename T>
-__global__ __launch_bounds__(256) void k_tl_fine(TLDev<T> d, const long *__restrict__ ukey, const int *__restrict__ start,
-                                                 const long *__restrict__ okey, const double *__restrict__ cst, int n)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double acc = 0;
-    for (int q = start[t]; q < start[t + 1]; q++) acc += cst[q] * (double)tl_lam(d, okey[q]);
-    T &x = tl_lam(d, ukey[t]);
-    x = (T)((double)x + acc);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_tl_zero(TLDev<T> d, const long *__restrict__ key, int n)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) tl_lam(d, key[t]) = T(0);
-}
-
-// Adjoint of the solve's output assignment: rhs[row] = lam[x entry] (divided
-// by the pointwise denominator when den != null), lam[x entry] = 0.
-template <typename T>
-__global__ __launch_bounds__(256) void k_tl_readx(TLDev<T> d, const long *__restrict__ xkey, const int *__restrict__ row,
-                                                  const double *__restrict__ den, double *__restrict__ rhs, int n)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    T &x = tl_lam(d, xkey[t]);
-    const double v = (double)x;
-    rhs[row[t]] = den ? v / den[row[t]] : v;
-    x = T(0);
-}
-
-// lam[entry] += sum over terms of cst * param * mu[row]   (B^T mu)
-template <typename T>
-__global__ __launch_bounds__(256) void k_tl_prop(TLDev<T> d, const long *__restrict__ ukey, const int *__restrict__ start,
-                                                 const int *__restrict__ row, const long *__restrict__ pkey,
-                                                 const double *__restrict__ cst, const double *__restrict__ mu, int n)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double acc = 0;
-    for (int q = start[t]; q < start[t + 1]; q++) {
-        const double p = pkey[q] < 0 ? 1.0 : tl_param(d, pkey[q]);
-        acc += cst[q] * p * mu[row[q]];
-    }
-    T &x = tl_lam(d, ukey[t]);
-    x = (T)((double)x + acc);
-}
-
-// grad[param] += sum over terms of cst * mu[row] * forward[entry]
-// (mu^T (dB/dp u - dA/dp x); the A terms carry the negated constant)
-template <typename T>
-__global__ __launch_bounds__(256) void k_tl_grad(TLDev<T> d, const long *__restrict__ ukey, const int *__restrict__ start,
-                                                 const int *__restrict__ row, const long *__restrict__ fkey,
-                                                 const double *__restrict__ cst, const double *__restrict__ mu, int n)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double acc = 0;
-    for (int q = start[t]; q < start[t + 1]; q++) {
-        const long k = fkey[q];
-        const double u = (double)d.fwd[tl_side(k)][tl_arr(k)][tl_idx(k)];
-        acc += cst[q] * mu[row[q]] * u;
-    }
-    const long k = ukey[t];
-    const int s = tl_side(k);
-    T &g = d.grad[s][(long)tl_arr(k) * d.astr[s] + tl_idx(k)];
-    g = (T)((double)g + acc);
-}
-
 // Tridiagonal solve with the prefactored elimination of the vz-szz system
 // (symmetric, so it also solves the transposed system); solution in B.
 __global__ __launch_bounds__(64) void k_tri_solve(double *__restrict__ B, const double *__restrict__ M,
@@ -532,5 +419,423 @@ __global__ __launch_bounds__(256) void k_ck_unpack(T *__restrict__ x, long n, co
         x[i] = (T)(v * s);
     }
 }
+
+}  // namespace ecsg
+
+namespace ecsg {
+
+// ---------------------------------------------------------------------------
+// Structured transposes of the three interface maps.  Each interface system
+// restricts per-fine-node quantities kf(i, j) with window weights w_q(i, j)
+// onto the coarse rows q:  rhs_q = r^m kc_q - sum_(i,j) w_q(i, j) kf(i, j),
+// and its matrix is A = diag(coarse) + k1z R diag(param) P.  With the
+// multipliers mu = A^-1 lam_x (A symmetric) prolonged to the fine nodes,
+//   nu(i, j) = sum_q w_q(i, j) mu_q,
+// the transpose and the parameter derivatives become local:
+//   lam[kf inputs] -= nu * d kf / d input,   grad[param] -= nu * d kf / d param,
+// and the matrix terms add  -k1z nu(i, j) (P x)(i, j)  to the gradient.
+template <typename T>
+struct IfAdj {
+    T *lf[9], *lc[9];            // adjoint wavefields, fine / coarse grid
+    const T *ff[9], *fc[9];      // forward state read by the gradient terms
+    T *gf[12], *gc[12];          // gradient accumulators
+    const T *cf[9], *cc[9];      // staged stiffness
+    const T *bf, *bc;            // buoyancy
+    long oF, PF, SF, oC, PC, SC;
+    int r, nx2, ny2, nz1;
+    double k1x, k1y, k1z, k2x, k2y, k2z;
+};
+
+#define AF_(i, j, k) (a.oF + (long)(k) * a.SF + (long)(j) * a.PF + (i))
+#define AC_(i, j, k) (a.oC + (long)(k) * a.SC + (long)(j) * a.PC + (i))
+
+template <typename T>
+__device__ __forceinline__ void acc_to(T *p, double v) { *p = (T)((double)*p + v); }
+
+template <typename T>
+__device__ __forceinline__ double xy_mean(const T *b, long q, long P)
+{
+    return 0.25 * ((((double)b[q] + (double)b[q + 1]) + (double)b[q + P]) + (double)b[q + P + 1]);
+}
+
+// ----- vx-sxz --------------------------------------------------------------
+// multiplier of interior coarse node (I, J) from the iterate layout (zero border)
+__device__ __forceinline__ double sxz_mu(const double *X, int I, int J, int nx2, int ny2)
+{
+    if (I < 0 || I > nx2 || J < 0 || J > ny2) return 0.0;
+    return X[(long)J * (nx2 + 1) + I];
+}
+__device__ __forceinline__ double sxz_nu(const double *X, int i, int j, int r, int nx2, int ny2)
+{
+    double s = 0;
+    const int I0 = i / r, J0 = j / r;
+    for (int dJ = 0; dJ <= 1; dJ++)
+        for (int dI = 0; dI <= 1; dI++) {
+            const int Ic = I0 + dI, Jc = J0 + dJ;
+            const int ai = i - r * Ic, bj = j - r * Jc;
+            if (ai <= -r || ai >= r || bj <= -r || bj >= r) continue;
+            s += (double)((r - (ai < 0 ? -ai : ai)) * (r - (bj < 0 ? -bj : bj))) * sxz_mu(X, Ic, Jc, nx2, ny2);
+        }
+    return s;
+}
+
+// fine-ghost transpose gathered on interior coarse nodes; rhs of the solve
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_ghost(IfAdj<T> a, double *__restrict__ rhs)
+{
+    const int n0 = a.nx2 - 1, n1 = a.ny2 - 1, r = a.r;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n0 * n1) return;
+    const int I = t % n0 + 1, J = t / n0 + 1;
+    double R = 0;
+    for (int bo = -(r - 1); bo <= r - 1; bo++)
+        for (int ao = -(r - 1); ao <= r - 1; ao++)
+            R += hatw(ao, r) * hatw(bo, r) * (double)a.lf[7][AF_(r * I + ao, r * J + bo, a.nz1)];
+    T *s2 = a.lc[7];
+    const long qc = AC_(I, J, 0);
+    acc_to(&s2[qc], R);
+    rhs[t] = (double)s2[qc - a.SC] + R;
+    s2[qc - a.SC] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_fine0(IfAdj<T> a)
+{
+    const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nx1 - 1) * (ny1 - 1)) return;
+    const int i = t % (nx1 - 1) + 1, j = t / (nx1 - 1) + 1;
+    const long q = AF_(i, j, a.nz1);
+    T *s1 = a.lf[7];
+    s1[q - a.SF] -= s1[q];
+    s1[q] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_fine(IfAdj<T> a, const double *__restrict__ X)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, nx1 = r * nx2, ny1 = r * ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nx1 + 1) * (ny1 + 1)) return;
+    const int i = t % (nx1 + 1), j = t / (nx1 + 1);
+    const double nu0 = sxz_nu(X, i, j, r, nx2, ny2);
+    const double nux = (i + 1 <= nx1) ? sxz_nu(X, i + 1, j, r, nx2, ny2) : 0.0;
+    const double nuy = (j + 1 <= ny1) ? sxz_nu(X, i, j + 1, r, nx2, ny2) : 0.0;
+    const long q = AF_(i, j, a.nz1);
+    const double b = (double)a.bf[q], bx = (double)a.bf[q + 1], by = (double)a.bf[q + a.PF];
+    if (nu0 != 0.0) {
+        acc_to(&a.lf[0][q], -nu0);
+        acc_to(&a.lf[7][q - a.SF], 2 * a.k1z * b * nu0);
+        // gradient w.r.t. b1 (node): B terms and the matrix terms (P x)
+        const int I0 = i / r, J0 = j / r;
+        double S = 0;
+        for (int dJ = 0; dJ <= 1; dJ++)
+            for (int dI = 0; dI <= 1; dI++) {
+                const int Ic = I0 + dI, Jc = J0 + dJ;
+                if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
+                const double pw = hatw(i - r * Ic, r) * hatw(j - r * Jc, r);
+                if (pw == 0) continue;
+                const long q2 = AC_(Ic, Jc, 0);
+                S += pw * ((double)a.fc[7][q2] + (double)a.fc[7][q2 - a.SC]);
+            }
+        const T *sxx = a.ff[3], *sxy = a.ff[8], *sxz = a.ff[7];
+        const double k = a.k1x * ((double)sxx[q] - (double)sxx[q - 1]) + a.k1y * ((double)sxy[q] - (double)sxy[q - a.PF])
+                       + a.k1z * (S - 2 * (double)sxz[q - a.SF]);
+        acc_to(&a.gf[9][q], -nu0 * k);
+    }
+    if (i <= nx1 - 1 && (nu0 != 0.0 || nux != 0.0)) acc_to(&a.lf[3][q], -a.k1x * (b * nu0 - bx * nux));
+    if (j <= ny1 - 1 && (nu0 != 0.0 || nuy != 0.0)) acc_to(&a.lf[8][q], -a.k1y * (b * nu0 - by * nuy));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_coarse(IfAdj<T> a, const double *__restrict__ X)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nx2 + 1) * (ny2 + 1)) return;
+    const int I = t % (nx2 + 1), J = t / (nx2 + 1);
+    const double r2 = (double)(r * r), r4 = r2 * r2;
+    const double mu0 = sxz_mu(X, I, J, nx2, ny2), mux = sxz_mu(X, I + 1, J, nx2, ny2), muy = sxz_mu(X, I, J + 1, nx2, ny2);
+    const long qc = AC_(I, J, 0);
+    const double b2 = (double)a.bc[qc], b2x = (double)a.bc[qc + 1], b2y = (double)a.bc[qc + a.PC];
+    if (mu0 != 0.0) acc_to(&a.lc[0][qc], r4 * mu0);
+    if (I <= nx2 - 1 && (mu0 != 0.0 || mux != 0.0)) acc_to(&a.lc[3][qc], r4 * a.k2x * (b2 * mu0 - b2x * mux));
+    if (J <= ny2 - 1 && (mu0 != 0.0 || muy != 0.0)) acc_to(&a.lc[8][qc], r4 * a.k2y * (b2 * mu0 - b2y * muy));
+    if (I >= 1 && I <= nx2 - 1 && J >= 1 && J <= ny2 - 1) {
+        // sxz2(I, J, 0): coarse term and the prolongation inside every fine kf
+        double sfine = 0;
+        for (int bo = -(r - 1); bo <= r - 1; bo++)
+            for (int ao = -(r - 1); ao <= r - 1; ao++) {
+                const int i = r * I + ao, j = r * J + bo;
+                const double nu = sxz_nu(X, i, j, r, nx2, ny2);
+                if (nu != 0.0) sfine += hatw(ao, r) * hatw(bo, r) * (double)a.bf[AF_(i, j, a.nz1)] * nu;
+            }
+        acc_to(&a.lc[7][qc], r4 * a.k2z * b2 * mu0 - a.k1z * sfine);
+        if (mu0 != 0.0) {
+            const T *sxx = a.fc[3], *sxy = a.fc[8], *sxz = a.fc[7];
+            const double k = a.k2x * ((double)sxx[qc] - (double)sxx[qc - 1]) + a.k2y * ((double)sxy[qc] - (double)sxy[qc - a.PC])
+                           + a.k2z * ((double)sxz[qc] - (double)sxz[qc - a.SC]);
+            acc_to(&a.gc[9][qc], mu0 * r4 * k);
+        }
+    }
+}
+
+// ----- vz-szz --------------------------------------------------------------
+__device__ __forceinline__ double vz_mu(const double *M, int I, int J, int nx2, int ny2)
+{
+    if (I < 0 || I > nx2 - 1 || J < 1 || J > ny2 - 1) return 0.0;
+    return M[(long)(J - 1) * nx2 + I];
+}
+__device__ __forceinline__ double vz_nu(const double *M, int i, int j, int r, int nx2, int ny2)
+{
+    if (i < 0 || i > r * nx2 - 1) return 0.0;
+    const int I = i / r, J0 = j / r;
+    double s = 0;
+    for (int dJ = 0; dJ <= 1; dJ++) {
+        const int J = J0 + dJ, bo = j - r * J;
+        if (bo <= -r || bo >= r) continue;
+        s += (double)(r - (bo < 0 ? -bo : bo)) * vz_mu(M, I, J, nx2, ny2);
+    }
+    return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_ghost(IfAdj<T> a, double *__restrict__ rhs)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, ny1 = r * ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx2 * (ny2 + 1)) return;
+    const int I = t % nx2, J = t / nx2;
+    double R = 0;
+    for (int bo = -(r - 1); bo <= r - 1; bo++) {
+        const int j = r * J + bo;
+        if (j < 1 || j > ny1 - 1) continue;
+        const double h = hatw(bo, r);
+        for (int ao = 0; ao < r; ao++) R += h * (double)a.lf[2][AF_(r * I + ao, j, a.nz1)];
+    }
+    T *v2 = a.lc[2];
+    const long qc = AC_(I, J, 0);
+    acc_to(&v2[qc], R);
+    if (J >= 1 && J <= ny2 - 1) {
+        rhs[(long)(J - 1) * nx2 + I] = (double)v2[qc - a.SC] + R;
+        v2[qc - a.SC] = T(0);
+    } else {
+        acc_to(&v2[qc - a.SC], R);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_fine0(IfAdj<T> a)
+{
+    const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx1 * (ny1 - 1)) return;
+    const int i = t % nx1, j = t / nx1 + 1;
+    const long q = AF_(i, j, a.nz1);
+    T *v1 = a.lf[2];
+    v1[q - a.SF] -= v1[q];
+    v1[q] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_fine(IfAdj<T> a, const double *__restrict__ M)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, nx1 = r * nx2, ny1 = r * ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nx1 + 1) * (ny1 + 1)) return;
+    const int i = t % (nx1 + 1), j = t / (nx1 + 1);
+    const double nu0 = vz_nu(M, i, j, r, nx2, ny2);
+    const double nuxm = vz_nu(M, i - 1, j, r, nx2, ny2);
+    const double nuyp = (j + 1 <= ny1) ? vz_nu(M, i, j + 1, r, nx2, ny2) : 0.0;
+    const long q = AF_(i, j, a.nz1);
+    const T *c13 = a.cf[4], *c23 = a.cf[3], *c33 = a.cf[2];
+    if (nu0 != 0.0 || nuxm != 0.0) {
+        const double cur = nu0 != 0.0 ? (double)c13[q] * nu0 : 0.0;
+        const double prv = nuxm != 0.0 ? (double)c13[q - 1] * nuxm : 0.0;
+        acc_to(&a.lf[0][q], a.k1x * (cur - prv));
+    }
+    if (i <= nx1 - 1 && j <= ny1 - 1 && (nu0 != 0.0 || nuyp != 0.0)) {
+        const double cur = nu0 != 0.0 ? (double)c23[q] * nu0 : 0.0;
+        const double nxt = nuyp != 0.0 ? (double)c23[q + a.PF] * nuyp : 0.0;
+        acc_to(&a.lf[1][q], -a.k1y * (cur - nxt));
+    }
+    if (nu0 != 0.0) {
+        acc_to(&a.lf[5][q], -nu0);
+        acc_to(&a.lf[2][q - a.SF], 2 * a.k1z * (double)c33[q] * nu0);
+        const T *vx = a.ff[0], *vy = a.ff[1], *vz1 = a.ff[2], *vz2 = a.fc[2];
+        const int I = i / r, J0 = j / r;
+        double pv = 0, px = 0;
+        for (int dJ = 0; dJ <= 1; dJ++) {
+            const int J = J0 + dJ, bo = j - r * J;
+            if (bo <= -r || bo >= r || J > ny2) continue;
+            const double h = hatw(bo, r);
+            const long q2 = AC_(I, J, 0);
+            pv += h * (double)vz2[q2];
+            if (J >= 1 && J <= ny2 - 1) px += h * (double)vz2[q2 - a.SC];
+        }
+        acc_to(&a.gf[4][q], -nu0 * a.k1x * ((double)vx[q + 1] - (double)vx[q]));
+        acc_to(&a.gf[3][q], -nu0 * a.k1y * ((double)vy[q] - (double)vy[q - a.PF]));
+        acc_to(&a.gf[2][q], -nu0 * a.k1z * (pv + px - 2 * (double)vz1[q - a.SF]));
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_coarse(IfAdj<T> a, const double *__restrict__ M)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, ny1 = r * ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nx2 + 1) * (ny2 + 1)) return;
+    const int I = t % (nx2 + 1), J = t / (nx2 + 1);
+    const double r3 = (double)(r * r * r);
+    const double mu0 = vz_mu(M, I, J, nx2, ny2), muxm = vz_mu(M, I - 1, J, nx2, ny2), muyp = vz_mu(M, I, J + 1, nx2, ny2);
+    const long qc = AC_(I, J, 0);
+    const T *c13 = a.cc[4], *c23 = a.cc[3], *c33 = a.cc[2];
+    if (mu0 != 0.0) acc_to(&a.lc[5][qc], r3 * mu0);
+    if (mu0 != 0.0 || muxm != 0.0) {
+        const double cur = mu0 != 0.0 ? (double)c13[qc] * mu0 : 0.0;
+        const double prv = muxm != 0.0 ? (double)c13[qc - 1] * muxm : 0.0;
+        acc_to(&a.lc[0][qc], r3 * a.k2x * (prv - cur));
+    }
+    if (I <= nx2 - 1 && J <= ny2 - 1 && (mu0 != 0.0 || muyp != 0.0)) {
+        const double cur = mu0 != 0.0 ? (double)c23[qc] * mu0 : 0.0;
+        const double nxt = muyp != 0.0 ? (double)c23[qc + a.PC] * muyp : 0.0;
+        acc_to(&a.lc[1][qc], r3 * a.k2y * (cur - nxt));
+    }
+    if (I <= nx2 - 1) {
+        // vz2(I, J, 0): coarse term and the y interpolation inside every fine kf
+        double sfine = 0;
+        for (int bo = -(r - 1); bo <= r - 1; bo++) {
+            const int j = r * J + bo;
+            if (j < 1 || j > ny1 - 1) continue;
+            const double h = hatw(bo, r);
+            for (int ao = 0; ao < r; ao++) {
+                const int i = r * I + ao;
+                const double nu = vz_nu(M, i, j, r, nx2, ny2);
+                if (nu != 0.0) sfine += h * (double)a.cf[2][AF_(i, j, a.nz1)] * nu;
+            }
+        }
+        const double own = mu0 != 0.0 ? r3 * a.k2z * (double)c33[qc] * mu0 : 0.0;
+        if (own != 0.0 || sfine != 0.0) acc_to(&a.lc[2][qc], own - a.k1z * sfine);
+        if (mu0 != 0.0) {
+            const T *vx = a.fc[0], *vy = a.fc[1], *vz = a.fc[2];
+            acc_to(&a.gc[4][qc], mu0 * r3 * a.k2x * ((double)vx[qc + 1] - (double)vx[qc]));
+            acc_to(&a.gc[3][qc], mu0 * r3 * a.k2y * ((double)vy[qc] - (double)vy[qc - a.PC]));
+            acc_to(&a.gc[2][qc], mu0 * r3 * a.k2z * ((double)vz[qc] - (double)vz[qc - a.SC]));
+        }
+    }
+}
+
+// ----- vy-syz --------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_ghost(IfAdj<T> a, const double *__restrict__ den, double *__restrict__ mu)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx2 * ny2) return;
+    const int I = t % nx2, J = t / nx2;
+    double R = 0;
+    for (int bo = 0; bo < r; bo++)
+        for (int ao = 0; ao < r; ao++) R += (double)a.lf[6][AF_(r * I + ao, r * J + bo, a.nz1)];
+    T *s2 = a.lc[6];
+    const long qc = AC_(I, J, 0);
+    acc_to(&s2[qc], R);
+    mu[t] = ((double)s2[qc - a.SC] + R) / den[t];
+    s2[qc - a.SC] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_fine0(IfAdj<T> a)
+{
+    const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx1 * ny1) return;
+    const int i = t % nx1, j = t / nx1;
+    const long q = AF_(i, j, a.nz1);
+    T *s1 = a.lf[6];
+    s1[q - a.SF] -= s1[q];
+    s1[q] = T(0);
+}
+
+__device__ __forceinline__ double syz_mu(const double *mu, int I, int J, int nx2, int ny2)
+{
+    if (I < 0 || I > nx2 - 1 || J < 0 || J > ny2 - 1) return 0.0;
+    return mu[(long)J * nx2 + I];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_fine(IfAdj<T> a, const double *__restrict__ mu)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, nx1 = r * nx2, ny1 = r * ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nx1 + 1) * (ny1 + 1)) return;
+    const int i = t % (nx1 + 1), j = t / (nx1 + 1);
+    auto nu = [&](int ii, int jj) -> double {
+        if (ii < 0 || ii > nx1 - 1 || jj < 0 || jj > ny1 - 1) return 0.0;
+        return syz_mu(mu, ii / r, jj / r, nx2, ny2);
+    };
+    const double nu0 = nu(i, j), nuxm = nu(i - 1, j), nuym = nu(i, j - 1);
+    const long q = AF_(i, j, a.nz1);
+    const T *b = a.bf;
+    const long P = a.PF;
+    if (j <= ny1 - 1 && (nu0 != 0.0 || nuxm != 0.0)) {
+        const double cur = nu0 != 0.0 ? xy_mean(b, q, P) * nu0 : 0.0;
+        const double prv = nuxm != 0.0 ? xy_mean(b, q - 1, P) * nuxm : 0.0;
+        acc_to(&a.lf[8][q], -a.k1x * (prv - cur));
+    }
+    if (i <= nx1 - 1 && (nu0 != 0.0 || nuym != 0.0)) {
+        const double cur = nu0 != 0.0 ? xy_mean(b, q, P) * nu0 : 0.0;
+        const double prv = nuym != 0.0 ? xy_mean(b, q - P, P) * nuym : 0.0;
+        acc_to(&a.lf[4][q], -a.k1y * (prv - cur));
+    }
+    if (nu0 != 0.0) {
+        const double c = xy_mean(b, q, P);
+        acc_to(&a.lf[1][q], -nu0);
+        acc_to(&a.lf[6][q - a.SF], 2 * a.k1z * c * nu0);
+        const T *sxy = a.ff[8], *syy = a.ff[4], *syz1 = a.ff[6], *syz2 = a.fc[6];
+        const long qc = AC_(i / r, j / r, 0);
+        const double k = a.k1x * ((double)sxy[q + 1] - (double)sxy[q]) + a.k1y * ((double)syy[q + P] - (double)syy[q])
+                       + a.k1z * ((double)syz2[qc] + (double)syz2[qc - a.SC] - 2 * (double)syz1[q - a.SF]);
+        acc_to(&a.gf[10][q], -nu0 * k);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_coarse(IfAdj<T> a, const double *__restrict__ mu)
+{
+    const int r = a.r, nx2 = a.nx2, ny2 = a.ny2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nx2 + 1) * (ny2 + 1)) return;
+    const int I = t % (nx2 + 1), J = t / (nx2 + 1);
+    const double r2 = (double)(r * r);
+    const double mu0 = syz_mu(mu, I, J, nx2, ny2), muxm = syz_mu(mu, I - 1, J, nx2, ny2), muym = syz_mu(mu, I, J - 1, nx2, ny2);
+    const long qc = AC_(I, J, 0);
+    const T *b = a.bc;
+    const long P = a.PC;
+    if (J <= ny2 - 1 && (mu0 != 0.0 || muxm != 0.0)) {
+        const double cur = mu0 != 0.0 ? xy_mean(b, qc, P) * mu0 : 0.0;
+        const double prv = muxm != 0.0 ? xy_mean(b, qc - 1, P) * muxm : 0.0;
+        acc_to(&a.lc[8][qc], r2 * a.k2x * (prv - cur));
+    }
+    if (I <= nx2 - 1 && (mu0 != 0.0 || muym != 0.0)) {
+        const double cur = mu0 != 0.0 ? xy_mean(b, qc, P) * mu0 : 0.0;
+        const double prv = muym != 0.0 ? xy_mean(b, qc - P, P) * muym : 0.0;
+        acc_to(&a.lc[4][qc], r2 * a.k2y * (prv - cur));
+    }
+    if (mu0 != 0.0) {
+        const double c0 = xy_mean(b, qc, P);
+        double cs = 0;
+        for (int bo = 0; bo < r; bo++)
+            for (int ao = 0; ao < r; ao++) cs += xy_mean(a.bf, AF_(r * I + ao, r * J + bo, a.nz1), a.PF);
+        acc_to(&a.lc[1][qc], r2 * mu0);
+        acc_to(&a.lc[6][qc], (r2 * c0 * a.k2z - cs * a.k1z) * mu0);
+        const T *sxy = a.fc[8], *syy = a.fc[4], *syz = a.fc[6];
+        const double k = a.k2x * ((double)sxy[qc + 1] - (double)sxy[qc]) + a.k2y * ((double)syy[qc + P] - (double)syy[qc])
+                       + a.k2z * ((double)syz[qc] - (double)syz[qc - a.SC]);
+        acc_to(&a.gc[10][qc], mu0 * r2 * k);
+    }
+}
+
+#undef AF_
+#undef AC_
 
 }  // namespace ecsg

@@ -1070,25 +1070,7 @@ public:
     // last, the forward steps are recomputed from the checkpoint into a ring
     // of saved states and the transposed steps run backwards over them.
     // ======================================================================
-    struct DevList {            // CSR on the device: distinct keys and their terms
-        long *ukey = nullptr;
-        int *start = nullptr;
-        int *row = nullptr;
-        long *tkey = nullptr;
-        double *cst = nullptr;
-        int n = 0;
-        long nt = 0;
-    };
-    struct DevOp {              // one interface map
-        DevList fine, prop, grad;
-        long *zkey = nullptr;   // fine-ghost outputs (zeroed after their transpose)
-        int nz = 0;
-        long *xkey = nullptr;   // entries written from the solution
-        int *xrow = nullptr;    // their right-hand-side rows
-        int nx = 0;
-    };
     struct AdjIface {
-        DevOp op[3];            // 0 vz-szz, 1 vx-sxz, 2 vy-syz
         double *mu_vz = nullptr, *mu_syz = nullptr, *den = nullptr;
         double *Ba = nullptr, *X0a = nullptr, *X1a = nullptr, *jpart = nullptr, *absz = nullptr;
         long long *stats = nullptr;
@@ -1125,7 +1107,6 @@ public:
 
     template <typename P>
     static void dfree(P *&p) { if (p) hipFree((void *)p); p = nullptr; }
-    static void list_free(DevList &L) { dfree(L.ukey); dfree(L.start); dfree(L.row); dfree(L.tkey); dfree(L.cst); L.n = 0; L.nt = 0; }
     void ck_free()
     {
         for (auto p : ck_nat) dfree(p);
@@ -1139,7 +1120,6 @@ public:
     void adj_free()
     {
         for (auto &a : AI) {
-            for (auto &o : a.op) { list_free(o.fine); list_free(o.prop); list_free(o.grad); dfree(o.zkey); dfree(o.xkey); dfree(o.xrow); }
             dfree(a.mu_vz); dfree(a.mu_syz); dfree(a.den); dfree(a.Ba); dfree(a.X0a); dfree(a.X1a); dfree(a.jpart); dfree(a.absz);
             dfree(a.stats);
         }
@@ -1161,241 +1141,12 @@ public:
         return ECSG_OK;
     }
 
-    static long key(int side, int a, long idx) { return ((long)(side * 16 + a) << KSH) | idx; }
-    struct GT { long key; int row; long tkey; double c; };
-
-    int upload_csr(std::vector<GT> &v, DevList &L)
-    {
-        std::stable_sort(v.begin(), v.end(), [](const GT &a, const GT &b) { return a.key < b.key; });
-        std::vector<long> uk, tk;
-        std::vector<int> st, rw;
-        std::vector<double> cs;
-        uk.reserve(v.size()); tk.reserve(v.size()); rw.reserve(v.size()); cs.reserve(v.size());
-        for (size_t i = 0; i < v.size(); i++) {
-            if (i == 0 || v[i].key != v[i - 1].key) { uk.push_back(v[i].key); st.push_back((int)i); }
-            rw.push_back(v[i].row); tk.push_back(v[i].tkey); cs.push_back(v[i].c);
-        }
-        st.push_back((int)v.size());
-        L.n = (int)uk.size();
-        L.nt = (long)v.size();
-        if (int rc = upload(&L.ukey, uk)) return rc;
-        if (int rc = upload(&L.start, st)) return rc;
-        if (int rc = upload(&L.row, rw)) return rc;
-        if (int rc = upload(&L.tkey, tk)) return rc;
-        if (int rc = upload(&L.cst, cs)) return rc;
-        return ECSG_OK;
-    }
-
-    // Host description of one interface map, mirroring the forward kernel:
-    //   B terms: rhs[row] += c * param(p) * u(f)  (p < 0: no parameter)
-    //   A terms: A[row][col] += c * param(p), stored with f = the entry that
-    //            holds x[col] (used by the gradient only)
-    //   fine terms: out = sum c * in (the fine ghosts written after the solve)
-    //   xs: the entry written from x[row] and the rhs row it is read back into
-    struct OpBuild {
-        std::vector<GT> b, a, fine;
-        std::vector<long> xkey;
-        std::vector<int> xrow;
-        void B(long f, long p, int row, double c) { b.push_back({f, row, p, c}); }
-        void A(long x, long p, int row, double c) { a.push_back({x, row, p, c}); }
-        void F(long in, long out, double c) { fine.push_back({in, 0, out, c}); }
-    };
-
-    int upload_op(OpBuild &ob, DevOp &op)
-    {
-        // prop: grouped by wavefield entry, term key = parameter
-        std::vector<GT> pv = ob.b;
-        if (int rc = upload_csr(pv, op.prop)) return rc;
-        // grad: grouped by parameter, term key = wavefield entry; A terms negated
-        std::vector<GT> gv;
-        gv.reserve(ob.b.size() + ob.a.size());
-        for (auto &t : ob.b)
-            if (t.tkey >= 0) gv.push_back({t.tkey, t.row, t.key, t.c});
-        for (auto &t : ob.a) gv.push_back({t.tkey, t.row, t.key, -t.c});
-        if (int rc = upload_csr(gv, op.grad)) return rc;
-        // fine: grouped by input entry, term key = output entry
-        std::vector<GT> fv = ob.fine;
-        std::vector<long> outs;
-        for (auto &t : fv) outs.push_back(t.tkey);
-        if (int rc = upload_csr(fv, op.fine)) return rc;
-        std::sort(outs.begin(), outs.end());
-        outs.erase(std::unique(outs.begin(), outs.end()), outs.end());
-        op.nz = (int)outs.size();
-        if (int rc = upload(&op.zkey, outs)) return rc;
-        op.nx = (int)ob.xkey.size();
-        if (int rc = upload(&op.xkey, ob.xkey)) return rc;
-        if (int rc = upload(&op.xrow, ob.xrow)) return rc;
-        return ECSG_OK;
-    }
-
-    // term lists of interface k (same loops and coefficients as k_gvz_rhs,
-    // k_gvz_fine, k_gsxz_rhs, the Jacobi scatter, k_gsyz and build_iface)
+    // work buffers of the transposed interface solves of interface k
     int build_adj_iface(int k)
     {
-        const DevGrid<T> &F = G[k], &C = G[k + 1];
         const DevIface &f = I[k];
         AdjIface &ai = AI[k];
-        const int r = f.r, nx2 = f.nx2, ny2 = f.ny2, nz1 = f.nz1;
-        const int nx1 = r * nx2, ny1 = r * ny2;
-        const double k1x = (double)F.dev.cx, k1y = (double)F.dev.cy, k1z = (double)F.dev.cz;
-        const double k2x = (double)C.dev.cx, k2y = (double)C.dev.cy, k2z = (double)C.dev.cz;
-        auto iF = [&](int i, int j, int z) { return F.o + (long)z * F.S2 + (long)j * F.P + i; };
-        auto iC = [&](int i, int j, int z) { return C.o + (long)z * C.S2 + (long)j * C.P + i; };
-        enum { VX = 0, VY, VZ, SXX, SYY, SZZ, SYZ, SXZ, SXY };
-        enum { PC33 = 2, PC23 = 3, PC13 = 4, PBN = 9, PBY = 10 };
-        const int SF = 0, SC = 1;
-        const double r2 = (double)(r * r), r3 = r2 * r, r4 = r2 * r2;
-        // ---- vz-szz: rows (J-1)*nx2 + I, tridiagonal in J
-        {
-            OpBuild ob;
-            for (int J = 1; J <= ny2 - 1; J++)
-                for (int Ii = 0; Ii < nx2; Ii++) {
-                    const int q = (J - 1) * nx2 + Ii;
-                    const long qc = iC(Ii, J, 0);
-                    ob.B(key(SC, SZZ, qc), -1, q, r3);
-                    ob.B(key(SC, VX, qc + 1), key(SC, PC13, qc), q, r3 * k2x);
-                    ob.B(key(SC, VX, qc), key(SC, PC13, qc), q, -r3 * k2x);
-                    ob.B(key(SC, VY, qc), key(SC, PC23, qc), q, r3 * k2y);
-                    ob.B(key(SC, VY, qc - C.P), key(SC, PC23, qc), q, -r3 * k2y);
-                    ob.B(key(SC, VZ, qc), key(SC, PC33, qc), q, r3 * k2z);
-                    ob.A(key(SC, VZ, iC(Ii, J, -1)), key(SC, PC33, qc), q, r3 * k2z);
-                    for (int bo = -(r - 1); bo <= r - 1; bo++) {
-                        const int j = r * J + bo;
-                        const double wj = (double)(r - std::abs(bo));
-                        const int Jl = (bo >= 0) ? J : J - 1;
-                        const int bb = j - r * Jl;
-                        for (int ao = 0; ao < r; ao++) {
-                            const int i = r * Ii + ao;
-                            const long qf = iF(i, j, nz1);
-                            const long pc = key(SF, PC33, qf);
-                            ob.B(key(SF, SZZ, qf), -1, q, -wj);
-                            ob.B(key(SF, VX, qf + 1), key(SF, PC13, qf), q, -wj * k1x);
-                            ob.B(key(SF, VX, qf), key(SF, PC13, qf), q, wj * k1x);
-                            ob.B(key(SF, VY, qf), key(SF, PC23, qf), q, -wj * k1y);
-                            ob.B(key(SF, VY, qf - F.P), key(SF, PC23, qf), q, wj * k1y);
-                            ob.B(key(SC, VZ, iC(Ii, Jl, 0)), pc, q, -wj * k1z * hat(bb, r));
-                            ob.B(key(SC, VZ, iC(Ii, Jl + 1, 0)), pc, q, -wj * k1z * hat(bb - r, r));
-                            ob.B(key(SF, VZ, qf - F.S2), pc, q, 2 * wj * k1z);
-                            ob.A(key(SC, VZ, iC(Ii, J, -1)), pc, q, k1z * wj * hat(bo, r));
-                            if (bo < 0 && J - 1 >= 1) ob.A(key(SC, VZ, iC(Ii, J - 1, -1)), pc, q, k1z * wj * hat(bo + r, r));
-                            if (bo > 0 && J + 1 <= ny2 - 1) ob.A(key(SC, VZ, iC(Ii, J + 1, -1)), pc, q, k1z * wj * hat(bo - r, r));
-                        }
-                    }
-                    ob.xkey.push_back(key(SC, VZ, iC(Ii, J, -1)));
-                    ob.xrow.push_back(q);
-                }
-            for (int j = 1; j <= ny1 - 1; j++)
-                for (int i = 0; i < nx1; i++) {
-                    const int Ii = i / r, J = j / r, bo = j - r * J;
-                    const long out = key(SF, VZ, iF(i, j, nz1));
-                    ob.F(key(SF, VZ, iF(i, j, nz1 - 1)), out, -1.0);
-                    const double c0 = bo == 0 ? 1.0 : hat(bo, r);
-                    ob.F(key(SC, VZ, iC(Ii, J, 0)), out, c0);
-                    ob.F(key(SC, VZ, iC(Ii, J, -1)), out, c0);
-                    if (bo != 0) {
-                        ob.F(key(SC, VZ, iC(Ii, J + 1, 0)), out, hat(bo - r, r));
-                        ob.F(key(SC, VZ, iC(Ii, J + 1, -1)), out, hat(bo - r, r));
-                    }
-                }
-            if (int rc = upload_op(ob, ai.op[0])) return rc;
-        }
-        // ---- vx-sxz: rows J*(nx2+1) + I of the iterate, rhs rows (J-1)*(nx2-1) + I-1
-        {
-            OpBuild ob;
-            const long w = nx2 + 1;
-            const int n0 = nx2 - 1;
-            for (int J = 1; J <= ny2 - 1; J++)
-                for (int Ii = 1; Ii <= nx2 - 1; Ii++) {
-                    const int row = (int)(J * w + Ii);
-                    const long qc = iC(Ii, J, 0);
-                    const long pb = key(SC, PBN, qc);
-                    ob.B(key(SC, VX, qc), -1, row, r4);
-                    ob.B(key(SC, SXX, qc), pb, row, r4 * k2x);
-                    ob.B(key(SC, SXX, qc - 1), pb, row, -r4 * k2x);
-                    ob.B(key(SC, SXY, qc), pb, row, r4 * k2y);
-                    ob.B(key(SC, SXY, qc - C.P), pb, row, -r4 * k2y);
-                    ob.B(key(SC, SXZ, qc), pb, row, r4 * k2z);
-                    ob.A(key(SC, SXZ, qc - C.S2), pb, row, r4 * k2z);
-                    for (int bo = -(r - 1); bo <= r - 1; bo++)
-                        for (int ao = -(r - 1); ao <= r - 1; ao++) {
-                            const int i = r * Ii + ao, j = r * J + bo;
-                            const double wgt = (double)((r - std::abs(ao)) * (r - std::abs(bo)));
-                            const long qf = iF(i, j, nz1);
-                            const long pf = key(SF, PBN, qf);
-                            ob.B(key(SF, VX, qf), -1, row, -wgt);
-                            ob.B(key(SF, SXX, qf), pf, row, -wgt * k1x);
-                            ob.B(key(SF, SXX, qf - 1), pf, row, wgt * k1x);
-                            ob.B(key(SF, SXY, qf), pf, row, -wgt * k1y);
-                            ob.B(key(SF, SXY, qf - F.P), pf, row, wgt * k1y);
-                            ob.B(key(SF, SXZ, qf - F.S2), pf, row, 2 * wgt * k1z);
-                            for (int dJ = -1; dJ <= 1; dJ++)
-                                for (int dI = -1; dI <= 1; dI++) {
-                                    const int Ic = Ii + dI, Jc = J + dJ;
-                                    const int ai2 = i - r * Ic, bj = j - r * Jc;
-                                    if (std::abs(ai2) >= r || std::abs(bj) >= r) continue;
-                                    if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
-                                    const double p = hat(ai2, r) * hat(bj, r);
-                                    ob.B(key(SC, SXZ, iC(Ic, Jc, 0)), pf, row, -wgt * k1z * p);
-                                    ob.A(key(SC, SXZ, iC(Ic, Jc, -1)), pf, row, k1z * wgt * p);
-                                }
-                        }
-                    ob.xkey.push_back(key(SC, SXZ, qc - C.S2));
-                    ob.xrow.push_back((J - 1) * n0 + (Ii - 1));
-                }
-            for (int j = 1; j < ny1; j++)
-                for (int i = 1; i < nx1; i++) {
-                    const int I0 = i / r, J0 = j / r, ao = i - r * I0, bo = j - r * J0;
-                    const long out = key(SF, SXZ, iF(i, j, nz1));
-                    ob.F(key(SF, SXZ, iF(i, j, nz1 - 1)), out, -1.0);
-                    for (int dJ = 0; dJ <= 1; dJ++)
-                        for (int dI = 0; dI <= 1; dI++) {
-                            const int Ic = I0 + dI, Jc = J0 + dJ;
-                            if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
-                            const double pw = hat(ao - r * dI, r) * hat(bo - r * dJ, r);
-                            if (pw == 0) continue;
-                            ob.F(key(SC, SXZ, iC(Ic, Jc, -1)), out, pw);
-                            ob.F(key(SC, SXZ, iC(Ic, Jc, 0)), out, pw);
-                        }
-                }
-            if (int rc = upload_op(ob, ai.op[1])) return rc;
-        }
-        // ---- vy-syz: pointwise, rows J*nx2 + I
-        {
-            OpBuild ob;
-            for (int J = 0; J < ny2; J++)
-                for (int Ii = 0; Ii < nx2; Ii++) {
-                    const int q = J * nx2 + Ii;
-                    const long qc = iC(Ii, J, 0);
-                    const long pc = key(SC, PBY, qc);
-                    ob.B(key(SC, VY, qc), -1, q, r2);
-                    ob.B(key(SC, SXY, qc + 1), pc, q, r2 * k2x);
-                    ob.B(key(SC, SXY, qc), pc, q, -r2 * k2x);
-                    ob.B(key(SC, SYY, qc + C.P), pc, q, r2 * k2y);
-                    ob.B(key(SC, SYY, qc), pc, q, -r2 * k2y);
-                    ob.B(key(SC, SYZ, qc), pc, q, r2 * k2z);
-                    ob.A(key(SC, SYZ, qc - C.S2), pc, q, r2 * k2z);
-                    for (int bo = 0; bo < r; bo++)
-                        for (int ao = 0; ao < r; ao++) {
-                            const long qf = iF(r * Ii + ao, r * J + bo, nz1);
-                            const long pf = key(SF, PBY, qf);
-                            ob.B(key(SF, VY, qf), -1, q, -1.0);
-                            ob.B(key(SF, SXY, qf + 1), pf, q, -k1x);
-                            ob.B(key(SF, SXY, qf), pf, q, k1x);
-                            ob.B(key(SF, SYY, qf + F.P), pf, q, -k1y);
-                            ob.B(key(SF, SYY, qf), pf, q, k1y);
-                            ob.B(key(SC, SYZ, qc), pf, q, -k1z);
-                            ob.B(key(SF, SYZ, qf - F.S2), pf, q, 2 * k1z);
-                            ob.A(key(SC, SYZ, qc - C.S2), pf, q, k1z);
-                            const long out = key(SF, SYZ, qf);
-                            ob.F(key(SF, SYZ, qf - F.S2), out, -1.0);
-                            ob.F(key(SC, SYZ, qc - C.S2), out, 1.0);
-                            ob.F(key(SC, SYZ, qc), out, 1.0);
-                        }
-                    ob.xkey.push_back(key(SC, SYZ, qc - C.S2));
-                    ob.xrow.push_back(q);
-                }
-            if (int rc = upload_op(ob, ai.op[2])) return rc;
-        }
+        const int nx2 = f.nx2, ny2 = f.ny2;
         const long nl = (long)nx2 * (ny2 - 1), nn = (long)(nx2 - 1) * (ny2 - 1);
         const long nxw = (long)(nx2 + 1) * (ny2 + 1);
         if (int rc = dalloc(&ai.mu_vz, nl)) return rc;
@@ -1520,20 +1271,28 @@ public:
         return slot < 0 ? (const T *)G[g].arr(f) : ring[(size_t)slot * G.size() + g] + (long)f * G[g].astride;
     }
 
-    TLDev<T> tl_dev(int k, int slot_v, int slot_s)
+    IfAdj<T> if_adj(int k, int slot_v, int slot_s)
     {
-        TLDev<T> d{};
-        for (int side = 0; side < 2; side++) {
-            const int g = k + side;
-            d.lam[side] = AG[g].lam;
-            d.astr[side] = G[g].astride;
-            for (int f = 0; f < 9; f++) d.fwd[side][f] = fview(f < 3 ? slot_v : slot_s, g, f);
-            d.par[side] = G[g].arr(9);
-            d.grad[side] = AG[g].grad;
-            d.P[side] = G[g].P;
-            d.S2[side] = G[g].S2;
+        IfAdj<T> a{};
+        const DevGrid<T> &F = G[k], &C = G[k + 1];
+        for (int f = 0; f < 9; f++) {
+            a.lf[f] = AG[k].lam + (long)f * F.astride;
+            a.lc[f] = AG[k + 1].lam + (long)f * C.astride;
+            a.ff[f] = fview(f < 3 ? slot_v : slot_s, k, f);
+            a.fc[f] = fview(f < 3 ? slot_v : slot_s, k + 1, f);
+            a.cf[f] = F.dev.c[f];
+            a.cc[f] = C.dev.c[f];
         }
-        return d;
+        for (int m = 0; m < 12; m++) {
+            a.gf[m] = AG[k].grad + (long)m * F.astride;
+            a.gc[m] = AG[k + 1].grad + (long)m * C.astride;
+        }
+        a.bf = F.dev.b; a.bc = C.dev.b;
+        a.oF = F.o; a.PF = F.P; a.SF = F.S2; a.oC = C.o; a.PC = C.P; a.SC = C.S2;
+        a.r = I[k].r; a.nx2 = I[k].nx2; a.ny2 = I[k].ny2; a.nz1 = I[k].nz1;
+        a.k1x = (double)F.dev.cx; a.k1y = (double)F.dev.cy; a.k1z = (double)F.dev.cz;
+        a.k2x = (double)C.dev.cx; a.k2y = (double)C.dev.cy; a.k2z = (double)C.dev.cz;
+        return a;
     }
 
     AdjGridDev<T> adj_dev(int g, int slot_v, int slot_s)
@@ -1564,25 +1323,6 @@ public:
         if (!m.n) return;
         hipLaunchKernelGGL((k_mirror_T<T>), dim3(nb(m.n)), dim3(256), 0, s, AG[g].lam, m.src, m.start, m.halo, m.n);
         hipLaunchKernelGGL((k_zero_list<T>), dim3(nb(m.nh)), dim3(256), 0, s, AG[g].lam, m.halo, m.nh);
-    }
-
-    void launch_op_pre(DevOp &op, const TLDev<T> &d, double *rhs, const double *den)
-    {
-        if (op.fine.n)
-            hipLaunchKernelGGL((k_tl_fine<T>), dim3(nb(op.fine.n)), dim3(256), 0, s, d, op.fine.ukey, op.fine.start, op.fine.tkey,
-                               op.fine.cst, op.fine.n);
-        if (op.nz) hipLaunchKernelGGL((k_tl_zero<T>), dim3(nb(op.nz)), dim3(256), 0, s, d, op.zkey, op.nz);
-        if (op.nx) hipLaunchKernelGGL((k_tl_readx<T>), dim3(nb(op.nx)), dim3(256), 0, s, d, op.xkey, op.xrow, den, rhs, op.nx);
-    }
-
-    void launch_op_post(DevOp &op, const TLDev<T> &d, const double *mu)
-    {
-        if (op.prop.n)
-            hipLaunchKernelGGL((k_tl_prop<T>), dim3(nb(op.prop.n)), dim3(256), 0, s, d, op.prop.ukey, op.prop.start, op.prop.row,
-                               op.prop.tkey, op.prop.cst, mu, op.prop.n);
-        if (op.grad.n)
-            hipLaunchKernelGGL((k_tl_grad<T>), dim3(nb(op.grad.n)), dim3(256), 0, s, d, op.grad.ukey, op.grad.start, op.grad.row,
-                               op.grad.tkey, op.grad.cst, mu, op.grad.n);
     }
 
     int launch_jacobi_adj(int k0, int n)
@@ -1618,28 +1358,51 @@ public:
     int adj_step_impl(int n, int slot_f, int slot_p, bool inject)
     {
         const int ng = (int)G.size(), ni = (int)I.size();
+        // vy-syz (pointwise)
         for (int k = ni - 1; k >= 0; k--) {
-            TLDev<T> d = tl_dev(k, slot_f, slot_f);
-            launch_op_pre(AI[k].op[2], d, AI[k].mu_syz, AI[k].den);
-            launch_op_post(AI[k].op[2], d, AI[k].mu_syz);
+            IfAdj<T> a = if_adj(k, slot_f, slot_f);
+            const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
+            hipLaunchKernelGGL((k_asyz_ghost<T>), dim3(nb((long)a.nx2 * a.ny2)), dim3(256), 0, s, a, (const double *)AI[k].den, AI[k].mu_syz);
+            hipLaunchKernelGGL((k_asyz_fine0<T>), dim3(nb((long)nx1 * ny1)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_asyz_fine<T>), dim3(nb((long)(nx1 + 1) * (ny1 + 1))), dim3(256), 0, s, a, (const double *)AI[k].mu_syz);
+            hipLaunchKernelGGL((k_asyz_coarse<T>), dim3(nb((long)(a.nx2 + 1) * (a.ny2 + 1))), dim3(256), 0, s, a,
+                               (const double *)AI[k].mu_syz);
         }
+        // vx-sxz (Jacobi)
         if (ni) {
-            for (int k = ni - 1; k >= 0; k--) launch_op_pre(AI[k].op[1], tl_dev(k, slot_f, slot_f), AI[k].Ba, nullptr);
+            for (int k = ni - 1; k >= 0; k--) {
+                IfAdj<T> a = if_adj(k, slot_f, slot_f);
+                const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
+                hipLaunchKernelGGL((k_asxz_ghost<T>), dim3(nb((long)(a.nx2 - 1) * (a.ny2 - 1))), dim3(256), 0, s, a, AI[k].Ba);
+                hipLaunchKernelGGL((k_asxz_fine0<T>), dim3(nb((long)(nx1 - 1) * (ny1 - 1))), dim3(256), 0, s, a);
+            }
             for (int k0 = 0; k0 < ni; k0 += MAXI)
                 if (int rc = launch_jacobi_adj(k0, std::min(MAXI, ni - k0))) return rc;
-            for (int k = ni - 1; k >= 0; k--) launch_op_post(AI[k].op[1], tl_dev(k, slot_f, slot_f), AI[k].X0a);
+            for (int k = ni - 1; k >= 0; k--) {
+                IfAdj<T> a = if_adj(k, slot_f, slot_f);
+                const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
+                hipLaunchKernelGGL((k_asxz_fine<T>), dim3(nb((long)(nx1 + 1) * (ny1 + 1))), dim3(256), 0, s, a, (const double *)AI[k].X0a);
+                hipLaunchKernelGGL((k_asxz_coarse<T>), dim3(nb((long)(a.nx2 + 1) * (a.ny2 + 1))), dim3(256), 0, s, a,
+                                   (const double *)AI[k].X0a);
+            }
         }
+        // stress stencils
         for (int g = ng - 1; g >= 0; g--) {
             launch_mirror(g, AG[g].sm);
             AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
             hipLaunchKernelGGL((k_adj_stress<T>), dim3(a.gx * a.gy * a.gz), dim3(BX, ABY), 0, s, a);
         }
+        // vz-szz (tridiagonal)
         for (int k = ni - 1; k >= 0; k--) {
-            TLDev<T> d = tl_dev(k, slot_f, slot_p);
-            launch_op_pre(AI[k].op[0], d, AI[k].mu_vz, nullptr);
+            IfAdj<T> a = if_adj(k, slot_f, slot_p);
+            const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
+            hipLaunchKernelGGL((k_avz_ghost<T>), dim3(nb((long)a.nx2 * (a.ny2 + 1))), dim3(256), 0, s, a, AI[k].mu_vz);
+            hipLaunchKernelGGL((k_avz_fine0<T>), dim3(nb((long)nx1 * (ny1 - 1))), dim3(256), 0, s, a);
             hipLaunchKernelGGL(k_tri_solve, dim3((I[k].nx2 + 63) / 64), dim3(64), 0, s, AI[k].mu_vz, I[k].M, I[k].Dp, I[k].U,
                                I[k].nx2, I[k].ny2 - 1);
-            launch_op_post(AI[k].op[0], d, AI[k].mu_vz);
+            hipLaunchKernelGGL((k_avz_fine<T>), dim3(nb((long)(nx1 + 1) * (ny1 + 1))), dim3(256), 0, s, a, (const double *)AI[k].mu_vz);
+            hipLaunchKernelGGL((k_avz_coarse<T>), dim3(nb((long)(a.nx2 + 1) * (a.ny2 + 1))), dim3(256), 0, s, a,
+                               (const double *)AI[k].mu_vz);
         }
         if (inject && nrloc) {
             RecAdj<T> ra{};
