@@ -30,133 +30,169 @@ struct AdjGridDev {
     T *gb[3];           // gradient w.r.t. buoyancy as used by vx (node), vy (xy mean), vz (xz mean)
     const T *c[9];
     const T *b;
+    const T *ac[12];    // masked coefficients (k_adj_coef): normal 11 22 33 23 13 12, 1/2 c44 c55 c66, b of vx vy vz
     int nx, ny, nz;
     long P, S2, o;
     T cx, cy, cz;
     int top_eff, bottom, free_top;
-    int gx, gy, gz;     // tiling of the index box [-1, n] x [-1, n] x [-1, n]
+    int gx, gy, gz;     // workgroup grid
 };
 
-constexpr int ABY = 4;   // adjoint stencil workgroup: 64 x ABY threads
-constexpr int AZC = 8;   // z planes per thread
+// Masked coefficients of the transposed stencils: the coefficient of each
+// update at the points of that update's set, zero elsewhere (halos
+// included), so that the adjoint weights need no set tests.  Normal block on
+// the free-surface row: the condensed moduli of update_stress.cpp:37-65.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_coef(GridDev<T> g, T *out, long astride)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long nxy = (long)(nx + 1) * (ny + 1);
+    if (t >= nxy * (nz + 1)) return;
+    const int ix = (int)(t % (nx + 1)), iy = (int)((t / (nx + 1)) % (ny + 1)), iz = (int)(t / nxy);
+    const long p = g.o + (long)iz * g.S2 + (long)iy * g.P + ix;
+    T *o[12];
+    for (int k = 0; k < 12; k++) o[k] = out + (long)k * astride;
+    const T *b = g.b;
+    if (ix <= nx - 1) {
+        const T c11 = g.c[0][p], c22 = g.c[1][p], c33 = g.c[2][p], c23 = g.c[3][p], c13 = g.c[4][p], c12 = g.c[5][p];
+        if (g.free_top && iz == 0) {
+            o[0][p] = c11 - c13 * c13 / c33;
+            o[1][p] = c22 - c23 * c23 / c33;
+            o[5][p] = c12 - c13 * c23 / c33;
+        } else {
+            o[0][p] = c11; o[1][p] = c22; o[2][p] = c33; o[3][p] = c23; o[4][p] = c13; o[5][p] = c12;
+        }
+    }
+    if (ix <= nx - 1 && iy <= ny - 1 && iz <= nz - 1) o[6][p] = T(0.5) * g.c[6][p];
+    if (iz <= nz - 1) o[7][p] = T(0.5) * g.c[7][p];
+    if (iy <= ny - 1) o[8][p] = T(0.5) * g.c[8][p];
+    const bool zin = iz >= g.top_eff && iz <= nz - g.bottom;
+    if (ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && zin) o[9][p] = b[p];
+    if (ix <= nx - 1 && iy <= ny - 1 && zin) o[10][p] = T(0.25) * (b[p] + b[p + 1] + b[p + g.P] + b[p + g.P + 1]);
+    if (ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && iz <= nz - 1)
+        o[11][p] = T(0.25) * (b[p] + b[p + g.S2] + b[p + 1] + b[p + g.S2 + 1]);
+}
+
 
 // ---------------------------------------------------------------------------
 // Transpose of UpdateStress (stress stencil) on the velocities, with the
 // gradient w.r.t. the nine staged stiffnesses.  Forward:
 //   normal:  sxx += c11 cx dvx + c12 cy dvy + c13 cz dvz (free row: condensed moduli)
 //   syz += 1/2 c44 (cz dz vy + cy dy vz), sxz += 1/2 c55 (...), sxy += 1/2 c66 (...)
-// The adjoint weights e (normal) and h (shear) live on the stress points and
-// vanish outside each stress field's update set.
-template <typename T>
-__device__ __forceinline__ void adj_normal_w(const AdjGridDev<T> &a, int ix, int iy, int iz, long p, T &ex, T &ey, T &ez)
-{
-    ex = T(0); ey = T(0); ez = T(0);
-    if (ix < 0 || ix > a.nx - 1 || iy < 0 || iy > a.ny || iz < 0 || iz > a.nz) return;
-    const T lx = a.lam[3][p], ly = a.lam[4][p];
-    const T c11 = a.c[0][p], c22 = a.c[1][p], c33 = a.c[2][p], c23 = a.c[3][p], c13 = a.c[4][p], c12 = a.c[5][p];
-    if (a.free_top && iz == 0) {
-        ex = a.cx * ((c11 - c13 * c13 / c33) * lx + (c12 - c23 * c13 / c33) * ly);
-        ey = a.cy * ((c12 - c13 * c23 / c33) * lx + (c22 - c23 * c23 / c33) * ly);
-    } else {
-        const T lz = a.lam[5][p];
-        ex = a.cx * (c11 * lx + c12 * ly + c13 * lz);
-        ey = a.cy * (c12 * lx + c22 * ly + c23 * lz);
-        ez = a.cz * (c13 * lx + c23 * ly + c33 * lz);
-    }
-}
-template <typename T>
-__device__ __forceinline__ T adj_hyz(const AdjGridDev<T> &a, int ix, int iy, int iz, long p)
-{
-    if (ix < 0 || ix > a.nx - 1 || iy < 0 || iy > a.ny - 1 || iz < 0 || iz > a.nz - 1) return T(0);
-    return T(0.5) * a.c[6][p] * a.lam[6][p];
-}
-template <typename T>
-__device__ __forceinline__ T adj_hxz(const AdjGridDev<T> &a, int ix, int iy, int iz, long p)
-{
-    if (ix < 0 || ix > a.nx || iy < 0 || iy > a.ny || iz < 0 || iz > a.nz - 1) return T(0);
-    return T(0.5) * a.c[7][p] * a.lam[7][p];
-}
-template <typename T>
-__device__ __forceinline__ T adj_hxy(const AdjGridDev<T> &a, int ix, int iy, int iz, long p)
-{
-    if (ix < 0 || ix > a.nx || iy < 0 || iy > a.ny - 1 || iz < 0 || iz > a.nz) return T(0);
-    return T(0.5) * a.c[8][p] * a.lam[8][p];
-}
+// The adjoint weights e = C lam (normal) and h = 1/2 c lam (shear) live on
+// the stress points and vanish outside each stress field's update set.
+// Adjoint stencil workgroups: 64 x AWY threads, each wavefront one x row of
+// 64 lanes of which lanes 1..AXO produce outputs (the outer lanes only supply
+// the x neighbours through wavefront shuffles); each thread marches AZ2 z
+// planes, carrying its z neighbours and the y neighbours it needs as
+// one-plane lookaheads in registers.  No barriers.
+constexpr int AWY = 4;
+constexpr int AXO = 62;
+constexpr int AZ2 = 16;
 
 template <typename T>
-__global__ __launch_bounds__(BX *ABY) void k_adj_stress(AdjGridDev<T> a)
+__device__ __forceinline__ T shfl_dn(T v) { return __shfl_down(v, 1, 64); }
+template <typename T>
+__device__ __forceinline__ T shfl_upw(T v) { return __shfl_up(v, 1, 64); }
+
+template <typename T>
+__global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
 {
-    const int bx = blockIdx.x % a.gx;
-    const int rest = blockIdx.x / a.gx;
+    const int tile = xcd_tile(blockIdx.x, a.gx * a.gy * a.gz);
+    const int bx = tile % a.gx;
+    const int rest = tile / a.gx;
     const int by = rest % a.gy;
     const int bz = rest / a.gy;
-    const int ix = bx * BX + threadIdx.x - 1;
-    const int iy = by * ABY + threadIdx.y - 1;
+    const int lane = threadIdx.x;
+    const int ix = bx * AXO + lane - 2;
+    const int iy = by * AWY + threadIdx.y - 1;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    if (ix > nx || iy > ny) return;
-    const int z0 = bz * AZC - 1;
-    const int z1 = min(z0 + AZC, nz + 1);
+    if (iy > ny) return;  // whole wavefront
+    const bool inx = ix >= -1 && ix <= nx + 1;
+    const bool out = lane >= 1 && lane <= AXO && ix <= nx;
+    const int z0 = bz * AZ2 - 1;
+    const int z1 = min(z0 + AZ2, nz + 1);
     const long P = a.P, S2 = a.S2;
     const T cx = a.cx, cy = a.cy, cz = a.cz;
+    const T *a11 = a.ac[0], *a22 = a.ac[1], *a33 = a.ac[2], *a23 = a.ac[3], *a13 = a.ac[4], *a12 = a.ac[5];
+    const T *h44 = a.ac[6], *h55 = a.ac[7], *h66 = a.ac[8];
+    const T *lxx = a.lam[3], *lyy = a.lam[4], *lzz = a.lam[5], *lyz = a.lam[6], *lxz = a.lam[7], *lxy = a.lam[8];
+    const bool ym_ok = iy - 1 >= -1;
+    const long qb = a.o + (long)iy * P + ix;
+    // adjoint weights at plane z (zero outside the array box)
+    auto wts = [&](int z, T &ex, T &ey, T &ez, T &hyz, T &hxz, T &hxy, T &eyp, T &hxym, T &hyzm) {
+        ex = ey = ez = hyz = hxz = hxy = eyp = hxym = hyzm = T(0);
+        if (!inx || z < -1 || z > nz + 1) return;
+        const long q = qb + (long)z * S2;
+        const T l1 = lxx[q], l2 = lyy[q], l3 = lzz[q];
+        const T A11 = a11[q], A22 = a22[q], A33 = a33[q], A23 = a23[q], A13 = a13[q], A12 = a12[q];
+        ex = cx * (A11 * l1 + A12 * l2 + A13 * l3);
+        ey = cy * (A12 * l1 + A22 * l2 + A23 * l3);
+        ez = cz * (A13 * l1 + A23 * l2 + A33 * l3);
+        hyz = h44[q] * lyz[q];
+        hxz = h55[q] * lxz[q];
+        hxy = h66[q] * lxy[q];
+        const long qp = q + P;
+        eyp = cy * (a12[qp] * lxx[qp] + a22[qp] * lyy[qp] + a23[qp] * lzz[qp]);
+        if (ym_ok) {
+            const long qm = q - P;
+            hxym = h66[qm] * lxy[qm];
+            hyzm = h44[qm] * lyz[qm];
+        }
+    };
+    T ex, ey, ez, hyz, hxz, hxy, eyp, hxym, hyzm;
+    T d0, d1, d2, d3, d4, d5, d6, hyz_m, hxz_m;
+    wts(z0 - 1, d0, d1, d2, hyz_m, hxz_m, d3, d4, d5, d6);
+    wts(z0, ex, ey, ez, hyz, hxz, hxy, eyp, hxym, hyzm);
     for (int iz = z0; iz < z1; iz++) {
-        const long q = a.o + (long)iz * S2 + (long)iy * P + ix;
-        T ex, ey, ez, exm, eyp, ezp, dum;
-        adj_normal_w(a, ix, iy, iz, q, ex, ey, ez);
-        adj_normal_w(a, ix - 1, iy, iz, q - 1, exm, dum, dum);
-        adj_normal_w(a, ix, iy + 1, iz, q + P, dum, eyp, dum);
-        adj_normal_w(a, ix, iy, iz + 1, q + S2, dum, dum, ezp);
-        const T hyz = adj_hyz(a, ix, iy, iz, q);
-        const T hxz = adj_hxz(a, ix, iy, iz, q);
-        const T hxy = adj_hxy(a, ix, iy, iz, q);
-        // vx on [0, nx] x [0, ny] x [0, nz]
-        if (ix >= 0 && iy >= 0 && iz >= 0 && iz <= nz) {
-            const T d = (exm - ex) + cz * (adj_hxz(a, ix, iy, iz - 1, q - S2) - hxz) + cy * (adj_hxy(a, ix, iy - 1, iz, q - P) - hxy);
-            a.lam[0][q] += d;
-        }
-        // vy on [-1, nx] x [-1, ny] x [0, nz]
-        if (iz >= 0 && iz <= nz) {
-            const T d = (ey - eyp) + cz * (adj_hyz(a, ix, iy, iz - 1, q - S2) - hyz) + cx * (hxy - adj_hxy(a, ix + 1, iy, iz, q + 1));
-            a.lam[1][q] += d;
-        }
-        // vz on [-1, nx] x [0, ny] x [-1, nz]
-        if (iy >= 0) {
-            const T d = (ez - ezp) + cy * (adj_hyz(a, ix, iy - 1, iz, q - P) - hyz) + cx * (hxz - adj_hxz(a, ix + 1, iy, iz, q + 1));
-            a.lam[2][q] += d;
-        }
-        // gradient w.r.t. the staged stiffnesses at this stress point
-        if (ix >= 0 && iy >= 0 && iz >= 0) {
-            const T *fvx = a.fv[0], *fvy = a.fv[1], *fvz = a.fv[2];
-            if (ix <= nx - 1) {
-                const T lx = a.lam[3][q], ly = a.lam[4][q];
-                const T X = cx * (fvx[q + 1] - fvx[q]);
-                const T Y = cy * (fvy[q] - fvy[q - P]);
-                if (a.free_top && iz == 0) {
-                    const T c33 = a.c[2][q], c23 = a.c[3][q], c13 = a.c[4][q];
-                    const T u = c13 / c33, v = c23 / c33;
-                    a.gc[0][q] += lx * X;
-                    a.gc[1][q] += ly * Y;
-                    a.gc[5][q] += lx * Y + ly * X;
-                    a.gc[4][q] += lx * (T(-2) * u * X - v * Y) + ly * (-v * X);
-                    a.gc[3][q] += lx * (-u * Y) + ly * (-u * X - T(2) * v * Y);
-                    a.gc[2][q] += lx * (u * u * X + u * v * Y) + ly * (v * u * X + v * v * Y);
-                } else {
-                    const T lz = a.lam[5][q];
-                    const T Z = cz * (fvz[q] - fvz[q - S2]);
-                    a.gc[0][q] += lx * X;
-                    a.gc[1][q] += ly * Y;
-                    a.gc[2][q] += lz * Z;
-                    a.gc[3][q] += ly * Z + lz * Y;
-                    a.gc[4][q] += lx * Z + lz * X;
-                    a.gc[5][q] += lx * Y + ly * X;
+        const long q = qb + (long)iz * S2;
+        T ex_n, ey_n, ez_n, hyz_n, hxz_n, hxy_n, eyp_n, hxym_n, hyzm_n;
+        wts(iz + 1, ex_n, ey_n, ez_n, hyz_n, hxz_n, hxy_n, eyp_n, hxym_n, hyzm_n);
+        const T ex_xm = shfl_upw(ex);
+        const T hxz_xp = shfl_dn(hxz), hxy_xp = shfl_dn(hxy);
+        if (out) {
+            if (ix >= 0 && iy >= 0 && iz >= 0) a.lam[0][q] += (ex_xm - ex) + cz * (hxz_m - hxz) + cy * (hxym - hxy);
+            if (iz >= 0) a.lam[1][q] += (ey - eyp) + cz * (hyz_m - hyz) + cx * (hxy - hxy_xp);
+            if (iy >= 0) a.lam[2][q] += (ez - ez_n) + cy * (hyzm - hyz) + cx * (hxz - hxz_xp);
+            // gradient w.r.t. the staged stiffnesses at this stress point
+            if (ix >= 0 && iy >= 0 && iz >= 0) {
+                const T *fvx = a.fv[0], *fvy = a.fv[1], *fvz = a.fv[2];
+                if (ix <= nx - 1) {
+                    const T l1 = lxx[q], l2 = lyy[q];
+                    const T X = cx * (fvx[q + 1] - fvx[q]);
+                    const T Y = cy * (fvy[q] - fvy[q - P]);
+                    if (a.free_top && iz == 0) {
+                        const T c33 = a.c[2][q], c23 = a.c[3][q], c13 = a.c[4][q];
+                        const T u = c13 / c33, v = c23 / c33;
+                        a.gc[0][q] += l1 * X;
+                        a.gc[1][q] += l2 * Y;
+                        a.gc[5][q] += l1 * Y + l2 * X;
+                        a.gc[4][q] += l1 * (T(-2) * u * X - v * Y) + l2 * (-v * X);
+                        a.gc[3][q] += l1 * (-u * Y) + l2 * (-u * X - T(2) * v * Y);
+                        a.gc[2][q] += l1 * (u * u * X + u * v * Y) + l2 * (v * u * X + v * v * Y);
+                    } else {
+                        const T l3 = lzz[q];
+                        const T Z = cz * (fvz[q] - fvz[q - S2]);
+                        a.gc[0][q] += l1 * X;
+                        a.gc[1][q] += l2 * Y;
+                        a.gc[2][q] += l3 * Z;
+                        a.gc[3][q] += l2 * Z + l3 * Y;
+                        a.gc[4][q] += l1 * Z + l3 * X;
+                        a.gc[5][q] += l1 * Y + l2 * X;
+                    }
                 }
+                if (ix <= nx - 1 && iy <= ny - 1 && iz <= nz - 1)
+                    a.gc[6][q] += lyz[q] * (T(0.5) * (cz * (fvy[q + S2] - fvy[q]) + cy * (fvz[q + P] - fvz[q])));
+                if (iz <= nz - 1)
+                    a.gc[7][q] += lxz[q] * (T(0.5) * (cz * (fvx[q + S2] - fvx[q]) + cx * (fvz[q] - fvz[q - 1])));
+                if (iy <= ny - 1)
+                    a.gc[8][q] += lxy[q] * (T(0.5) * (cy * (fvx[q + P] - fvx[q]) + cx * (fvy[q] - fvy[q - 1])));
             }
-            if (ix <= nx - 1 && iy <= ny - 1 && iz <= nz - 1)
-                a.gc[6][q] += a.lam[6][q] * (T(0.5) * (cz * (fvy[q + S2] - fvy[q]) + cy * (fvz[q + P] - fvz[q])));
-            if (iz <= nz - 1)
-                a.gc[7][q] += a.lam[7][q] * (T(0.5) * (cz * (fvx[q + S2] - fvx[q]) + cx * (fvz[q] - fvz[q - 1])));
-            if (iy <= ny - 1)
-                a.gc[8][q] += a.lam[8][q] * (T(0.5) * (cy * (fvx[q + P] - fvx[q]) + cx * (fvy[q] - fvy[q - 1])));
         }
+        hyz_m = hyz; hxz_m = hxz;
+        ex = ex_n; ey = ey_n; ez = ez_n; hyz = hyz_n; hxz = hxz_n; hxy = hxy_n;
+        eyp = eyp_n; hxym = hxym_n; hyzm = hyzm_n;
     }
 }
 
@@ -166,70 +202,75 @@ __global__ __launch_bounds__(BX *ABY) void k_adj_stress(AdjGridDev<T> a)
 //   vx += b (cx dx sxx + cy dy sxy + cz dz sxz)                on the vx update set
 //   vy += mean_xy(b) (cx dx sxy + cz dz syz + cy dy syy)       on the vy update set
 //   vz += mean_xz(b) (cx dx sxz + cz dz szz + cy dy syz)       on the vz update set
+// with the adjoint weights w = (masked buoyancy) * lam_v.
 template <typename T>
-__device__ __forceinline__ T adj_wx(const AdjGridDev<T> &a, int ix, int iy, int iz, long p)
+__global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
 {
-    if (ix < 1 || ix > a.nx - 1 || iy < 1 || iy > a.ny - 1 || iz < a.top_eff || iz > a.nz - a.bottom) return T(0);
-    return a.b[p] * a.lam[0][p];
-}
-template <typename T>
-__device__ __forceinline__ T adj_wy(const AdjGridDev<T> &a, int ix, int iy, int iz, long p)
-{
-    if (ix < 0 || ix > a.nx - 1 || iy < 0 || iy > a.ny - 1 || iz < a.top_eff || iz > a.nz - a.bottom) return T(0);
-    const T *b = a.b;
-    return T(0.25) * (b[p] + b[p + 1] + b[p + a.P] + b[p + a.P + 1]) * a.lam[1][p];
-}
-template <typename T>
-__device__ __forceinline__ T adj_wz(const AdjGridDev<T> &a, int ix, int iy, int iz, long p)
-{
-    if (ix < 0 || ix > a.nx - 1 || iy < 1 || iy > a.ny - 1 || iz < 0 || iz > a.nz - 1) return T(0);
-    const T *b = a.b;
-    return T(0.25) * (b[p] + b[p + a.S2] + b[p + 1] + b[p + a.S2 + 1]) * a.lam[2][p];
-}
-
-template <typename T>
-__global__ __launch_bounds__(BX *ABY) void k_adj_vel(AdjGridDev<T> a)
-{
-    const int bx = blockIdx.x % a.gx;
-    const int rest = blockIdx.x / a.gx;
+    const int tile = xcd_tile(blockIdx.x, a.gx * a.gy * a.gz);
+    const int bx = tile % a.gx;
+    const int rest = tile / a.gx;
     const int by = rest % a.gy;
     const int bz = rest / a.gy;
-    const int ix = bx * BX + threadIdx.x;
-    const int iy = by * ABY + threadIdx.y;
+    const int lane = threadIdx.x;
+    const int ix = bx * AXO + lane - 1;
+    const int iy = by * AWY + threadIdx.y;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    if (ix > nx || iy > ny) return;
-    const int z0 = bz * AZC - 1;
-    const int z1 = min(z0 + AZC, nz + 1);
+    if (iy > ny) return;  // whole wavefront
+    const bool inx = ix >= -1 && ix <= nx + 1;
+    const bool out = lane >= 1 && lane <= AXO && ix <= nx;
+    const int z0 = bz * AZ2 - 1;
+    const int z1 = min(z0 + AZ2, nz + 1);
     const long P = a.P, S2 = a.S2;
     const T cx = a.cx, cy = a.cy, cz = a.cz;
+    const T *mbx = a.ac[9], *mby = a.ac[10], *mbz = a.ac[11];
+    const T *lvx = a.lam[0], *lvy = a.lam[1], *lvz = a.lam[2];
+    const bool ym_ok = iy - 1 >= -1;
+    const long qb = a.o + (long)iy * P + ix;
+    auto wts = [&](int z, T &wx, T &wy, T &wz, T &wxp, T &wzp, T &wym) {
+        wx = wy = wz = wxp = wzp = wym = T(0);
+        if (!inx || z < -1 || z > nz + 1) return;
+        const long q = qb + (long)z * S2;
+        wx = mbx[q] * lvx[q];
+        wy = mby[q] * lvy[q];
+        wz = mbz[q] * lvz[q];
+        wxp = mbx[q + P] * lvx[q + P];
+        wzp = mbz[q + P] * lvz[q + P];
+        if (ym_ok) wym = mby[q - P] * lvy[q - P];
+    };
+    T wx, wy, wz, wxp, wzp, wym, d0, d1, d2, d3, d4, wz_m;
+    wts(z0 - 1, d0, d1, wz_m, d2, d3, d4);
+    wts(z0, wx, wy, wz, wxp, wzp, wym);
     for (int iz = z0; iz < z1; iz++) {
-        const long q = a.o + (long)iz * S2 + (long)iy * P + ix;
-        const T wx = adj_wx(a, ix, iy, iz, q);
-        const T wy = adj_wy(a, ix, iy, iz, q);
-        const T wz = adj_wz(a, ix, iy, iz, q);
-        if (iz >= 0) {
-            if (ix <= nx - 1) {
-                a.lam[3][q] += cx * (wx - adj_wx(a, ix + 1, iy, iz, q + 1));
-                a.lam[4][q] += cy * (adj_wy(a, ix, iy - 1, iz, q - P) - wy);
-                a.lam[5][q] += cz * (adj_wz(a, ix, iy, iz - 1, q - S2) - wz);
+        const long q = qb + (long)iz * S2;
+        T wx_n, wy_n, wz_n, wxp_n, wzp_n, wym_n;
+        wts(iz + 1, wx_n, wy_n, wz_n, wxp_n, wzp_n, wym_n);
+        const T wx_xp = shfl_dn(wx);
+        const T wy_xm = shfl_upw(wy), wz_xm = shfl_upw(wz);
+        if (out) {
+            if (iz >= 0) {
+                if (ix <= nx - 1) {
+                    a.lam[3][q] += cx * (wx - wx_xp);
+                    a.lam[4][q] += cy * (wym - wy);
+                    a.lam[5][q] += cz * (wz_m - wz);
+                }
+                if (iy <= ny - 1) a.lam[8][q] += cy * (wx - wxp) + cx * (wy_xm - wy);
             }
-            if (iy <= ny - 1)
-                a.lam[8][q] += cy * (wx - adj_wx(a, ix, iy + 1, iz, q + P)) + cx * (adj_wy(a, ix - 1, iy, iz, q - 1) - wy);
+            a.lam[7][q] += cz * (wx - wx_n) + cx * (wz_xm - wz);
+            if (ix <= nx - 1 && iy <= ny - 1) a.lam[6][q] += cz * (wy - wy_n) + cy * (wz - wzp);
+            // gradient w.r.t. buoyancy at this velocity point, forward s^n
+            if (iz >= 0) {
+                const T *sxx = a.fs[0], *syy = a.fs[1], *szz = a.fs[2], *syz = a.fs[3], *sxz = a.fs[4], *sxy = a.fs[5];
+                const bool zin = iz >= a.top_eff && iz <= nz - a.bottom;
+                if (ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && zin)
+                    a.gb[0][q] += lvx[q] * (cx * (sxx[q] - sxx[q - 1]) + cy * (sxy[q] - sxy[q - P]) + cz * (sxz[q] - sxz[q - S2]));
+                if (ix <= nx - 1 && iy <= ny - 1 && zin)
+                    a.gb[1][q] += lvy[q] * (cx * (sxy[q + 1] - sxy[q]) + cz * (syz[q] - syz[q - S2]) + cy * (syy[q + P] - syy[q]));
+                if (ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && iz <= nz - 1)
+                    a.gb[2][q] += lvz[q] * (cx * (sxz[q + 1] - sxz[q]) + cz * (szz[q + S2] - szz[q]) + cy * (syz[q] - syz[q - P]));
+            }
         }
-        a.lam[7][q] += cz * (wx - adj_wx(a, ix, iy, iz + 1, q + S2)) + cx * (adj_wz(a, ix - 1, iy, iz, q - 1) - wz);
-        if (ix <= nx - 1 && iy <= ny - 1)
-            a.lam[6][q] += cz * (wy - adj_wy(a, ix, iy, iz + 1, q + S2)) + cy * (wz - adj_wz(a, ix, iy + 1, iz, q + P));
-        // gradient w.r.t. buoyancy at this velocity point, forward s^n
-        if (iz >= 0) {
-            const T *sxx = a.fs[0], *syy = a.fs[1], *szz = a.fs[2], *syz = a.fs[3], *sxz = a.fs[4], *sxy = a.fs[5];
-            const bool zin = iz >= a.top_eff && iz <= nz - a.bottom;
-            if (ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && zin)
-                a.gb[0][q] += a.lam[0][q] * (cx * (sxx[q] - sxx[q - 1]) + cy * (sxy[q] - sxy[q - P]) + cz * (sxz[q] - sxz[q - S2]));
-            if (ix <= nx - 1 && iy <= ny - 1 && zin)
-                a.gb[1][q] += a.lam[1][q] * (cx * (sxy[q + 1] - sxy[q]) + cz * (syz[q] - syz[q - S2]) + cy * (syy[q + P] - syy[q]));
-            if (ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && iz <= nz - 1)
-                a.gb[2][q] += a.lam[2][q] * (cx * (sxz[q + 1] - sxz[q]) + cz * (szz[q + S2] - szz[q]) + cy * (syz[q] - syz[q - P]));
-        }
+        wz_m = wz;
+        wx = wx_n; wy = wy_n; wz = wz_n; wxp = wxp_n; wzp = wzp_n; wym = wym_n;
     }
 }
 

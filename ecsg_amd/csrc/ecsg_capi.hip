@@ -1084,6 +1084,7 @@ public:
     struct AdjGrid {
         T *lam = nullptr;       // 9 adjoint wavefields
         T *grad = nullptr;      // 12 gradient accumulators (9 staged stiffness, b node, b xy mean, b xz mean)
+        T *coef = nullptr;      // 12 masked coefficients of the transposed stencils
         MirrorDev vA, vB, sm;   // transposed ghost mirrors: vy y walls, other velocity mirrors, free surface
     };
     std::vector<AdjIface> AI;
@@ -1124,7 +1125,7 @@ public:
             dfree(a.stats);
         }
         for (auto &g : AG) {
-            dfree(g.lam); dfree(g.grad);
+            dfree(g.lam); dfree(g.grad); dfree(g.coef);
             for (MirrorDev *m : {&g.vA, &g.vB, &g.sm}) { dfree(m->src); dfree(m->start); dfree(m->halo); }
         }
         AI.clear(); AG.clear();
@@ -1221,6 +1222,7 @@ public:
         if (int rc = upload_mirror(S, a.sm)) return rc;
         if (int rc = dalloc(&a.lam, (size_t)9 * as)) return rc;
         if (int rc = dalloc(&a.grad, (size_t)12 * as)) return rc;
+        if (int rc = dalloc(&a.coef, (size_t)12 * as)) return rc;
         return ECSG_OK;
     }
 
@@ -1310,9 +1312,12 @@ public:
         a.P = d.P; a.S2 = d.S2; a.o = d.o;
         a.cx = d.dev.cx; a.cy = d.dev.cy; a.cz = d.dev.cz;
         a.top_eff = d.dev.top_eff; a.bottom = d.dev.bottom; a.free_top = d.dev.free_top;
-        a.gx = (d.n[0] + 2 + BX - 1) / BX;
-        a.gy = (d.n[1] + 2 + ABY - 1) / ABY;
-        a.gz = (d.n[2] + 2 + AZC - 1) / AZC;
+        // output boxes: [-1, n] x [-1, n] (stress transpose) or [0, n] x [0, n]
+        // (velocity transpose) in x, y; [-1, n] in z
+        for (int m = 0; m < 12; m++) a.ac[m] = AG[g].coef + (long)m * d.astride;
+        a.gx = (d.n[0] + 2 + AXO - 1) / AXO;
+        a.gy = (d.n[1] + 2 + AWY - 1) / AWY;
+        a.gz = (d.n[2] + 2 + AZ2 - 1) / AZ2;
         return a;
     }
 
@@ -1390,7 +1395,7 @@ public:
         for (int g = ng - 1; g >= 0; g--) {
             launch_mirror(g, AG[g].sm);
             AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
-            hipLaunchKernelGGL((k_adj_stress<T>), dim3(a.gx * a.gy * a.gz), dim3(BX, ABY), 0, s, a);
+            hipLaunchKernelGGL((k_adj_stress<T>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
         }
         // vz-szz (tridiagonal)
         for (int k = ni - 1; k >= 0; k--) {
@@ -1414,15 +1419,28 @@ public:
             launch_mirror(g, AG[g].vA);
             launch_mirror(g, AG[g].vB);
             AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
-            hipLaunchKernelGGL((k_adj_vel<T>), dim3(a.gx * a.gy * a.gz), dim3(BX, ABY), 0, s, a);
+            hipLaunchKernelGGL((k_adj_vel<T>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
         }
         HIPCHK(hipGetLastError());
         return ECSG_OK;
     }
     int resid_steps = 0;
 
+    int adj_coef()
+    {
+        for (int g = 0; g < (int)G.size(); g++) {
+            const DevGrid<T> &d = G[g];
+            const long np = (long)(d.n[0] + 1) * (d.n[1] + 1) * (d.n[2] + 1);
+            HIPCHK(hipMemsetAsync(AG[g].coef, 0, sizeof(T) * 12 * d.astride, s));
+            hipLaunchKernelGGL((k_adj_coef<T>), dim3(nb(np)), dim3(256), 0, s, d.dev, AG[g].coef, d.astride);
+        }
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
     int adj_zero()
     {
+        if (int rc = adj_coef()) return rc;
         for (int g = 0; g < (int)G.size(); g++) {
             HIPCHK(hipMemsetAsync(AG[g].lam, 0, sizeof(T) * 9 * G[g].astride, s));
             HIPCHK(hipMemsetAsync(AG[g].grad, 0, sizeof(T) * 12 * G[g].astride, s));
@@ -1707,6 +1725,7 @@ public:
     int adjoint_step() override
     {
         if (int rc = adj_build()) return rc;
+        if (int rc = adj_coef()) return rc;
         for (int k = 0; k < (int)I.size(); k++) {
             const long nc = (long)I[k].nx2 * I[k].ny2;
             hipLaunchKernelGGL((k_gsyz_den<T>), dim3(nb(nc)), dim3(256), 0, s, iface_dev(k), AI[k].den);
