@@ -653,6 +653,7 @@ public:
             J.A = f.A; J.B = f.Bs; J.X0 = f.X0; J.X1 = f.X1; J.abs_part = f.absp; J.nabs = f.nabs;
             J.part = f.jpart; J.stats = f.stats;
             J.sxz1 = G[k0 + q].dev.f[7]; J.sxz2 = G[k0 + q + 1].dev.f[7];
+            J.sxz1s = G[k0 + q].dev.st[7]; J.sxz2s = G[k0 + q + 1].dev.st[7];
             J.o1 = G[k0 + q].o; J.P1 = G[k0 + q].P; J.S21 = G[k0 + q].S2;
             J.o2 = G[k0 + q + 1].o; J.P2 = G[k0 + q + 1].P; J.S22 = G[k0 + q + 1].S2;
             J.nx2 = f.nx2; J.ny2 = f.ny2; J.nz1 = f.nz1; J.r = f.r;
@@ -708,6 +709,7 @@ public:
         for (int g = 0; g < (int)G.size(); g++) {
             a.vx[g] = G[g].dev.f[0];
             a.sxx[g] = G[g].dev.f[3]; a.syy[g] = G[g].dev.f[4]; a.szz[g] = G[g].dev.f[5];
+            a.st[g][0] = G[g].dev.st[0]; a.st[g][1] = G[g].dev.st[3]; a.st[g][2] = G[g].dev.st[4]; a.st[g][3] = G[g].dev.st[5];
             a.b[g] = G[g].dev.b;
             a.o[g] = G[g].o; a.P[g] = G[g].P; a.S2[g] = G[g].S2;
             for (int d = 0; d < 3; d++) a.n[g][d] = G[g].n[d];
@@ -1340,7 +1342,7 @@ public:
             JacItf<T> &J = jb.it[q];
             J.A = f.A; J.B = a.Ba; J.X0 = a.X0a; J.X1 = a.X1a; J.abs_part = a.absz; J.nabs = f.nabs;
             J.part = a.jpart; J.stats = a.stats;
-            J.sxz1 = nullptr; J.sxz2 = nullptr;
+            J.sxz1 = nullptr; J.sxz2 = nullptr; J.sxz1s = nullptr; J.sxz2s = nullptr;
             J.nx2 = f.nx2; J.ny2 = f.ny2; J.nz1 = f.nz1; J.r = f.r;
             J.blk0 = blk; J.nblk = f.tiles; J.rho2 = 0;
             blk += J.nblk;
@@ -1577,6 +1579,15 @@ public:
         return ECSG_OK;
     }
 
+    // direct the forward kernels' second writes to ring slot `slot` (-1: off)
+    void set_store(int slot)
+    {
+        const int ng = (int)G.size();
+        for (int g = 0; g < ng; g++)
+            for (int f = 0; f < 9; f++)
+                G[g].dev.st[f] = slot < 0 ? nullptr : ring[(size_t)slot * ng + g] + (long)f * G[g].astride;
+    }
+
     int ring_save(int slot)
     {
         const int ng = (int)G.size();
@@ -1626,8 +1637,11 @@ public:
             if (int rc = ck_restore(j, j * K)) return rc;
             if (int rc = ring_save(0)) return rc;
             for (int t = 0; t < len; t++) {
-                if (int rc = enqueue_step()) return rc;
-                if (int rc = ring_save(t + 1)) return rc;
+                // the step's kernels also write every value into ring slot t+1
+                set_store(t + 1);
+                const int rc = enqueue_step();
+                set_store(-1);
+                if (rc) return rc;
             }
             for (int t = len - 1; t >= 0; t--)
                 if (int rc = adj_step_impl(j * K + t, t + 1, t, true)) return rc;

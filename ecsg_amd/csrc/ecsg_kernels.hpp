@@ -30,6 +30,8 @@ constexpr int MAXG = 8;
 template <typename T>
 struct GridDev {
     T *f[9];        // vx vy vz sxx syy szz syz sxz sxy
+    T *st[9];       // when set: every value written to f (and every owned entry)
+                    // is also written here -- the saved state of this step
     const T *c[9];  // c11 c22 c33 c23 c13 c12 c44 c55 c66 at the stress positions
     const T *b;     // buoyancy, primary nodes
     int nx, ny, nz;
@@ -104,6 +106,7 @@ __global__ __launch_bounds__(BX * BYv) void k_vel(GridDev<T> g, double *__restri
         const bool ok_z = ix <= nx - 1 && iy >= 1 && iy <= ny - 1;
         const int zlo = g.top_eff, zhi = nz - g.bottom;
         const T cx = g.cx, cy = g.cy, cz = g.cz;
+        const bool sv = g.st[0] != nullptr;
         T sxz_m = sxz[p - S2];
         T syz_m = syz[p - S2];
         T szz_c = szz[p];
@@ -146,25 +149,38 @@ __global__ __launch_bounds__(BX * BYv) void k_vel(GridDev<T> g, double *__restri
             }
             // ghost velocities for radius 1 (update_vel.cpp:75-246), same
             // index ranges as the reference loops
-            if (g.top_eff && iz == 0 && ix <= nx - 1) vz[p - S2] = -vz_c;
-            if (g.bottom && iz == nz - 1 && ix <= nx - 1) vz[p + S2] = -vz_c;
+#define PUT(f, arr, i, v)                      \
+    {                                          \
+        const T v_ = (v);                      \
+        arr[i] = v_;                           \
+        if (sv) g.st[f][i] = v_;               \
+    }
+            if (g.top_eff && iz == 0 && ix <= nx - 1) PUT(2, vz, p - S2, -vz_c)
+            if (g.bottom && iz == nz - 1 && ix <= nx - 1) PUT(2, vz, p + S2, -vz_c)
             if (ix == 0) {
-                if (iz <= nz - 1) vz[p - 1] = -vz_c;
-                if (iy <= ny - 1) vy[p - 1] = -vy_c;
+                if (iz <= nz - 1) PUT(2, vz, p - 1, -vz_c)
+                if (iy <= ny - 1) PUT(1, vy, p - 1, -vy_c)
             }
             if (ix == nx - 1) {
-                if (iz <= nz - 1) vz[p + 1] = -vz_c;
-                if (iy <= ny - 1) vy[p + 1] = -vy_c;
+                if (iz <= nz - 1) PUT(2, vz, p + 1, -vz_c)
+                if (iy <= ny - 1) PUT(1, vy, p + 1, -vy_c)
             }
             if (ix <= nx - 1) {
-                if (iy == 0 && iz <= g.zy) vy[p - P] = -vy_c;
+                if (iy == 0 && iz <= g.zy) PUT(1, vy, p - P, -vy_c)
                 if (iy == ny - 1) {
                     // the x mirrors of row iy = ny copy that halo row before
                     // the y mirror below overwrites it (reference loop order)
-                    if (ix == 0) vy[p + P - 1] = -vy[p + P];
-                    if (ix == nx - 1) vy[p + P + 1] = -vy[p + P];
-                    if (iz <= g.zy) vy[p + P] = -vy_c;
+                    if (ix == 0) PUT(1, vy, p + P - 1, -vy[p + P])
+                    if (ix == nx - 1) PUT(1, vy, p + P + 1, -vy[p + P])
+                    if (iz <= g.zy) PUT(1, vy, p + P, -vy_c)
                 }
+            }
+#undef PUT
+            if (sv) {
+                // the entries this thread owns, updated or not
+                g.st[0][p] = vx_c;
+                if (ok_y) g.st[1][p] = vy_c;
+                if (ix <= nx - 1 && iz <= nz - 1) g.st[2][p] = vz_c;
             }
             if (EN) {
                 const double m2 = (iz == 0 || iz == nz) ? 0.5 : 1.0;
@@ -265,6 +281,7 @@ __global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__res
         const bool ok_yz = ix <= nx - 1 && iy <= ny - 1;
         const bool ok_xy = iy <= ny - 1;
         const T cx = g.cx, cy = g.cy, cz = g.cz;
+        const bool sv = g.st[0] != nullptr;
         T vz_m = vz[p - S2];
         T vx_c = vx[p];
         T vy_c = vy[p];
@@ -281,12 +298,14 @@ __global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__res
                     // update_stress.cpp:37-65
                     sxx[p] = sxx_o + ((c11 - c13 * c13 / c33) * cx * dvx + (c12 - c13 * c23 / c33) * cy * dvy);
                     syy[p] = syy_o + ((c12 - c23 * c13 / c33) * cx * dvx + (c22 - c23 * c23 / c33) * cy * dvy);
+                    if (sv) { g.st[3][p] = sxx[p]; g.st[4][p] = syy[p]; g.st[5][p] = szz_o; }
                 } else {
                     // update_stress.cpp:69-97
                     const T dvz = vz_c - vz_m;
                     sxx[p] = sxx_o + (c11 * cx * dvx + c12 * cy * dvy + c13 * cz * dvz);
                     syy[p] = syy_o + (c12 * cx * dvx + c22 * cy * dvy + c23 * cz * dvz);
                     szz[p] = szz_o + (c13 * cx * dvx + c23 * cy * dvy + c33 * cz * dvz);
+                    if (sv) { g.st[3][p] = sxx[p]; g.st[4][p] = syy[p]; g.st[5][p] = szz[p]; }
                 }
                 if (EN) {
                     double s11, s22, s33, s23, s13, s12;
@@ -304,6 +323,10 @@ __global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__res
                 const T syz_n = syz_o + T(0.5) * c44 * (cz * (vy_n - vy_c) + cy * (vz[p + P] - vz_c));
                 syz[p] = syz_n;
                 if (g.free_top && iz == 0) syz[p - S2] = -syz_n;  // update_stress.cpp:264-267
+                if (sv) {
+                    g.st[6][p] = syz_n;
+                    if (g.free_top && iz == 0) g.st[6][p - S2] = -syz_n;
+                }
                 if (EN) acc += 2 * ((1.0 / (double)c44) * (double)syz_o * (double)syz_o);
             }
             // sxz, update_stress.cpp:115-121
@@ -312,8 +335,11 @@ __global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__res
                 const T c55 = C55[p];
                 const T sxz_n = sxz_o + T(0.5) * c55 * (cz * (vx_n - vx_c) + cx * (vz_c - vz[p - 1]));
                 sxz[p] = sxz_n;
-                if (g.free_top && iz == 0 && ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1)
+                if (g.free_top && iz == 0 && ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1) {
                     sxz[p - S2] = -sxz_n;  // update_stress.cpp:274-277
+                    if (sv) g.st[7][p - S2] = -sxz_n;
+                }
+                if (sv) g.st[7][p] = sxz_n;
                 if (EN) {
                     const double m = ((iy == 0 || iy == ny) ? 0.5 : 1.0) * ((ix == 0 || ix == nx) ? 0.5 : 1.0);
                     acc += 2 * (m * (1.0 / (double)c55) * (double)sxz_o * (double)sxz_o);
@@ -324,6 +350,7 @@ __global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__res
                 const T sxy_o = sxy[p];
                 const T c66 = C66[p];
                 sxy[p] = sxy_o + T(0.5) * c66 * (cy * (vx[p + P] - vx_c) + cx * (vy_c - vy[p - 1]));
+                if (sv) g.st[8][p] = sxy[p];
                 if (EN) {
                     const double m = ((iz == 0 || iz == nz) ? 0.5 : 1.0) * ((ix == 0 || ix == nx) ? 0.5 : 1.0);
                     acc += 2 * (m * (1.0 / (double)c66) * (double)sxy_o * (double)sxy_o);
@@ -432,7 +459,9 @@ __global__ __launch_bounds__(64) void k_gvz_solve(IfaceDev<T> a, double *__restr
     }
     T *vz2 = a.C.f[2];
     double x = prev / Dp[(long)(n - 1) * nx2 + I];
+    T *vz2s = a.C.st[2];
     vz2[FIDX(a.C, I, n, -1)] = (T)x;
+    if (vz2s) vz2s[FIDX(a.C, I, n, -1)] = (T)x;
     for (int q1 = n - 2; q1 >= 0; q1 -= CH) {
         double bq[CH], uq[CH], dq[CH];
 #pragma unroll
@@ -447,6 +476,7 @@ __global__ __launch_bounds__(64) void k_gvz_solve(IfaceDev<T> a, double *__restr
             if (q1 - t >= 0) {
                 x = (bq[t] - uq[t] * x) / dq[t];
                 vz2[FIDX(a.C, I, q1 - t + 1, -1)] = (T)x;
+                if (vz2s) vz2s[FIDX(a.C, I, q1 - t + 1, -1)] = (T)x;
             }
     }
 }
@@ -473,7 +503,9 @@ __global__ __launch_bounds__(256) void k_gvz_fine(IfaceDev<T> a)
         tmp = hatw(bo, r) * t0 + hatw(bo - r, r) * t1;
     }
     const long q = FIDX(a.F, i, j, a.nz1);
-    vz1[q] = (T)(-ld(vz1, q - a.F.S2) + tmp);
+    const T v = (T)(-ld(vz1, q - a.F.S2) + tmp);
+    vz1[q] = v;
+    if (a.F.st[2]) a.F.st[2][q] = v;
 }
 
 // vx-sxz pair: right-hand side on interior coarse nodes (derived ratio-r form
@@ -572,6 +604,7 @@ struct JacItf {
     double *part;           // 2 parities x 2 sums x nblk
     long long *stats;       // total, last, capped
     T *sxz1, *sxz2;
+    T *sxz1s, *sxz2s;       // optional second destinations of the ghost writes
     long o1, P1, S21, o2, P2, S22;
     int nx2, ny2, nz1, r;
     int blk0, nblk;         // workgroups [blk0, blk0+nblk) work on this interface
@@ -757,6 +790,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
         if (Xo != J.X0) J.X0[c] = x;
         const long i1 = c / w, i0 = c - i1 * w;
         J.sxz2[J.o2 + (long)(-1) * J.S22 + i1 * J.P2 + i0] = (T)x;
+        if (J.sxz2s) J.sxz2s[J.o2 + (long)(-1) * J.S22 + i1 * J.P2 + i0] = (T)x;
     }
     grid.sync();
     // fine ghosts on fine interior nodes
@@ -776,7 +810,9 @@ __global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
                 S += pw * ((double)J.sxz2[q2 - J.S22] + (double)J.sxz2[q2]);
             }
         const long qf = J.o1 + (long)J.nz1 * J.S21 + (long)jj * J.P1 + i;
-        J.sxz1[qf] = (T)(-(double)J.sxz1[qf - J.S21] + S);
+        const T v = (T)(-(double)J.sxz1[qf - J.S21] + S);
+        J.sxz1[qf] = v;
+        if (J.sxz1s) J.sxz1s[qf] = v;
     }
     if (lb == 0 && threadIdx.x == 0) {
         J.stats[0] += iters;
@@ -991,7 +1027,10 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
         if (I <= n0 && Jg <= n1) {
             const double x = stash[stop - 1][p];
             J.X0[(long)Jg * w + I] = x;
-            if (!jb.solve_only) J.sxz2[J.o2 - J.S22 + (long)Jg * J.P2 + I] = (T)x;
+            if (!jb.solve_only) {
+                J.sxz2[J.o2 - J.S22 + (long)Jg * J.P2 + I] = (T)x;
+                if (J.sxz2s) J.sxz2s[J.o2 - J.S22 + (long)Jg * J.P2 + I] = (T)x;
+            }
         }
     }
     if (jb.solve_only) {
@@ -1022,7 +1061,9 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
                 S += pw * ((double)J.sxz2[q2 - J.S22] + (double)J.sxz2[q2]);
             }
         const long qf = J.o1 + (long)J.nz1 * J.S21 + (long)jj * J.P1 + i;
-        J.sxz1[qf] = (T)(-(double)J.sxz1[qf - J.S21] + S);
+        const T v = (T)(-(double)J.sxz1[qf - J.S21] + S);
+        J.sxz1[qf] = v;
+        if (J.sxz1s) J.sxz1s[qf] = v;
     }
     if (lb == 0 && tid == 0) {
         J.stats[0] += iters;
@@ -1065,11 +1106,14 @@ __global__ __launch_bounds__(256) void k_gsyz(IfaceDev<T> a)
         }
     const T g2 = (T)(num / den);
     syz2[qc - C.S2] = g2;
+    if (C.st[6]) C.st[6][qc - C.S2] = g2;
     const double tmp = (double)g2 + s2;
     for (int bo = 0; bo < r; bo++)
         for (int ao = 0; ao < r; ao++) {
             const long q = FIDX(F, r * I + ao, r * J + bo, nz1);
-            syz1[q] = (T)(-ld(syz1, q - F.S2) + tmp);
+            const T v = (T)(-ld(syz1, q - F.S2) + tmp);
+            syz1[q] = v;
+            if (F.st[6]) F.st[6][q] = v;
         }
 }
 
@@ -1079,6 +1123,7 @@ template <typename T>
 struct PointArgs {
     T *vx[MAXG];
     T *sxx[MAXG], *syy[MAXG], *szz[MAXG];
+    T *st[MAXG][4];      // optional second destinations of vx, sxx, syy, szz
     const T *b[MAXG];
     long o[MAXG], P[MAXG], S2[MAXG];
     int n[MAXG][3];
@@ -1113,6 +1158,7 @@ __global__ __launch_bounds__(256) void k_recsrc(PointArgs<T> a)
             if (a.src_type == 0) {
                 const float add = 1000 * a.src_tr[step];
                 a.vx[g][q] = a.vx[g][q] + (T)add;
+                if (a.st[g][0]) a.st[g][0][q] = a.vx[g][q];
                 if (a.energy) {
                     const int *n = a.n[g];
                     const double m = ((a.src_k == 0 || a.src_k == n[2]) ? 0.5 : 1.0)
@@ -1125,6 +1171,11 @@ __global__ __launch_bounds__(256) void k_recsrc(PointArgs<T> a)
                 a.sxx[g][q] = a.sxx[g][q] + add;
                 a.syy[g][q] = a.syy[g][q] + add;
                 a.szz[g][q] = a.szz[g][q] + add;
+                if (a.st[g][1]) {
+                    a.st[g][1][q] = a.sxx[g][q];
+                    a.st[g][2][q] = a.syy[g][q];
+                    a.st[g][3][q] = a.szz[g][q];
+                }
             }
         }
         *a.fix = fix;
