@@ -4,6 +4,7 @@
 #include "ecsg_hip.h"
 #include "ecsg_kernels.hpp"
 #include "ecsg_adjoint.hpp"
+#include "ecsg_batch.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -131,6 +132,8 @@ public:
     double tol_cap = 0, tol_fac = 0;
     int max_coop = 0;  // co-resident workgroups of the register Jacobi kernel
     int max_tiled = 0; // co-resident workgroups of the tiled Jacobi kernel
+    bool lds_ok = false;  // every interface solved by k_jacobi_lds
+    long lds_maxw = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     bool graph_failed = false;
@@ -272,6 +275,21 @@ public:
                 return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
         }
         if (c.sxz_solver != 0 && c.sxz_solver != 1) return fail(ECSG_EINVAL, "sxz_solver must be 0 (Jacobi) or 1 (Chebyshev)");
+        // interfaces whose iterate fits in LDS are solved by one workgroup
+        // each (no cooperative launch: the step can be graph-captured)
+        lds_maxw = 0;
+        lds_ok = !I.empty();
+        for (auto &f : I) {
+            const long W = (long)(f.nx2 + 1) * (f.ny2 + 1);
+            lds_maxw = std::max(lds_maxw, W);
+            if (W > JL_MAXW) lds_ok = false;
+        }
+        if (lds_ok) {
+            HIPCHK(hipFuncSetAttribute((const void *)k_jacobi_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(2 * JL_MAXW * sizeof(double))));
+            HIPCHK(hipFuncSetAttribute((const void *)k_jacobi_lds_batch<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(2 * JL_MAXW * sizeof(double))));
+        }
         if (int rc = dalloc(&d_step, 1)) return rc;
         if (int rc = dalloc(&d_energy, cfg.max_steps)) return rc;
         if (int rc = dalloc(&d_fix, 1)) return rc;
@@ -642,6 +660,43 @@ public:
         return ECSG_OK;
     }
 
+    JacLds<T> jac_lds(int k, bool adjoint) const
+    {
+        const DevIface &f = I[k];
+        JacLds<T> J{};
+        J.A = f.A;
+        if (adjoint) {
+            const AdjIface &a = AI[k];
+            J.B = a.Ba; J.abs_part = a.absz; J.X0 = a.X0a; J.stats = a.stats;
+        } else {
+            J.B = f.Bs; J.abs_part = f.absp; J.X0 = f.X0; J.stats = f.stats;
+            J.sxz1 = G[k].dev.f[7]; J.sxz2 = G[k + 1].dev.f[7];
+            J.sxz1s = G[k].dev.st[7]; J.sxz2s = G[k + 1].dev.st[7];
+        }
+        J.o1 = G[k].o; J.P1 = G[k].P; J.S21 = G[k].S2;
+        J.o2 = G[k + 1].o; J.P2 = G[k + 1].P; J.S22 = G[k + 1].S2;
+        J.nx2 = f.nx2; J.ny2 = f.ny2; J.nz1 = f.nz1; J.r = f.r; J.nabs = f.nabs;
+        return J;
+    }
+
+    int launch_jacobi_lds(bool adjoint)
+    {
+        JacLdsSet<T> set{};
+        const int ni = (int)I.size();
+        for (int k = 0; k < ni; k++) set.it[k] = jac_lds(k, adjoint);
+        JacParams jp{};
+        if (adjoint) {
+            jp.tol_cap = 0; jp.tol_fac = 0; jp.rel_tol = adj_rel_tol; jp.solve_only = 1;
+        } else {
+            jp.tol_cap = tol_cap; jp.tol_fac = tol_fac; jp.rel_tol = 0; jp.solve_only = 0;
+        }
+        jp.floor = 1024 * 2.220446049250313e-16;
+        jp.cap = JACOBI_CAP;
+        hipLaunchKernelGGL((k_jacobi_lds<T>), dim3(ni), dim3(JL), (size_t)(2 * lds_maxw * sizeof(double)), s, set, jp);
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
     // Jacobi solves + ghost scatter of interfaces [k0, k0+n) in one launch
     int launch_jacobi(int k0, int n)
     {
@@ -686,6 +741,16 @@ public:
     {
         ev_begin();
         if (int rc = launch_sxz_rhs(k)) return rc;
+        if (lds_ok && cfg.sxz_solver == 0) {
+            JacLdsSet<T> set{};
+            set.it[0] = jac_lds(k, false);
+            JacParams jp{};
+            jp.tol_cap = tol_cap; jp.tol_fac = tol_fac; jp.floor = 1024 * 2.220446049250313e-16; jp.cap = JACOBI_CAP;
+            hipLaunchKernelGGL((k_jacobi_lds<T>), dim3(1), dim3(JL), (size_t)(2 * lds_maxw * sizeof(double)), s, set, jp);
+            HIPCHK(hipGetLastError());
+            ev_end(4);
+            return ECSG_OK;
+        }
         if (int rc = launch_jacobi(k, 1)) return rc;
         ev_end(4);
         return ECSG_OK;
@@ -771,8 +836,12 @@ public:
             ev_begin();
             for (int k = 0; k + 1 < ng; k++)
                 if (int rc = launch_sxz_rhs(k)) return rc;
-            for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
-                if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
+            if (lds_ok && cfg.sxz_solver == 0) {
+                if (int rc = launch_jacobi_lds(false)) return rc;
+            } else {
+                for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
+                    if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
+            }
             ev_end(4);
         }
         for (int k = 0; k + 1 < ng; k++)
@@ -898,9 +967,10 @@ public:
         if (int rc = check_ready()) return rc;
         if (n < 0) return fail(ECSG_EINVAL, "negative step count");
         if (host_step + n > cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded");
-        // The interface Jacobi is a cooperative launch; its behaviour inside a
-        // captured graph is not relied on, so composite models run eagerly.
-        if (use_graph && !graph_failed && !prof && I.empty()) {
+        // The tiled interface Jacobi is a cooperative launch; its behaviour
+        // inside a captured graph is not relied on, so composite models with
+        // interfaces too large for the LDS solver run eagerly.
+        if (use_graph && !graph_failed && !prof && (I.empty() || (lds_ok && cfg.sxz_solver == 0))) {
             if (!gexec) {
                 if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                     int rc = enqueue_step();
@@ -1383,8 +1453,12 @@ public:
                 hipLaunchKernelGGL((k_asxz_ghost<T>), dim3(nb((long)(a.nx2 - 1) * (a.ny2 - 1))), dim3(256), 0, s, a, AI[k].Ba);
                 hipLaunchKernelGGL((k_asxz_fine0<T>), dim3(nb((long)(nx1 - 1) * (ny1 - 1))), dim3(256), 0, s, a);
             }
-            for (int k0 = 0; k0 < ni; k0 += MAXI)
-                if (int rc = launch_jacobi_adj(k0, std::min(MAXI, ni - k0))) return rc;
+            if (lds_ok) {
+                if (int rc = launch_jacobi_lds(true)) return rc;
+            } else {
+                for (int k0 = 0; k0 < ni; k0 += MAXI)
+                    if (int rc = launch_jacobi_adj(k0, std::min(MAXI, ni - k0))) return rc;
+            }
             for (int k = ni - 1; k >= 0; k--) {
                 IfAdj<T> a = if_adj(k, slot_f, slot_f);
                 const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;

@@ -73,11 +73,9 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
 // EN: also the kinetic part of ComputeEnergy (compute_energy.cpp:38-39,
 // TriMulFields field.cpp:482-518) from the pre- and post-update velocities.
 template <typename T, bool EN, int BYv>
-__global__ __launch_bounds__(BX * BYv) void k_vel(GridDev<T> g, double *__restrict__ part,
-                                                  const int *__restrict__ src, T *__restrict__ src_old)
+__device__ __forceinline__ void vel_tile(const GridDev<T> &g, const int tile, double *__restrict__ part,
+                                         const int *__restrict__ src, T *__restrict__ src_old, double *lds)
 {
-    __shared__ double lds[BYv];
-    const int tile = xcd_tile(blockIdx.x, g.nblk);
     const int bx = tile % g.gx;
     const int rest = tile / g.gx;
     const int by = rest % g.gy;
@@ -214,6 +212,14 @@ __global__ __launch_bounds__(BX * BYv) void k_vel(GridDev<T> g, double *__restri
     }
 }
 
+template <typename T, bool EN, int BYv>
+__global__ __launch_bounds__(BX * BYv) void k_vel(GridDev<T> g, double *__restrict__ part,
+                                                  const int *__restrict__ src, T *__restrict__ src_old)
+{
+    __shared__ double lds[BYv];
+    vel_tile<T, EN, BYv>(g, xcd_tile(blockIdx.x, g.nblk), part, src, src_old, lds);
+}
+
 // compliance of the normal block at one point (energy weights)
 template <typename T>
 __device__ __forceinline__ void compliance(const GridDev<T> &g, T c11, T c22, T c33, T c23, T c13, T c12,
@@ -242,10 +248,8 @@ __device__ __forceinline__ void compliance(const GridDev<T> &g, T c11, T c22, T 
 // stresses fused.  EN: strain part of ComputeEnergy (compute_energy.cpp:40-49)
 // from the pre-update stresses.
 template <typename T, bool EN, int BYv>
-__global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__restrict__ part)
+__device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile, double *__restrict__ part, double *lds)
 {
-    __shared__ double lds[BYv];
-    const int tile = xcd_tile(blockIdx.x, g.nblk);
     const int bx = tile % g.gx;
     const int rest = tile / g.gx;
     const int by = rest % g.gy;
@@ -367,6 +371,48 @@ __global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__res
     }
 }
 
+template <typename T, bool EN, int BYv>
+__global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__restrict__ part)
+{
+    __shared__ double lds[BYv];
+    stress_tile<T, EN, BYv>(g, xcd_tile(blockIdx.x, g.nblk), part, lds);
+}
+
+// Batched stencil launches: the workgroups of many grids (of many models) in
+// one launch; start[m] is the first workgroup of grid m (prefix sums over n
+// grids, ascending).  Tiles are taken in XCD order over the whole launch, so
+// the tiles of one grid stay together on an XCD.
+__device__ __forceinline__ int batch_find(const int *__restrict__ start, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T, int BYv>
+__global__ __launch_bounds__(BX * BYv) void k_vel_batch(const GridDev<T> *__restrict__ gs, const int *__restrict__ start, int n,
+                                                        int total)
+{
+    __shared__ double lds[BYv];
+    const int b = xcd_tile(blockIdx.x, total);
+    const int m = batch_find(start, n, b);
+    vel_tile<T, false, BYv>(gs[m], b - start[m], nullptr, nullptr, nullptr, lds);
+}
+
+template <typename T, int BYv>
+__global__ __launch_bounds__(BX * BYv) void k_stress_batch(const GridDev<T> *__restrict__ gs, const int *__restrict__ start, int n,
+                                                           int total)
+{
+    __shared__ double lds[BYv];
+    const int b = xcd_tile(blockIdx.x, total);
+    const int m = batch_find(start, n, b);
+    stress_tile<T, false, BYv>(gs[m], b - start[m], nullptr, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Interface kernels.  Fine grid F (upper, z index nz1 on the interface plane)
 // and coarse grid C (lower, z index 0 on the plane); refinement ratio r.
@@ -392,9 +438,8 @@ __device__ __forceinline__ double ld(const T *a, long i) { return (double)a[i]; 
 // x-line (derived ratio-r form of update_ghost_vz.cpp:152-203).
 // B layout [J-1][I].
 template <typename T>
-__global__ __launch_bounds__(256) void k_gvz_rhs(IfaceDev<T> a, double *__restrict__ B)
+__device__ __forceinline__ void gvz_rhs_body(const IfaceDev<T> &a, const int t, double *__restrict__ B)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx2 = a.nx2, ny2 = a.ny2, r = a.r, nz1 = a.nz1;
     if (t >= nx2 * (ny2 - 1)) return;
     const int I = t % nx2;
@@ -429,14 +474,20 @@ __global__ __launch_bounds__(256) void k_gvz_rhs(IfaceDev<T> a, double *__restri
     B[(long)(J - 1) * nx2 + I] = (double)(r * r * r) * kc - fs;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_gvz_rhs(IfaceDev<T> a, double *__restrict__ B)
+{
+    gvz_rhs_body(a, blockIdx.x * blockDim.x + threadIdx.x, B);
+}
+
 // Prefactored Thomas solve, one thread per coarse x-line (update_ghost_vz.cpp:
 // 211-220 with the time-invariant elimination factors computed once).
 // M: multipliers, Dp: eliminated diagonal, U: super-diagonal, all [q][I].
 template <typename T>
-__global__ __launch_bounds__(64) void k_gvz_solve(IfaceDev<T> a, double *__restrict__ B, const double *__restrict__ M,
-                                                  const double *__restrict__ Dp, const double *__restrict__ U)
+__device__ __forceinline__ void gvz_solve_body(const IfaceDev<T> &a, const int I, double *__restrict__ B,
+                                               const double *__restrict__ M, const double *__restrict__ Dp,
+                                               const double *__restrict__ U)
 {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx2 = a.nx2, n = a.ny2 - 1;
     if (I >= nx2) return;
     constexpr int CH = 16;
@@ -481,13 +532,19 @@ __global__ __launch_bounds__(64) void k_gvz_solve(IfaceDev<T> a, double *__restr
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(64) void k_gvz_solve(IfaceDev<T> a, double *__restrict__ B, const double *__restrict__ M,
+                                                  const double *__restrict__ Dp, const double *__restrict__ U)
+{
+    gvz_solve_body(a, blockIdx.x * blockDim.x + threadIdx.x, B, M, Dp, U);
+}
+
 // Fine ghosts vz1(., ., nz1): injection in x, hat in y (update_ghost_vz.cpp:
 // 231-248 generalised).  One thread per fine (i, j) with j in [1, ny1-1].
 template <typename T>
-__global__ __launch_bounds__(256) void k_gvz_fine(IfaceDev<T> a)
+__device__ __forceinline__ void gvz_fine_body(const IfaceDev<T> &a, const int t)
 {
     const int r = a.r, nx1 = r * a.nx2, ny1 = r * a.ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx1 * (ny1 - 1)) return;
     const int i = t % nx1;
     const int j = t / nx1 + 1;
@@ -508,14 +565,20 @@ __global__ __launch_bounds__(256) void k_gvz_fine(IfaceDev<T> a)
     if (a.F.st[2]) a.F.st[2][q] = v;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_gvz_fine(IfaceDev<T> a)
+{
+    gvz_fine_body(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // vx-sxz pair: right-hand side on interior coarse nodes (derived ratio-r form
 // of update_ghost_sxz.cpp:139-226) and per-workgroup sums of |B|
 // (AvgAbs, array.cpp:188-197).  B layout [J-1][I-1].
 template <typename T>
-__global__ __launch_bounds__(256) void k_gsxz_rhs(IfaceDev<T> a, double *__restrict__ B, double *__restrict__ abs_part)
+__device__ __forceinline__ void gsxz_rhs_body(const IfaceDev<T> &a, const int lb, double *__restrict__ B,
+                                              double *__restrict__ abs_part, double *lds)
 {
-    __shared__ double lds[4];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = lb * 256 + threadIdx.x;
     const int nx2 = a.nx2, ny2 = a.ny2, r = a.r, nz1 = a.nz1;
     const int n0 = nx2 - 1, n1 = ny2 - 1;
     double v = 0;
@@ -562,7 +625,14 @@ __global__ __launch_bounds__(256) void k_gsxz_rhs(IfaceDev<T> a, double *__restr
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) abs_part[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    if (threadIdx.x == 0) abs_part[lb] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gsxz_rhs(IfaceDev<T> a, double *__restrict__ B, double *__restrict__ abs_part)
+{
+    __shared__ double lds[4];
+    gsxz_rhs_body(a, blockIdx.x, B, abs_part, lds);
 }
 
 struct JacobiArgs {
@@ -1075,9 +1145,8 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
 // vy-syz pair: pointwise coarse ghost and its fine children (derived ratio-r
 // form of update_ghost_syz.cpp:162-238).
 template <typename T>
-__global__ __launch_bounds__(256) void k_gsyz(IfaceDev<T> a)
+__device__ __forceinline__ void gsyz_body(const IfaceDev<T> &a, const int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx2 = a.nx2, ny2 = a.ny2, r = a.r, nz1 = a.nz1;
     if (t >= nx2 * ny2) return;
     const int I = t % nx2, J = t / nx2;
@@ -1117,6 +1186,12 @@ __global__ __launch_bounds__(256) void k_gsyz(IfaceDev<T> a)
         }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_gsyz(IfaceDev<T> a)
+{
+    gsyz_body(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Receivers, source, energy finalisation and the step counter.
 template <typename T>
@@ -1141,7 +1216,7 @@ struct PointArgs {
 
 // main.cpp:183-195: receivers sample vx after UpdateVel, then the source is added.
 template <typename T>
-__global__ __launch_bounds__(256) void k_recsrc(PointArgs<T> a)
+__device__ __forceinline__ void recsrc_body(const PointArgs<T> &a)
 {
     const int step = *a.step;
     if (step < a.max_steps)
@@ -1181,6 +1256,12 @@ __global__ __launch_bounds__(256) void k_recsrc(PointArgs<T> a)
         *a.fix = fix;
         if (!a.energy) *a.step = step + 1;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_recsrc(PointArgs<T> a)
+{
+    recsrc_body(a);
 }
 
 struct EnergyArgs {
