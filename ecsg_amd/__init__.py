@@ -31,7 +31,9 @@ ABI_SYMBOLS = (
     "ecsg_compute_res", "ecsg_step_n", "ecsg_steps_done", "ecsg_synchronize", "ecsg_field_dims", "ecsg_get_field",
     "ecsg_set_field", "ecsg_get_traces", "ecsg_get_energy", "ecsg_jacobi_stats", "ecsg_profile_kernels",
     "ecsg_reset", "ecsg_set_checkpointing", "ecsg_gradient", "ecsg_get_gradient", "ecsg_adjoint_step",
-    "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats",
+    "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune",
+    "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
+    "ecsg_batch_set_tiling", "ecsg_batch_autotune",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -103,6 +105,15 @@ def load_library(path: Optional[str] = None):
     L.ecsg_get_adjoint_field.argtypes = [vp, i, i, P(d)]
     L.ecsg_set_adjoint_field.argtypes = [vp, i, i, P(d)]
     L.ecsg_adjoint_stats.argtypes = [vp, P(C.c_longlong)]
+    L.ecsg_autotune.argtypes = [vp, i, P(i), P(i)]
+    L.ecsg_batch_create.argtypes = [P(vp), i, P(vp)]
+    L.ecsg_batch_destroy.argtypes = [vp]
+    L.ecsg_batch_last_error.argtypes = [vp]
+    L.ecsg_batch_last_error.restype = C.c_char_p
+    L.ecsg_batch_step_n.argtypes = [vp, i, i]
+    L.ecsg_batch_synchronize.argtypes = [vp]
+    L.ecsg_batch_set_tiling.argtypes = [vp, i, i]
+    L.ecsg_batch_autotune.argtypes = [vp, i, P(i), P(i)]
     if path is None:
         _LIB = L
     return L
@@ -288,3 +299,58 @@ class Simulation:
         n = C.c_longlong()
         self._chk(self.L.ecsg_adjoint_stats(self.h, C.byref(n)))
         return n.value
+
+    def autotune(self, reps: int = 5):
+        """Per-grid stencil tiling chosen by timing (state preserved):
+        returns [(rows per workgroup, z planes per thread), ...]."""
+        ng = self.model.num_grids
+        by, zc = (C.c_int * ng)(), (C.c_int * ng)()
+        self._chk(self.L.ecsg_autotune(self.h, int(reps), by, zc))
+        return [(by[g], zc[g]) for g in range(ng)]
+
+
+class Batch:
+    """Many Simulations (shots, models of any sizes) advanced together with one
+    launch per phase of the time step (ecsg_batch_*).  The members keep their
+    state and accessors and must outlive the batch."""
+
+    def __init__(self, sims):
+        self.L = load_library()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        self.h = C.c_void_p()
+        rc = self.L.ecsg_batch_create(arr, len(self.sims), C.byref(self.h))
+        if rc != 0:
+            msg = self.L.ecsg_batch_last_error(self.h).decode() if self.h else ""
+            self.L.ecsg_batch_destroy(self.h)
+            self.h = None
+            raise EcsgError("ecsg_batch_create failed (%d): %s" % (rc, msg))
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise EcsgError("ecsg batch error %d: %s" % (rc, self.L.ecsg_batch_last_error(self.h).decode()))
+
+    def step(self, n: int = 1, graph: bool = True):
+        self._chk(self.L.ecsg_batch_step_n(self.h, int(n), int(graph)))
+
+    def sync(self):
+        self._chk(self.L.ecsg_batch_synchronize(self.h))
+
+    def set_tiling(self, by: int, zc: int):
+        self._chk(self.L.ecsg_batch_set_tiling(self.h, int(by), int(zc)))
+
+    def autotune(self, reps: int = 5):
+        by, zc = C.c_int(), C.c_int()
+        self._chk(self.L.ecsg_batch_autotune(self.h, int(reps), C.byref(by), C.byref(zc)))
+        return by.value, zc.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ecsg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

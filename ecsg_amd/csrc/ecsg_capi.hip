@@ -76,6 +76,7 @@ public:
     virtual int adjoint_step() = 0;
     virtual int adj_field(int g, int sel, double *out, const double *in) = 0;
     virtual int adjoint_stats(long long *sweeps, double *ms) = 0;
+    virtual int autotune(int reps, int *by_out, int *zc_out) = 0;
 };
 
 #define HIPCHK(x)                                                                                  \
@@ -1851,6 +1852,52 @@ public:
         return ECSG_OK;
     }
 
+    // per grid: the stencil tiling (workgroup height, z chunk) with the
+    // shortest velocity + stress launches, timed on a saved copy of the state
+    int autotune(int reps, int *by_out, int *zc_out) override
+    {
+        if (int rc = check_ready()) return rc;
+        if (reps < 1) return fail(ECSG_EINVAL, "reps must be >= 1");
+        HIPCHK(hipStreamSynchronize(s));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const int bys[4] = {1, 2, 4, 8}, zcs[4] = {2, 4, 8, 16};
+        for (int g = 0; g < (int)G.size(); g++) {
+            DevGrid<T> &d = G[g];
+            T *save = nullptr;
+            HIPCHK(hipMalloc((void **)&save, sizeof(T) * 9 * d.astride));
+            HIPCHK(hipMemcpy(save, d.base, sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+            float best = 1e30f;
+            int bb = d.by, bz = d.dev.zc;
+            for (int b : bys)
+                for (int z : zcs) {
+                    set_tiling(d, b, z);
+                    launch_vel(g);
+                    launch_stress(g);
+                    HIPCHK(hipEventRecord(e0, s));
+                    for (int r = 0; r < reps; r++) {
+                        launch_vel(g);
+                        launch_stress(g);
+                    }
+                    HIPCHK(hipEventRecord(e1, s));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) { best = ms; bb = b; bz = z; }
+                }
+            set_tiling(d, bb, bz);
+            HIPCHK(hipMemcpy(d.base, save, sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+            hipFree(save);
+            if (by_out) by_out[g] = bb;
+            if (zc_out) zc_out[g] = bz;
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        invalidate_graph();
+        return ECSG_OK;
+    }
+
     int adjoint_stats(long long *sweeps, double *unused) override
     {
         long long tot = 0;
@@ -1865,7 +1912,293 @@ public:
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// Batch: many models advanced together, one launch per phase of the step.
+class BatchBase {
+public:
+    virtual ~BatchBase() {}
+    Status st;
+    int fail(int code, const std::string &m)
+    {
+        st.code = code;
+        st.msg = m;
+        return code;
+    }
+    virtual int step_n(int n, int use_graph) = 0;
+    virtual int sync() = 0;
+    virtual int set_tiling(int by, int zc) = 0;
+    virtual int autotune(int reps, int *by, int *zc) = 0;
+};
+
+template <typename T>
+class Batch : public BatchBase {
+public:
+    std::vector<Engine<T> *> M;
+    hipStream_t s = nullptr;
+    hipEvent_t done = nullptr;
+    int by = 4, zc = ZCHUNK;
+    GridDev<T> *d_g = nullptr;
+    int *d_gs = nullptr;
+    int ng = 0, gtot = 0;
+    IfWork<T> *d_if = nullptr;
+    int *d_is[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int ni = 0, itot[5] = {0, 0, 0, 0, 0};
+    JacLds<T> *d_jl = nullptr;
+    long maxw = 0;
+    PointArgs<T> *d_pa = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    ~Batch() override
+    {
+        drop_graph();
+        hipFree(d_g); hipFree(d_gs); hipFree(d_if); hipFree(d_jl); hipFree(d_pa);
+        for (auto p : d_is) hipFree(p);
+        if (done) hipEventDestroy(done);
+        if (s) hipStreamDestroy(s);
+    }
+    void drop_graph()
+    {
+        if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+    }
+    template <typename P>
+    int upload(P **dst, const std::vector<P> &v)
+    {
+        hipFree(*dst);
+        *dst = nullptr;
+        HIPCHK(hipMalloc((void **)dst, sizeof(P) * std::max<size_t>(v.size(), 1)));
+        if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), sizeof(P) * v.size(), hipMemcpyHostToDevice));
+        return ECSG_OK;
+    }
+
+    int init(const std::vector<Engine<T> *> &members)
+    {
+        M = members;
+        for (size_t m = 0; m < M.size(); m++) {
+            Engine<T> *e = M[m];
+            const std::string id = "member " + std::to_string(m) + ": ";
+            if (!e->all_params()) return fail(ECSG_ESTATE, id + "parameters not set for every grid");
+            if (e->cfg.check_energy) return fail(ECSG_EINVAL, id + "batched models run without the energy check");
+            if (e->cfg.sxz_solver != 0) return fail(ECSG_EINVAL, id + "batched models use the Jacobi sweeps (sxz_solver 0)");
+            if (!e->I.empty() && !e->lds_ok) return fail(ECSG_EINVAL, id + "interface too large for the LDS Jacobi of a batch");
+        }
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        if (int rc = build_grids()) return rc;
+        // interfaces, Jacobi descriptors and point kernels
+        std::vector<IfWork<T>> ifs;
+        std::vector<int> st[5];
+        std::vector<JacLds<T>> jl;
+        std::vector<PointArgs<T>> pa;
+        int acc[5] = {0, 0, 0, 0, 0};
+        for (Engine<T> *e : M) {
+            pa.push_back(e->point_args());
+            for (int k = 0; k < (int)e->I.size(); k++) {
+                DevIface &f = e->I[k];
+                IfWork<T> w{};
+                w.a = e->iface_dev(k);
+                w.B = f.B; w.M = f.M; w.Dp = f.Dp; w.U = f.U; w.Bs = f.Bs; w.absp = f.absp;
+                ifs.push_back(w);
+                const long r = f.r;
+                const long nb[5] = {((long)f.nx2 * (f.ny2 - 1) + 255) / 256, ((long)f.nx2 + 63) / 64,
+                                    (r * f.nx2 * (r * f.ny2 - 1) + 255) / 256, (long)f.nabs, ((long)f.nx2 * f.ny2 + 255) / 256};
+                for (int q = 0; q < 5; q++) { st[q].push_back(acc[q]); acc[q] += (int)nb[q]; }
+                jl.push_back(e->jac_lds(k, false));
+                maxw = std::max(maxw, (long)(f.nx2 + 1) * (f.ny2 + 1));
+            }
+        }
+        ni = (int)ifs.size();
+        for (int q = 0; q < 5; q++) itot[q] = acc[q];
+        if (int rc = upload(&d_if, ifs)) return rc;
+        for (int q = 0; q < 5; q++)
+            if (int rc = upload(&d_is[q], st[q])) return rc;
+        if (int rc = upload(&d_jl, jl)) return rc;
+        if (int rc = upload(&d_pa, pa)) return rc;
+        if (ni) HIPCHK(hipFuncSetAttribute((const void *)k_jacobi_lds_batch<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(2 * JL_MAXW * sizeof(double))));
+        return ECSG_OK;
+    }
+
+    // flat tile list over every grid of every member, with the batch tiling
+    int build_grids()
+    {
+        std::vector<GridDev<T>> gs;
+        std::vector<int> st;
+        int acc = 0;
+        for (Engine<T> *e : M)
+            for (auto &d : e->G) {
+                GridDev<T> v = d.dev;
+                for (int f = 0; f < 9; f++) v.st[f] = nullptr;
+                v.zc = zc;
+                v.gx = (d.n[0] + 1 + BX - 1) / BX;
+                v.gy = (d.n[1] + 1 + by - 1) / by;
+                v.nblk = v.gx * v.gy * ((d.n[2] + 1 + zc - 1) / zc);
+                gs.push_back(v);
+                st.push_back(acc);
+                acc += v.nblk;
+            }
+        ng = (int)gs.size();
+        gtot = acc;
+        if (int rc = upload(&d_g, gs)) return rc;
+        if (int rc = upload(&d_gs, st)) return rc;
+        drop_graph();
+        return ECSG_OK;
+    }
+
+    template <int BYv>
+    void stencils(bool vel)
+    {
+        if (vel)
+            hipLaunchKernelGGL((k_vel_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, (const GridDev<T> *)d_g, (const int *)d_gs, ng, gtot);
+        else
+            hipLaunchKernelGGL((k_stress_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, (const GridDev<T> *)d_g, (const int *)d_gs, ng,
+                               gtot);
+    }
+    void launch_stencils(bool vel)
+    {
+        switch (by) {
+        case 1: stencils<1>(vel); break;
+        case 2: stencils<2>(vel); break;
+        case 8: stencils<8>(vel); break;
+        default: stencils<4>(vel); break;
+        }
+    }
+
+    // one batched time step (main.cpp:165-261 for every member)
+    int enqueue()
+    {
+        launch_stencils(true);
+        hipLaunchKernelGGL((k_recsrc_batch<T>), dim3((int)M.size()), dim3(256), 0, s, (const PointArgs<T> *)d_pa);
+        if (ni) {
+            hipLaunchKernelGGL((k_gvz_rhs_batch<T>), dim3(itot[0]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[0], ni);
+            hipLaunchKernelGGL((k_gvz_solve_batch<T>), dim3(itot[1]), dim3(64), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[1], ni);
+            hipLaunchKernelGGL((k_gvz_fine_batch<T>), dim3(itot[2]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[2], ni);
+        }
+        launch_stencils(false);
+        if (ni) {
+            hipLaunchKernelGGL((k_gsxz_rhs_batch<T>), dim3(itot[3]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[3], ni);
+            JacParams jp{};
+            jp.tol_cap = M[0]->tol_cap; jp.tol_fac = M[0]->tol_fac;
+            jp.floor = 1024 * 2.220446049250313e-16;
+            jp.cap = JACOBI_CAP;
+            hipLaunchKernelGGL((k_jacobi_lds_batch<T>), dim3(ni), dim3(JL), (size_t)(2 * maxw * sizeof(double)), s,
+                               (const JacLds<T> *)d_jl, jp);
+            hipLaunchKernelGGL((k_gsyz_batch<T>), dim3(itot[4]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[4], ni);
+        }
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
+    int step_n(int n, int use_graph) override
+    {
+        if (n < 0) return fail(ECSG_EINVAL, "negative step count");
+        for (Engine<T> *e : M) {
+            if (e->host_step + n > e->cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded by a member");
+            HIPCHK(hipStreamSynchronize(e->s));
+        }
+        if (use_graph && !gexec) {
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue();
+            hipGraph_t gr = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(s, &gr);
+            if (rc != ECSG_OK || e2 != hipSuccess || !gr) {
+                if (gr) hipGraphDestroy(gr);
+                return fail(ECSG_EHIP, "graph capture of the batched step failed");
+            }
+            graph = gr;
+            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < n; i++) {
+            if (use_graph) HIPCHK(hipGraphLaunch(gexec, s));
+            else if (int rc = enqueue()) return rc;
+        }
+        // members' own calls (traces, fields) are ordered after the batch
+        HIPCHK(hipEventRecord(done, s));
+        for (Engine<T> *e : M) {
+            HIPCHK(hipStreamWaitEvent(e->s, done, 0));
+            e->host_step += n;
+        }
+        return ECSG_OK;
+    }
+
+    int sync() override
+    {
+        HIPCHK(hipStreamSynchronize(s));
+        for (size_t m = 0; m < M.size(); m++)
+            if (int rc = M[m]->sync()) return fail(rc, "member " + std::to_string(m) + ": " + M[m]->st.msg);
+        return ECSG_OK;
+    }
+
+    int set_tiling(int b, int z) override
+    {
+        if (b != 1 && b != 2 && b != 4 && b != 8) return fail(ECSG_EINVAL, "tile height must be 1, 2, 4 or 8");
+        if (z < 1) return fail(ECSG_EINVAL, "z chunk must be >= 1");
+        by = b;
+        zc = z;
+        return build_grids();
+    }
+
+    // times the batched stencil launches for each tiling candidate on a saved
+    // copy of the state, keeps the fastest, restores the state
+    int autotune(int reps, int *by_out, int *zc_out) override
+    {
+        if (reps < 1) return fail(ECSG_EINVAL, "reps must be >= 1");
+        for (Engine<T> *e : M) HIPCHK(hipStreamSynchronize(e->s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<T *> save;
+        for (Engine<T> *e : M)
+            for (auto &d : e->G) {
+                T *p = nullptr;
+                HIPCHK(hipMalloc((void **)&p, sizeof(T) * 9 * d.astride));
+                HIPCHK(hipMemcpy(p, d.base, sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+                save.push_back(p);
+            }
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const int bys[4] = {1, 2, 4, 8}, zcs[4] = {2, 4, 8, 16};
+        float best = 1e30f;
+        int bb = by, bz = zc;
+        for (int b : bys)
+            for (int z : zcs) {
+                if (int rc = set_tiling(b, z)) return rc;
+                launch_stencils(true);
+                launch_stencils(false);
+                HIPCHK(hipEventRecord(e0, s));
+                for (int r = 0; r < reps; r++) {
+                    launch_stencils(true);
+                    launch_stencils(false);
+                }
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; bb = b; bz = z; }
+            }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        size_t q = 0;
+        for (Engine<T> *e : M)
+            for (auto &d : e->G) {
+                HIPCHK(hipMemcpy(d.base, save[q], sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+                hipFree(save[q]);
+                q++;
+            }
+        if (int rc = set_tiling(bb, bz)) return rc;
+        if (by_out) *by_out = bb;
+        if (zc_out) *zc_out = bz;
+        return ECSG_OK;
+    }
+};
+
 }  // namespace
+
+struct ecsg_batch {
+    std::unique_ptr<BatchBase> b;
+    std::string err;
+};
 
 struct ecsg_ctx {
     std::unique_ptr<EngineBase> e;
@@ -1985,5 +2318,60 @@ int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps)
     CTX_CHECK(ctx);
     return ret(ctx, ctx->e->adjoint_stats(sweeps, nullptr));
 }
+
+int ecsg_autotune(ecsg_ctx *ctx, int reps, int *by_out, int *zc_out)
+{
+    CTX_CHECK(ctx);
+    return ret(ctx, ctx->e->autotune(reps, by_out, zc_out));
+}
+
+static int bret(ecsg_batch *b, int rc)
+{
+    if (b && rc != ECSG_OK) b->err = b->b->st.msg;
+    return rc;
+}
+
+int ecsg_batch_create(ecsg_ctx *const *members, int n, ecsg_batch **out)
+{
+    if (!members || n < 1 || !out) return ECSG_EINVAL;
+    *out = nullptr;
+    ecsg_batch *b = new ecsg_batch();
+    int prec = -1;
+    for (int m = 0; m < n; m++) {
+        if (!members[m] || !members[m]->e) { delete b; return ECSG_EINVAL; }
+        const int p = dynamic_cast<Engine<double> *>(members[m]->e.get()) ? ECSG_F64 : ECSG_F32;
+        if (prec >= 0 && p != prec) {
+            b->err = "members must share one precision";
+            *out = b;
+            return ECSG_EINVAL;
+        }
+        prec = p;
+    }
+    int rc;
+    if (prec == ECSG_F64) {
+        auto *x = new Batch<double>();
+        b->b.reset(x);
+        std::vector<Engine<double> *> v;
+        for (int m = 0; m < n; m++) v.push_back(static_cast<Engine<double> *>(members[m]->e.get()));
+        rc = x->init(v);
+    } else {
+        auto *x = new Batch<float>();
+        b->b.reset(x);
+        std::vector<Engine<float> *> v;
+        for (int m = 0; m < n; m++) v.push_back(static_cast<Engine<float> *>(members[m]->e.get()));
+        rc = x->init(v);
+    }
+    if (rc != ECSG_OK) b->err = b->b->st.msg;
+    *out = b;
+    return rc;
+}
+void ecsg_batch_destroy(ecsg_batch *b) { delete b; }
+const char *ecsg_batch_last_error(const ecsg_batch *b) { return b ? b->err.c_str() : "null batch"; }
+#define BATCH_CHECK(b) \
+    if (!(b) || !(b)->b) return ECSG_EINVAL
+int ecsg_batch_step_n(ecsg_batch *b, int nsteps, int use_graph) { BATCH_CHECK(b); return bret(b, b->b->step_n(nsteps, use_graph)); }
+int ecsg_batch_synchronize(ecsg_batch *b) { BATCH_CHECK(b); return bret(b, b->b->sync()); }
+int ecsg_batch_set_tiling(ecsg_batch *b, int by, int zc) { BATCH_CHECK(b); return bret(b, b->b->set_tiling(by, zc)); }
+int ecsg_batch_autotune(ecsg_batch *b, int reps, int *by, int *zc) { BATCH_CHECK(b); return bret(b, b->b->autotune(reps, by, zc)); }
 
 }  // extern "C"

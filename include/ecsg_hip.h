@@ -184,6 +184,32 @@ int ecsg_set_adjoint_field(ecsg_ctx *ctx, int g, int sel, const double *in);
 /* Total Jacobi sweeps of the transposed vx-sxz solves since the last gradient. */
 int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps);
 
+/* Stencil tiling of each grid (workgroup height by_out[g] in {1,2,4,8} rows
+ * of 64, z chunk zc_out[g] in {2,4,8,16} planes per thread) chosen by timing
+ * the velocity and stress launches of every candidate reps times on a saved
+ * copy of the state, which is restored.  New (the reference has no tiling). */
+int ecsg_autotune(ecsg_ctx *ctx, int reps, int *by_out, int *zc_out);
+
+/* --- Batches.  No counterpart in the reference, which runs one model per
+ * process.  A batch advances many handles (shots, models of any sizes; one
+ * precision; no energy check; Jacobi sweeps with interfaces of at most 9728
+ * iterate entries) with one launch per phase of the time step: the stencil
+ * tiles of every grid of every member in one launch, the interface kernels
+ * over every interface, one workgroup per member for receivers/source and
+ * per interface for the Jacobi solve.  With use_graph the batched step is
+ * captured once in a hipGraph and replayed.  Members keep their state,
+ * traces and accessors; calls on a member after ecsg_batch_step_n are
+ * ordered after the batch's work.  Members must outlive the batch. --- */
+typedef struct ecsg_batch ecsg_batch;
+int ecsg_batch_create(ecsg_ctx *const *members, int n, ecsg_batch **out);
+void ecsg_batch_destroy(ecsg_batch *b);
+const char *ecsg_batch_last_error(const ecsg_batch *b);
+int ecsg_batch_step_n(ecsg_batch *b, int nsteps, int use_graph);
+int ecsg_batch_synchronize(ecsg_batch *b);
+/* one tiling for the whole batch, or the fastest of the candidates above */
+int ecsg_batch_set_tiling(ecsg_batch *b, int by, int zc);
+int ecsg_batch_autotune(ecsg_batch *b, int reps, int *by, int *zc);
+
 #ifdef __cplusplus
 }
 #endif
