@@ -102,6 +102,45 @@ def adjoint_run(m, steps: int, every: int, fmt: str, precision: str = "f32"):
     return el, J, sweeps
 
 
+def batch_run(steps: int, solo_steps: int, warmup: int):
+    """Batched shots over uneven two-grid models (16 models x 4 shots):
+    one launch per phase for all 64 members, graph-captured, autotuned
+    tiling; beside it the same members stepped one at a time (each with its
+    own graph-captured step)."""
+    import ecsg_amd
+    from ecsg_amd import model as M
+    t0 = time.time()
+    ms = M.batch_workload(nsteps=warmup + steps + solo_steps + 10)
+    nodes = sum(m.total_nodes for m in ms)
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=warmup + steps + solo_steps + 10)
+            for m in ms]
+    log("[bench] batch: %d members, %.1f M nodes, ready in %.1fs" % (len(ms), nodes / 1e6, time.time() - t0))
+    b = ecsg_amd.Batch(sims)
+    by, zc = b.autotune(reps=3)
+    b.step(warmup)
+    b.sync()
+    t = time.perf_counter()
+    b.step(steps, graph=True)
+    b.sync()
+    el = time.perf_counter() - t
+    b.close()
+    # the same members one at a time
+    t = time.perf_counter()
+    for sm in sims:
+        sm.step(solo_steps, graph=True)
+    for sm in sims:
+        sm.sync()
+    el1 = time.perf_counter() - t
+    for sm in sims:
+        sm.close()
+    return {"value": round(nodes * steps / el / 1e9, 3), "unit": "Gpt/s", "members": len(ms), "models": 16,
+            "shots_per_model": 4, "nodes": nodes, "steps": steps, "ms_per_step": round(el / steps * 1e3, 4),
+            "tiling": {"rows": by, "z_planes": zc}, "graph": True,
+            "one_at_a_time_value": round(nodes * solo_steps / el1 / 1e9, 3), "one_at_a_time_steps": solo_steps,
+            "workload": "16 ratio-3 two-grid models, fine (nx,ny) in [120,240] (multiples of 3), nz in [30,60] cells, "
+                        "coarse (nx/3, ny/3, 2nz); 4 Ricker shots each (10-25 Hz); FP32"}
+
+
 def cpu_baseline(m, steps: int):
     """The oracle (CPU restatement, OpenMP) on the same model for `steps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -162,6 +201,8 @@ def main():
                     help="steps of the timed forward+adjoint gradient (0: skip)")
     ap.add_argument("--ckpt-every", type=int, default=10)
     ap.add_argument("--ckpt-format", default="native", choices=["native", "fp16", "bf16"])
+    ap.add_argument("--batch-steps", type=int, default=400, help="steps of the batched-shots leg (0: skip)")
+    ap.add_argument("--batch-solo-steps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (scale factor for HBM byte counters)")
     args = ap.parse_args()
@@ -222,6 +263,11 @@ def main():
                    "adjoint_jacobi_sweeps_per_step": round(sweeps / args.adjoint_steps / max(1, m.num_grids - 1), 2)}
         log("[bench] fwd+adjoint: %.2f Gpt/s, %.3f ms/step (%.2fx the FP32 forward)" % (ga, ms_a, ms_a / ms_step))
 
+    batch = None
+    if args.batch_steps > 0:
+        batch = batch_run(args.batch_steps, args.batch_solo_steps, args.warmup)
+        log("[bench] batch: %s" % batch)
+
     cpu = None
     if not args.no_cpu:
         ce, threads = cpu_baseline(m, args.cpu_steps)
@@ -255,6 +301,7 @@ def main():
                      "vel_kernel_GBps": round(vel_bw / 1e9, 1)},
         "fp64": fp64,
         "adjoint": adjoint,
+        "batch": batch,
         "jacobi_iters_per_step": [round(s[0] / (args.warmup + args.steps), 2) for s in stats],
         "cpu_baseline": cpu,
     }

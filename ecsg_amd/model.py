@@ -458,9 +458,61 @@ def two_grid_workload(seed: int = 0, nsteps: int = 1500) -> Model:
         z0 += h * nz
     dt = cfl_dt(grids, iso=False, frac=0.9)
     g0 = grids[0]
-    src = map_location(grids, g0.n[0] // 2 * h, g0.n[1] // 2 * h, 30.0)
     src = map_location(grids, g0.n[0] // 2 * g0.h[0], g0.n[1] // 2 * g0.h[1], 30.0)
     rec = np.array([[0, x, g0.n[1] // 2, 1] for x in range(0, g0.n[0] + 1, 4)], np.int32)
     return Model(grids=grids, dt=dt, nsteps=nsteps, iso=False, top_bc=1, src_grid=src[0], src_loc=src[1:],
                  src_type=0, src_trace=ricker(min(nsteps, 300), dt, 20.0), rec_loc=rec,
                  name="two_grid_241x241x61_81x81x121")
+
+
+def batch_workload(seed: int = 0, nmodels: int = 16, shots: int = 4, nsteps: int = 1000, pert: float = 0.05) -> List[Model]:
+    """Batched shots over uneven two-grid models: nmodels ratio-3 composites
+    whose fine grids have (nx, ny) cells drawn from multiples of 3 in
+    [120, 240] and nz in [30, 60] at h = 5 m (vp 1200, vs 600, rho 1800), over
+    coarse grids of (nx/3, ny/3, 2 nz) cells at 15 m (vp 3500, vs 2000,
+    rho 2300); each of the 9 Cij and rho carries a seeded per-node
+    +-pert perturbation.  Every model gets `shots` Ricker sources (10-25 Hz,
+    seeded positions in the fine grid, vx force) and a receiver line along x.
+    Returns nmodels * shots Models; the shots of one model share its media."""
+    rng = np.random.default_rng(seed)
+    props = [(1200.0, 600.0, 1800.0), (3500.0, 2000.0, 2300.0)]
+    out = []
+    for mi in range(nmodels):
+        nx = 3 * int(rng.integers(40, 81))
+        ny = 3 * int(rng.integers(40, 81))
+        nz = int(rng.integers(30, 61))
+        sizes = [(nx, ny, nz), (nx // 3, ny // 3, 2 * nz)]
+        hs = [5.0, 15.0]
+        grids = []
+        z0 = 0.0
+        for n, h, (vp, vs, rho) in zip(sizes, hs, props):
+            shape = (n[2] + 1, n[1] + 1, n[0] + 1)
+            c11 = rho * vp * vp
+            c44 = rho * vs * vs
+            c12 = c11 - 2 * c44
+            base = [c11, c11, c11, c12, c12, c12, 2 * c44, 2 * c44, 2 * c44]
+
+            def noisy(v):
+                f = rng.random(shape, dtype=np.float32)
+                f *= np.float32(2 * pert)
+                f += np.float32(1 - pert)
+                f *= np.float32(v)
+                return f.astype(np.float64)
+
+            stiff = [noisy(v) for v in base]
+            buoy = (1.0 / noisy(rho)).astype(np.float32).astype(np.float64)
+            grids.append(Grid(tuple(n), (h, h, h), buoy, stiff, origin_z=z0))
+            z0 += h * n[2]
+        dt = cfl_dt(grids, iso=False, frac=0.9)
+        g0 = grids[0]
+        rec = np.array([[0, x, g0.n[1] // 2, 1] for x in range(0, g0.n[0] + 1, 4)], np.int32)
+        for si in range(shots):
+            fpk = float(rng.uniform(10.0, 25.0))
+            x = float(rng.uniform(0.2, 0.8)) * g0.n[0] * g0.h[0]
+            y = float(rng.uniform(0.2, 0.8)) * g0.n[1] * g0.h[1]
+            z = float(rng.uniform(10.0, 0.8 * g0.n[2] * g0.h[2]))
+            src = map_location(grids, x, y, z)
+            out.append(Model(grids=grids, dt=dt, nsteps=nsteps, iso=False, top_bc=1, src_grid=src[0], src_loc=src[1:],
+                             src_type=0, src_trace=ricker(min(nsteps, 400), dt, fpk), rec_loc=rec,
+                             name="batch_m%02d_s%d_%dx%dx%d" % (mi, si, nx + 1, ny + 1, nz + 1)))
+    return out
