@@ -31,7 +31,7 @@ ABI_SYMBOLS = (
     "ecsg_compute_res", "ecsg_step_n", "ecsg_steps_done", "ecsg_synchronize", "ecsg_field_dims", "ecsg_get_field",
     "ecsg_set_field", "ecsg_get_traces", "ecsg_get_energy", "ecsg_jacobi_stats", "ecsg_profile_kernels",
     "ecsg_reset", "ecsg_set_checkpointing", "ecsg_gradient", "ecsg_get_gradient", "ecsg_adjoint_step",
-    "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune",
+    "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune", "ecsg_set_fused",
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
     "ecsg_batch_set_tiling", "ecsg_batch_autotune",
 )
@@ -106,6 +106,7 @@ def load_library(path: Optional[str] = None):
     L.ecsg_set_adjoint_field.argtypes = [vp, i, i, P(d)]
     L.ecsg_adjoint_stats.argtypes = [vp, P(C.c_longlong)]
     L.ecsg_autotune.argtypes = [vp, i, P(i), P(i)]
+    L.ecsg_set_fused.argtypes = [vp, i]
     L.ecsg_batch_create.argtypes = [P(vp), i, P(vp)]
     L.ecsg_batch_destroy.argtypes = [vp]
     L.ecsg_batch_last_error.argtypes = [vp]
@@ -299,6 +300,10 @@ class Simulation:
         n = C.c_longlong()
         self._chk(self.L.ecsg_adjoint_stats(self.h, C.byref(n)))
         return n.value
+
+    def set_fused(self, on: bool = True):
+        """Fused velocity+stress pass in step() (identical results)."""
+        self._chk(self.L.ecsg_set_fused(self.h, int(bool(on))))
 
     def autotune(self, reps: int = 5):
         """Per-grid stencil tiling chosen by timing (state preserved):

@@ -5,6 +5,7 @@
 #include "ecsg_kernels.hpp"
 #include "ecsg_adjoint.hpp"
 #include "ecsg_batch.hpp"
+#include "ecsg_fused.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -77,6 +78,7 @@ public:
     virtual int adj_field(int g, int sel, double *out, const double *in) = 0;
     virtual int adjoint_stats(long long *sweeps, double *ms) = 0;
     virtual int autotune(int reps, int *by_out, int *zc_out) = 0;
+    virtual int set_fused(int on) = 0;
 };
 
 #define HIPCHK(x)                                                                                  \
@@ -99,7 +101,14 @@ struct DevGrid {
     // host copies (double of the T-rounded values) of the planes the
     // interface matrices need: buoyancy and c33 on iz = 0 and iz = nz
     std::vector<double> b_top, b_bot, c33_top, c33_bot;
-    T *arr(int a) const { return base + (long)a * astride; }  // 0..8 fields, 9..17 stiffness, 18 buoy
+    // arrays of base: 0..8 field set 0, 9..17 stiffness, 18 buoyancy; field
+    // set 1 (alt, allocated when the fused pass is enabled) holds the other
+    // buffer of the fused step; the fields of the current state are set `cur`
+    int cur = 0;
+    T *alt = nullptr;
+    T *fset(int k) const { return k ? alt : base; }
+    T *fields() const { return fset(cur); }
+    T *arr(int a) const { return a < 9 ? fields() + (long)a * astride : base + (long)a * astride; }
 };
 
 struct DevIface {
@@ -135,6 +144,12 @@ public:
     int max_tiled = 0; // co-resident workgroups of the tiled Jacobi kernel
     bool lds_ok = false;  // every interface solved by k_jacobi_lds
     long lds_maxw = 0;
+    // fused velocity+stress pass on double-buffered fields
+    int use_fused = 0;
+    int fby = 8;                 // rows per fused workgroup (4 or 8)
+    bool sets_dirty = false;     // field sets may differ in entries no kernel writes
+    hipGraphExec_t gexec_f = nullptr;
+    hipGraph_t graph_f = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     bool graph_failed = false;
@@ -149,6 +164,7 @@ public:
         if (graph) hipGraphDestroy(graph);
         for (auto &g : G) {
             hipFree(g.base);
+            hipFree(g.alt);
             hipFree(g.ke);
             hipFree(g.se);
         }
@@ -186,6 +202,7 @@ public:
         if (sizeof(T) == 8) { tol_cap = 1e-12; tol_fac = 1e-5; } else { tol_cap = 1e-3; tol_fac = 1e-4; }
         if (const char *e = getenv("ECSG_TILE_BY")) tile_by = atoi(e);
         if (const char *e = getenv("ECSG_TILE_ZC")) tile_zc = atoi(e);
+        if (const char *e = getenv("ECSG_FUSED_ROWS")) fby = atoi(e) == 4 ? 4 : 8;
         if (tile_by != 1 && tile_by != 2 && tile_by != 4 && tile_by != 8) return fail(ECSG_EINVAL, "tile height must be 1, 2, 4 or 8");
         if (tile_zc < 1) return fail(ECSG_EINVAL, "z chunk must be >= 1");
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -549,6 +566,185 @@ public:
     {
         if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+        if (gexec_f) { hipGraphExecDestroy(gexec_f); gexec_f = nullptr; }
+        if (graph_f) { hipGraphDestroy(graph_f); graph_f = nullptr; }
+    }
+
+    // ----- fused step ------------------------------------------------------
+    bool fused_ok() const { return use_fused && !cfg.check_energy; }
+    int set_fused(int on) override
+    {
+        HIPCHK(hipStreamSynchronize(s));
+        if (on)
+            for (auto &d : G)
+                if (!d.alt) {
+                    if (int rc = dalloc(&d.alt, (size_t)d.astride * 9)) return rc;
+                    HIPCHK(hipMemcpy(d.alt, d.fields(), sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+                }
+        use_fused = on ? 1 : 0;
+        invalidate_graph();
+        return ECSG_OK;
+    }
+
+    FusedDev<T> fused_dev(int g)
+    {
+        DevGrid<T> &d = G[g];
+        FusedDev<T> a{};
+        a.g = d.dev;
+        T *o = d.fset(d.cur ^ 1);
+        for (int f = 0; f < 9; f++) a.o[f] = o + (long)f * d.astride;
+        a.defer0 = g > 0;
+        a.defern = g + 1 < (int)G.size();
+        a.zc = d.dev.zc;
+        a.gx = (d.n[0] + 1 + FXO - 1) / FXO;
+        a.gy = (d.n[1] + 1 + fby - 1) / fby;
+        a.gz = (d.n[2] + 1 + a.zc - 1) / a.zc;
+        a.src_p = -1;
+        if (g == src_g && src_len > 0) {
+            a.src_p = d.o + (long)src_loc[2] * d.S2 + (long)src_loc[1] * d.P + src_loc[0];
+            a.src_type = src_type;
+            a.src_len = src_len;
+            a.src_tr = d_src_tr;
+        }
+        a.step = d_step;
+        return a;
+    }
+
+    template <int FB>
+    void fused_launch(const FusedDev<T> &a)
+    {
+        hipLaunchKernelGGL((k_fused<T, FB>), dim3(a.gx * a.gy * a.gz), dim3(64, FB + 2), 0, s, a);
+    }
+
+    int sync_sets()
+    {
+        for (auto &d : G)
+            HIPCHK(hipMemcpyAsync(d.fset(d.cur ^ 1), d.fields(), sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice, s));
+        sets_dirty = false;
+        return ECSG_OK;
+    }
+
+    // one time step with the fused pass (main.cpp:165-261, same results as
+    // enqueue_step): fused velocity+stress into the other field set,
+    // receivers/source, vz-szz ghosts, deferred interface-plane stresses,
+    // then the new set becomes current for the sxz and syz ghosts
+    int enqueue_fused_step()
+    {
+        const int ng = (int)G.size();
+        for (int g = 0; g < ng; g++) {
+            ev_begin();
+            FusedDev<T> a = fused_dev(g);
+            if (fby == 4) fused_launch<4>(a);
+            else fused_launch<8>(a);
+            ev_end(g == 0 ? 0 : 2);
+        }
+        HIPCHK(hipGetLastError());
+        {
+            PointArgs<T> pa = point_args();
+            bool sdefer = false;
+            if (src_g >= 0 && src_type == 1)
+                sdefer = (src_g > 0 && src_loc[2] == 0) || (src_g + 1 < ng && src_loc[2] == G[src_g].n[2]);
+            for (int g = 0; g < ng; g++) {
+                pa.vx[g] = G[g].fset(G[g].cur ^ 1);
+                pa.st[g][0] = pa.st[g][1] = pa.st[g][2] = pa.st[g][3] = nullptr;
+            }
+            pa.skip_s = sdefer ? 0 : 1;
+            pa.energy = 0;
+            ev_begin();
+            hipLaunchKernelGGL((k_recsrc<T>), dim3(1), dim3(256), 0, s, pa);
+            ev_end(5);
+        }
+        for (int k = 0; k + 1 < ng; k++) {
+            DevIface &f = I[k];
+            IfaceDev<T> a = iface_dev(k);
+            for (int c = 0; c < 3; c++) {
+                a.F.f[c] = G[k].fset(G[k].cur ^ 1) + (long)c * G[k].astride;
+                a.C.f[c] = G[k + 1].fset(G[k + 1].cur ^ 1) + (long)c * G[k + 1].astride;
+            }
+            const long nl = (long)f.nx2 * (f.ny2 - 1);
+            ev_begin();
+            hipLaunchKernelGGL((k_gvz_rhs<T>), dim3((nl + 255) / 256), dim3(256), 0, s, a, f.B);
+            hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Dp, f.U);
+            const long nf = (long)f.r * f.nx2 * (f.r * f.ny2 - 1);
+            hipLaunchKernelGGL((k_gvz_fine<T>), dim3((nf + 255) / 256), dim3(256), 0, s, a);
+            ev_end(4);
+        }
+        for (int g = 0; g < ng; g++) {
+            DevGrid<T> &d = G[g];
+            T *o = d.fset(d.cur ^ 1);
+            const long as = d.astride;
+            const int planes[2] = {g > 0 ? 0 : -1, g + 1 < ng ? d.n[2] : -1};
+            for (int pl : planes)
+                if (pl >= 0) {
+                    const long nt = (long)d.n[0] * (d.n[1] + 1);
+                    ev_begin();
+                    hipLaunchKernelGGL((k_stress_defer<T>), dim3((nt + 255) / 256), dim3(256), 0, s, d.dev, o, o + as, o + 2 * as,
+                                       o + 3 * as, o + 4 * as, o + 5 * as, pl);
+                    ev_end(4);
+                }
+        }
+        for (auto &d : G) {
+            d.cur ^= 1;
+            for (int f = 0; f < 9; f++) d.dev.f[f] = d.arr(f);
+        }
+        if (ng > 1) {
+            ev_begin();
+            for (int k = 0; k + 1 < ng; k++)
+                if (int rc = launch_sxz_rhs(k)) return rc;
+            if (lds_ok && cfg.sxz_solver == 0) {
+                if (int rc = launch_jacobi_lds(false)) return rc;
+            } else {
+                for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
+                    if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
+            }
+            ev_end(4);
+        }
+        for (int k = 0; k + 1 < ng; k++)
+            if (int rc = launch_ghost_syz(k)) return rc;
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
+    int step_fused(int n, int use_graph)
+    {
+        if (sets_dirty)
+            if (int rc = sync_sets()) return rc;
+        int left = n;
+        const bool can_graph = use_graph && !graph_failed && !prof && (I.empty() || (lds_ok && cfg.sxz_solver == 0));
+        if (can_graph) {
+            if (G[0].cur == 1 && left > 0) {
+                if (int rc = enqueue_fused_step()) return rc;
+                left--;
+            }
+            if (left >= 2 && !gexec_f) {
+                // a pair of steps returns the field sets to their places
+                if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                    int rc = enqueue_fused_step();
+                    if (rc == ECSG_OK) rc = enqueue_fused_step();
+                    hipGraph_t gr = nullptr;
+                    const hipError_t e2 = hipStreamEndCapture(s, &gr);
+                    if (rc == ECSG_OK && e2 == hipSuccess && gr && hipGraphInstantiate(&gexec_f, gr, nullptr, nullptr, 0) == hipSuccess) {
+                        graph_f = gr;
+                    } else {
+                        if (gr) hipGraphDestroy(gr);
+                        gexec_f = nullptr;
+                        graph_failed = true;
+                        hipGetLastError();
+                        st = Status{};
+                        // the capture toggled the host-side sets twice: unchanged
+                    }
+                } else {
+                    graph_failed = true;
+                    hipGetLastError();
+                }
+            }
+            if (gexec_f)
+                for (; left >= 2; left -= 2) HIPCHK(hipGraphLaunch(gexec_f, s));
+        }
+        for (; left > 0; left--)
+            if (int rc = enqueue_fused_step()) return rc;
+        host_step += n;
+        return ECSG_OK;
     }
 
     // ----- launches --------------------------------------------------------
@@ -968,6 +1164,7 @@ public:
         if (int rc = check_ready()) return rc;
         if (n < 0) return fail(ECSG_EINVAL, "negative step count");
         if (host_step + n > cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded");
+        if (fused_ok()) return step_fused(n, use_graph);
         // The tiled interface Jacobi is a cooperative launch; its behaviour
         // inside a captured graph is not relied on, so composite models with
         // interfaces too large for the LDS solver run eagerly.
@@ -1065,7 +1262,9 @@ public:
         for (int k = -r; k < len[2] - r; k++)
             for (int j = -r; j < len[1] - r; j++)
                 for (int i = -r; i < len[0] - r; i++, q++) buf[d.o + (long)k * d.S2 + (long)j * d.P + i] = (T)in[q];
-        HIPCHK(hipMemcpy(d.arr(sel), buf.data(), sizeof(T) * d.astride, hipMemcpyHostToDevice));
+        // both field sets: entries no kernel writes must agree between them
+        HIPCHK(hipMemcpy(d.fset(0) + (long)sel * d.astride, buf.data(), sizeof(T) * d.astride, hipMemcpyHostToDevice));
+        if (d.alt) HIPCHK(hipMemcpy(d.alt + (long)sel * d.astride, buf.data(), sizeof(T) * d.astride, hipMemcpyHostToDevice));
         return ECSG_OK;
     }
 
@@ -1115,8 +1314,9 @@ public:
         ev_next = 0;
         ev_kind.clear();
         prof = true;
+        if (fused_ok() && sets_dirty) sync_sets();
         for (int i = 0; i < nsteps; i++)
-            if (int rc = enqueue_step()) { prof = false; return rc; }
+            if (int rc = fused_ok() ? enqueue_fused_step() : enqueue_step()) { prof = false; return rc; }
         host_step += nsteps;
         prof = false;
         HIPCHK(hipStreamSynchronize(s));
@@ -1536,7 +1736,10 @@ public:
 
     int reset() override
     {
-        for (auto &d : G) HIPCHK(hipMemsetAsync(d.base, 0, sizeof(T) * 9 * d.astride, s));
+        for (auto &d : G) {
+            HIPCHK(hipMemsetAsync(d.fset(0), 0, sizeof(T) * 9 * d.astride, s));
+            if (d.alt) HIPCHK(hipMemsetAsync(d.alt, 0, sizeof(T) * 9 * d.astride, s));
+        }
         for (auto &f : I) {
             const long nxw = (long)(f.nx2 + 1) * (f.ny2 + 1);
             HIPCHK(hipMemsetAsync(f.X0, 0, sizeof(double) * nxw, s));
@@ -1613,7 +1816,7 @@ public:
         for (int g = 0; g < ng; g++) {
             const long as = G[g].astride;
             if (ck_fmt == 0) {
-                HIPCHK(hipMemcpyAsync(ck_nat[(size_t)j * ng + g], G[g].base, sizeof(T) * 9 * as, hipMemcpyDeviceToDevice, s));
+                HIPCHK(hipMemcpyAsync(ck_nat[(size_t)j * ng + g], G[g].fields(), sizeof(T) * 9 * as, hipMemcpyDeviceToDevice, s));
             } else {
                 for (int f = 0; f < 9; f++) {
                     const T *x = G[g].arr(f);
@@ -1638,7 +1841,7 @@ public:
         for (int g = 0; g < ng; g++) {
             const long as = G[g].astride;
             if (ck_fmt == 0) {
-                HIPCHK(hipMemcpyAsync(G[g].base, ck_nat[(size_t)j * ng + g], sizeof(T) * 9 * as, hipMemcpyDeviceToDevice, s));
+                HIPCHK(hipMemcpyAsync(G[g].fields(), ck_nat[(size_t)j * ng + g], sizeof(T) * 9 * as, hipMemcpyDeviceToDevice, s));
             } else {
                 for (int f = 0; f < 9; f++)
                     hipLaunchKernelGGL((k_ck_unpack<T>), dim3(1024), dim3(256), 0, s, G[g].arr(f), as,
@@ -1650,6 +1853,7 @@ public:
             HIPCHK(hipMemcpyAsync(I[k].X0, ck_x0[(size_t)j * ni + k], sizeof(double) * (I[k].nx2 + 1) * (I[k].ny2 + 1),
                                   hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, d_step, step);
+        sets_dirty = true;
         HIPCHK(hipGetLastError());
         return ECSG_OK;
     }
@@ -1667,7 +1871,7 @@ public:
     {
         const int ng = (int)G.size();
         for (int g = 0; g < ng; g++)
-            HIPCHK(hipMemcpyAsync(ring[(size_t)slot * ng + g], G[g].base, sizeof(T) * 9 * G[g].astride, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(ring[(size_t)slot * ng + g], G[g].fields(), sizeof(T) * 9 * G[g].astride, hipMemcpyDeviceToDevice, s));
         return ECSG_OK;
     }
 
@@ -1867,7 +2071,7 @@ public:
             DevGrid<T> &d = G[g];
             T *save = nullptr;
             HIPCHK(hipMalloc((void **)&save, sizeof(T) * 9 * d.astride));
-            HIPCHK(hipMemcpy(save, d.base, sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpy(save, d.fields(), sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
             float best = 1e30f;
             int bb = d.by, bz = d.dev.zc;
             for (int b : bys)
@@ -1887,7 +2091,7 @@ public:
                     if (ms < best) { best = ms; bb = b; bz = z; }
                 }
             set_tiling(d, bb, bz);
-            HIPCHK(hipMemcpy(d.base, save, sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpy(d.fields(), save, sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
             hipFree(save);
             if (by_out) by_out[g] = bb;
             if (zc_out) zc_out[g] = bz;
@@ -1986,6 +2190,16 @@ public:
         }
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        return build_tables();
+    }
+
+    // descriptor tables of the members' current field sets
+    std::vector<int> built_cur;
+    int build_tables()
+    {
+        built_cur.clear();
+        for (Engine<T> *e : M) built_cur.push_back(e->G[0].cur);
+        maxw = 0;
         if (int rc = build_grids()) return rc;
         // interfaces, Jacobi descriptors and point kernels
         std::vector<IfWork<T>> ifs;
@@ -2098,6 +2312,12 @@ public:
             if (e->host_step + n > e->cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded by a member");
             HIPCHK(hipStreamSynchronize(e->s));
         }
+        // a member stepped on its own since (fused steps swap its field sets)
+        for (size_t m = 0; m < M.size(); m++)
+            if (M[m]->G[0].cur != built_cur[m]) {
+                if (int rc = build_tables()) return rc;
+                break;
+            }
         if (use_graph && !gexec) {
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const int rc = enqueue();
@@ -2152,7 +2372,7 @@ public:
             for (auto &d : e->G) {
                 T *p = nullptr;
                 HIPCHK(hipMalloc((void **)&p, sizeof(T) * 9 * d.astride));
-                HIPCHK(hipMemcpy(p, d.base, sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+                HIPCHK(hipMemcpy(p, d.fields(), sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
                 save.push_back(p);
             }
         hipEvent_t e0, e1;
@@ -2182,7 +2402,7 @@ public:
         size_t q = 0;
         for (Engine<T> *e : M)
             for (auto &d : e->G) {
-                HIPCHK(hipMemcpy(d.base, save[q], sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
+                HIPCHK(hipMemcpy(d.fields(), save[q], sizeof(T) * 9 * d.astride, hipMemcpyDeviceToDevice));
                 hipFree(save[q]);
                 q++;
             }
@@ -2318,6 +2538,8 @@ int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps)
     CTX_CHECK(ctx);
     return ret(ctx, ctx->e->adjoint_stats(sweeps, nullptr));
 }
+
+int ecsg_set_fused(ecsg_ctx *ctx, int on) { CTX_CHECK(ctx); return ret(ctx, ctx->e->set_fused(on)); }
 
 int ecsg_autotune(ecsg_ctx *ctx, int reps, int *by_out, int *zc_out)
 {

@@ -1212,6 +1212,7 @@ struct PointArgs {
     const T *src_old;    // pre-update vx at the source point (energy)
     double *fix;         // kinetic-energy correction of the injected source
     int energy;
+    int skip_s;          // 1: an explosive source is not added here (the fused pass folded it in)
 };
 
 // main.cpp:183-195: receivers sample vx after UpdateVel, then the source is added.
@@ -1241,7 +1242,7 @@ __device__ __forceinline__ void recsrc_body(const PointArgs<T> &a)
                                    * ((a.src_i == 0 || a.src_i == n[0]) ? 0.5 : 1.0);
                     fix = m * (1.0 / (double)a.b[g][q]) * (double)(T)add * (double)(*a.src_old);
                 }
-            } else {
+            } else if (!a.skip_s) {
                 const T add = (T)a.src_tr[step];
                 a.sxx[g][q] = a.sxx[g][q] + add;
                 a.syy[g][q] = a.syy[g][q] + add;

@@ -184,6 +184,13 @@ int ecsg_set_adjoint_field(ecsg_ctx *ctx, int g, int sel, const double *in);
 /* Total Jacobi sweeps of the transposed vx-sxz solves since the last gradient. */
 int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps);
 
+/* ecsg_step_n's velocity and stress updates in one fused pass (default off;
+ * used when the energy check is off): the pass reads the state before the
+ * step and writes the state after it into a second set of field arrays
+ * (allocated by the first call with on = 1), then the sets swap.  Results are
+ * identical to the separate in-place passes. */
+int ecsg_set_fused(ecsg_ctx *ctx, int on);
+
 /* Stencil tiling of each grid (workgroup height by_out[g] in {1,2,4,8} rows
  * of 64, z chunk zc_out[g] in {2,4,8,16} planes per thread) chosen by timing
  * the velocity and stress launches of every candidate reps times on a saved
