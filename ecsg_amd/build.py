@@ -1,4 +1,5 @@
-"""Build libecsg_hip.so (gfx950) in-tree with hipcc.
+"""Build libecsg_hip.so (gfx950) in-tree with hipcc, and the command-line
+driver ecsg_run (host C++, linked against it) beside it.
 
     python -m ecsg_amd.build
 
@@ -17,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libecsg_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "ecsg_capi.hip")]
+DRIVER = os.path.join(HERE, "lib", "ecsg_run")
+DRIVER_SOURCES = [os.path.join(HERE, "driver", "ecsg_run.cpp")]
 ARCH = os.environ.get("ECSG_OFFLOAD_ARCH", "gfx950")
 
 
@@ -43,7 +46,25 @@ def build(verbose: bool = False) -> str:
     if out.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + out.stdout + out.stderr)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+def build_driver(verbose: bool = False) -> str:
+    """g++ build of ecsg_run, found at run time next to libecsg_hip.so."""
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    tmp = DRIVER + ".tmp%d" % os.getpid()
+    libdir = os.path.dirname(LIB)
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "driver")]
+    cmd += DRIVER_SOURCES + ["-L", libdir, "-lecsg_hip", "-Wl,-rpath,$ORIGIN",
+                             "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + out.stdout + out.stderr)
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -60,6 +60,7 @@ public:
     virtual int ghost_syz(int k) = 0;
     virtual int end_step() = 0;
     virtual int residual(int which, double *out) = 0;
+    virtual int avg_abs(int g, int sel, double *out) = 0;
     virtual int step_n(int n, int use_graph) = 0;
     virtual int steps_done() const = 0;
     virtual int sync() = 0;
@@ -134,6 +135,9 @@ public:
     T *d_traces = nullptr;
     int *d_step = nullptr;
     double *d_energy = nullptr, *d_fix = nullptr;
+    double *d_diag = nullptr;  // residual / AvgAbs reductions (DIAG_N doubles)
+    T *e_save = nullptr;       // stress snapshot of the per-phase energy call
+    size_t e_save_n = 0;
     T *d_src_old = nullptr;
     int *d_src_loc = nullptr;
     float *d_src_tr = nullptr;
@@ -173,7 +177,7 @@ public:
             hipFree(f.Bs); hipFree(f.A); hipFree(f.absp); hipFree(f.X0); hipFree(f.X1);
             hipFree(f.jpart); hipFree(f.stats);
         }
-        hipFree(d_rec); hipFree(d_traces); hipFree(d_step); hipFree(d_energy); hipFree(d_fix);
+        hipFree(d_rec); hipFree(d_traces); hipFree(d_step); hipFree(d_energy); hipFree(d_fix); hipFree(d_diag); hipFree(e_save);
         hipFree(d_src_old); hipFree(d_src_loc); hipFree(d_src_tr);
         adj_free();
         for (auto e : evs) hipEventDestroy(e);
@@ -1108,10 +1112,19 @@ public:
         // strain energy without modifying the fields: run the stress kernel on
         // a copy of the stress arrays would double the traffic; instead the
         // per-phase path computes it from a snapshot of the six stresses.
+        size_t need = 0;
+        for (const DevGrid<T> &d : G) need = std::max(need, (size_t)d.astride * 6);
+        if (e_save_n < need) {
+            HIPCHK(hipStreamSynchronize(s));
+            hipFree(e_save);
+            e_save = nullptr;
+            e_save_n = 0;
+            if (int rc = dalloc(&e_save, need)) return rc;
+            e_save_n = need;
+        }
         for (int g = 0; g < (int)G.size(); g++) {
             DevGrid<T> &d = G[g];
-            T *save = nullptr;
-            if (int rc = dalloc(&save, (size_t)d.astride * 6)) return rc;
+            T *save = e_save;
             HIPCHK(hipMemcpyAsync(save, d.arr(3), sizeof(T) * d.astride * 6, hipMemcpyDeviceToDevice, s));
             switch (d.by) {
             case 1: stress_launch<1>(d, true); break;
@@ -1121,8 +1134,6 @@ public:
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(d.arr(3), save, sizeof(T) * d.astride * 6, hipMemcpyDeviceToDevice, s));
-            HIPCHK(hipStreamSynchronize(s));
-            hipFree(save);
         }
         // energy kernel writes energy[step] and advances the counter; undo
         int step = 0;
@@ -1146,16 +1157,40 @@ public:
     int residual(int which, double *out) override
     {
         if (which < 0 || which > 2) return fail(ECSG_EINVAL, "which must be 0, 1 or 2");
-        double *d_out = nullptr;
         const int ni = (int)I.size();
         if (ni == 0) return ECSG_OK;
-        if (int rc = dalloc(&d_out, ni)) return rc;
+        if (!d_diag)
+            if (int rc = dalloc(&d_diag, DIAG_N)) return rc;
         for (int k = 0; k < ni; k++)
-            hipLaunchKernelGGL((k_res<T>), dim3(1), dim3(256), 0, s, iface_dev(k), which, d_out + k);
+            hipLaunchKernelGGL((k_res<T>), dim3(1), dim3(256), 0, s, iface_dev(k), which, d_diag + k);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, d_diag, sizeof(double) * ni, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        HIPCHK(hipMemcpy(out, d_out, sizeof(double) * ni, hipMemcpyDeviceToHost));
-        hipFree(d_out);
+        return ECSG_OK;
+    }
+
+    static constexpr int DIAG_PARTS = 1024;
+    static constexpr int DIAG_N = DIAG_PARTS + 8;
+
+    // Field::AvgAbs of wavefield sel on grid g (over the field's own range)
+    int avg_abs(int g, int sel, double *out) override
+    {
+        if (int rc = check_ready()) return rc;
+        if (g < 0 || g >= (int)G.size() || sel < 0 || sel > 8) return fail(ECSG_EINVAL, "bad grid or wavefield selector");
+        if (!d_diag)
+            if (int rc = dalloc(&d_diag, DIAG_N)) return rc;
+        const DevGrid<T> &d = G[g];
+        int len[3];
+        for (int k = 0; k < 3; k++) len[k] = SIM_POS[sel][k] ? d.n[k] : d.n[k] + 1;
+        const long rows = (long)len[1] * len[2];
+        const int nb = (int)std::min<long>(rows, DIAG_PARTS);
+        hipLaunchKernelGGL((k_avgabs_part<T>), dim3(nb), dim3(256), 0, s, (const T *)d.arr(sel), d.o, d.P, d.S2,
+                           len[0], len[1], len[2], d_diag);
+        hipLaunchKernelGGL(k_avgabs_final, dim3(1), dim3(64), 0, s, (const double *)d_diag, nb, (double)len[2],
+                           (double)len[1], (double)len[0], d_diag + DIAG_PARTS);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, d_diag + DIAG_PARTS, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
         return ECSG_OK;
     }
 
@@ -2480,6 +2515,7 @@ int ecsg_update_ghost_sxz(ecsg_ctx *ctx, int k) { CTX_CHECK(ctx); return ret(ctx
 int ecsg_update_ghost_syz(ecsg_ctx *ctx, int k) { CTX_CHECK(ctx); return ret(ctx, ctx->e->ghost_syz(k)); }
 int ecsg_end_step(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->end_step()); }
 int ecsg_compute_res(ecsg_ctx *ctx, int which, double *out) { CTX_CHECK(ctx); return ret(ctx, ctx->e->residual(which, out)); }
+int ecsg_field_avgabs(ecsg_ctx *ctx, int g, int sel, double *out) { CTX_CHECK(ctx); return ret(ctx, ctx->e->avg_abs(g, sel, out)); }
 int ecsg_step_n(ecsg_ctx *ctx, int nsteps, int use_graph) { CTX_CHECK(ctx); return ret(ctx, ctx->e->step_n(nsteps, use_graph)); }
 int ecsg_steps_done(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->steps_done() : -1; }
 int ecsg_synchronize(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sync()); }

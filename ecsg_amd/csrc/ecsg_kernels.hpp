@@ -1338,4 +1338,29 @@ __global__ __launch_bounds__(256) void k_res(IfaceDev<T> a, int which, double *o
 
 #undef FIDX
 
+// Field::AvgAbs (field.cpp:215-228) over the field's own range (no halo):
+// per-workgroup sums of |f| over x-rows, then one fixed-order sum.
+template <typename T>
+__global__ __launch_bounds__(256) void k_avgabs_part(const T *__restrict__ f, long o, long P, long S2, int lx, int ly, int lz,
+                                                     double *__restrict__ part)
+{
+    __shared__ double lds[4];
+    const long rows = (long)ly * lz;
+    double acc = 0;
+    for (long row = blockIdx.x; row < rows; row += gridDim.x) {
+        const long iy = row % ly, iz = row / ly;
+        const T *p = f + o + iz * S2 + iy * P;
+        for (int ix = threadIdx.x; ix < lx; ix += 256) acc += fabs((double)p[ix]);
+    }
+    const double s = block_sum<4>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(64) void k_avgabs_final(const double *__restrict__ part, int n, double cz, double cy, double cx,
+                                                    double *__restrict__ out)
+{
+    const double s = wave_sum_fixed(part, n);
+    if (threadIdx.x == 0) *out = s / cz / cy / cx;
+}
+
 }  // namespace ecsg

@@ -64,6 +64,12 @@ class Model:
     src_trace: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
     rec_loc: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
     name: str = "model"
+    # positions in metres and movie timing as a flow script hands them to the
+    # command-line driver (None: derived from the grid indices and nsteps)
+    src_xyz: Optional[tuple] = None
+    rec_xyz: Optional[np.ndarray] = None
+    movie_dt: Optional[float] = None
+    movie_nt: Optional[int] = None
 
     @property
     def num_grids(self) -> int:
@@ -210,17 +216,21 @@ def reference_flow(test: int = 1, steps: Optional[int] = None, eci: bool = True)
     rx = [int(lx * f / dx2) * dx2 for f in (0.55, 0.6, 0.65, 0.7)]
     rz = int(lz1 * 0.2 / dx2) * dx2
     rec = []
+    rec_xyz = []
     for x in rx:
         # receiver file: ascii %f -> float32 (eci/SConstruct:169-175)
         xf, yf, zf = (float(np.float32(_f6(v))) for v in (x, src_y, rz))
         rec.append(map_location(grids, xf, yf, zf))
+        rec_xyz.append((xf, yf, zf))
 
     movie_dt = _f6(30 * dt_exact)
     movie_nt = int(duration / (30 * dt_exact))
     nsteps = int(math.ceil(movie_dt * movie_nt / dt)) + 1 if steps is None else steps
     return Model(grids=grids, dt=dt, nsteps=nsteps, iso=True, top_bc=1, imode=1 if eci else 0,
                  src_grid=sg, src_loc=(six, siy, siz), src_type=0, src_trace=src_trace,
-                 rec_loc=np.array(rec, np.int32), name="reference_test%d" % test)
+                 rec_loc=np.array(rec, np.int32), name="reference_test%d" % test,
+                 src_xyz=(_f6(src_x), _f6(src_y), _f6(src_z)), rec_xyz=np.array(rec_xyz, np.float32),
+                 movie_dt=movie_dt, movie_nt=movie_nt)
 
 
 # ---------------------------------------------------------------------------

@@ -113,6 +113,12 @@ int ecsg_end_step(ecsg_ctx *ctx);                     /* niter++ of main.cpp:165
  * value per interface. */
 int ecsg_compute_res(ecsg_ctx *ctx, int which, double *out);
 
+/* Field::AvgAbs (field.cpp:215-228) of wavefield sel (0..8) of grid g, over
+ * the field's own range without halo: the companion values of the residual
+ * trace (compute_res.cpp:21-23, 52-54, 84-86).  Summed on the device in a
+ * fixed order. */
+int ecsg_field_avgabs(ecsg_ctx *ctx, int g, int sel, double *out);
+
 /* Runs nsteps full steps of main.cpp:165-261 on the device (receivers and
  * energy recorded on the device), captured once into a hipGraph and replayed
  * when use_graph != 0. */
