@@ -53,6 +53,7 @@ class EcsgConfig(C.Structure):
         ("max_steps", C.c_int),
         ("vy_mirror_all", C.c_int),
         ("sxz_solver", C.c_int),
+        ("ii_vz_xend", C.c_int),
     ]
 
 
@@ -133,7 +134,7 @@ class Simulation:
     """
 
     def __init__(self, m, precision: str = "f64", check_energy: bool = True, max_steps: Optional[int] = None,
-                 vy_mirror_all: bool = False, sxz_solver: str = "jacobi"):
+                 vy_mirror_all: bool = False, sxz_solver: str = "jacobi", ii_vz_xend: bool = False):
         self.L = load_library()
         self.model = m
         cfg = EcsgConfig()
@@ -151,6 +152,7 @@ class Simulation:
         cfg.max_steps = int(m.nsteps if max_steps is None else max_steps)
         cfg.vy_mirror_all = int(vy_mirror_all)
         cfg.sxz_solver = {"jacobi": 0, "chebyshev": 1}[sxz_solver]
+        cfg.ii_vz_xend = int(ii_vz_xend)
         self.max_steps = cfg.max_steps
         self.precision = precision
         self.h = C.c_void_p()

@@ -6,6 +6,7 @@
 #include "ecsg_adjoint.hpp"
 #include "ecsg_batch.hpp"
 #include "ecsg_fused.hpp"
+#include "ecsg_ii.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -199,7 +200,7 @@ public:
         cfg = c;
         if (c.num_grids < 1 || c.num_grids > ECSG_MAX_GRIDS) return fail(ECSG_EINVAL, "num_grids out of range");
         if (c.radius != 1) return fail(ECSG_EINVAL, "only radius 1 is implemented on the device");
-        if (c.imode != 1) return fail(ECSG_EINVAL, "only the energy-conserving interface (imode 1) is implemented");
+        if (c.imode != 0 && c.imode != 1) return fail(ECSG_EINVAL, "imode must be 0 (II) or 1 (ECI)");
         if (c.top_bc != 0 && c.top_bc != 1) return fail(ECSG_EINVAL, "top_bc must be 0 or 1");
         if (!(c.dt > 0)) return fail(ECSG_EINVAL, "dt must be positive");
         if (c.max_steps < 1) return fail(ECSG_EINVAL, "max_steps must be >= 1");
@@ -255,6 +256,8 @@ public:
             const int r = F.n[0] / C.n[0];
             if (r < 2 || F.n[0] != r * C.n[0] || F.n[1] != r * C.n[1])
                 return fail(ECSG_EINVAL, "interface " + std::to_string(k) + ": fine x/y cells must be an integer ratio r >= 2 of the coarse ones");
+            if (c.imode == 0 && r != 2)
+                return fail(ECSG_EINVAL, "interface " + std::to_string(k) + ": the II interpolation (imode 0) is defined for ratio 2 only");
             DevIface &f = I[k];
             f.r = r; f.nx2 = C.n[0]; f.ny2 = C.n[1]; f.nz1 = F.n[2];
             const long nl = (long)f.nx2 * (f.ny2 - 1);
@@ -578,6 +581,7 @@ public:
     bool fused_ok() const { return use_fused && !cfg.check_energy; }
     int set_fused(int on) override
     {
+        if (on && cfg.imode == 0 && !I.empty()) return fail(ECSG_EINVAL, "the fused pass runs the energy-conserving interface (imode 1) only");
         HIPCHK(hipStreamSynchronize(s));
         if (on)
             for (auto &d : G)
@@ -840,6 +844,14 @@ public:
     {
         DevIface &f = I[k];
         IfaceDev<T> a = iface_dev(k);
+        if (cfg.imode == 0) {
+            const long nt = (long)f.nx2 * (f.ny2 - 1) + (long)(f.nx2 - 1) * f.ny2 + f.ny2;
+            ev_begin();
+            hipLaunchKernelGGL((k_ii_vz<T>), dim3((nt + 255) / 256), dim3(256), 0, s, a, cfg.ii_vz_xend ? 1 : 0);
+            ev_end(4);
+            HIPCHK(hipGetLastError());
+            return ECSG_OK;
+        }
         const long nl = (long)f.nx2 * (f.ny2 - 1);
         ev_begin();
         hipLaunchKernelGGL((k_gvz_rhs<T>), dim3((nl + 255) / 256), dim3(256), 0, s, a, f.B);
@@ -940,6 +952,15 @@ public:
 
     int launch_ghost_sxz(int k)
     {
+        if (cfg.imode == 0) {
+            const DevIface &f = I[k];
+            const long nt = (long)(f.nx2 + 1) * (f.ny2 + 1);
+            ev_begin();
+            hipLaunchKernelGGL((k_ii_sxz<T>), dim3((nt + 255) / 256), dim3(256), 0, s, iface_dev(k));
+            ev_end(4);
+            HIPCHK(hipGetLastError());
+            return ECSG_OK;
+        }
         ev_begin();
         if (int rc = launch_sxz_rhs(k)) return rc;
         if (lds_ok && cfg.sxz_solver == 0) {
@@ -963,7 +984,12 @@ public:
         IfaceDev<T> a = iface_dev(k);
         const long nc = (long)f.nx2 * f.ny2;
         ev_begin();
-        hipLaunchKernelGGL((k_gsyz<T>), dim3((nc + 255) / 256), dim3(256), 0, s, a);
+        if (cfg.imode == 0) {
+            const long nt = nc + 2L * f.nx2 + (2L * f.ny2 - 2);
+            hipLaunchKernelGGL((k_ii_syz<T>), dim3((nt + 255) / 256), dim3(256), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_gsyz<T>), dim3((nc + 255) / 256), dim3(256), 0, s, a);
+        }
         ev_end(4);
         HIPCHK(hipGetLastError());
         return ECSG_OK;
@@ -1031,6 +1057,13 @@ public:
             if (int rc = launch_stress(g)) return rc;
         if (cfg.check_energy)
             if (int rc = launch_energy()) return rc;
+        if (cfg.imode == 0) {  // main.cpp:242-251 with the II ghosts
+            for (int k = 0; k + 1 < ng; k++) {
+                if (int rc = launch_ghost_sxz(k)) return rc;
+                if (int rc = launch_ghost_syz(k)) return rc;
+            }
+            return ECSG_OK;
+        }
         // UpdateGhostSXZ of every interface (main.cpp:242-246): right-hand
         // sides, then the Jacobi solves of up to MAXI interfaces side by side
         if (ng > 1) {
@@ -1203,7 +1236,7 @@ public:
         // The tiled interface Jacobi is a cooperative launch; its behaviour
         // inside a captured graph is not relied on, so composite models with
         // interfaces too large for the LDS solver run eagerly.
-        if (use_graph && !graph_failed && !prof && (I.empty() || (lds_ok && cfg.sxz_solver == 0))) {
+        if (use_graph && !graph_failed && !prof && (I.empty() || cfg.imode == 0 || (lds_ok && cfg.sxz_solver == 0))) {
             if (!gexec) {
                 if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                     int rc = enqueue_step();
@@ -1565,6 +1598,7 @@ public:
     {
         if (adj_ready) return ECSG_OK;
         if (int rc = check_ready()) return rc;
+        if (cfg.imode == 0 && !I.empty()) return fail(ECSG_EINVAL, "the adjoint is implemented for the energy-conserving interface (imode 1)");
         adj_free();
         AG.resize(G.size());
         AI.resize(I.size());
@@ -2221,6 +2255,7 @@ public:
             if (!e->all_params()) return fail(ECSG_ESTATE, id + "parameters not set for every grid");
             if (e->cfg.check_energy) return fail(ECSG_EINVAL, id + "batched models run without the energy check");
             if (e->cfg.sxz_solver != 0) return fail(ECSG_EINVAL, id + "batched models use the Jacobi sweeps (sxz_solver 0)");
+            if (e->cfg.imode != 1 && !e->I.empty()) return fail(ECSG_EINVAL, id + "batched composite models use the energy-conserving interface (imode 1)");
             if (!e->I.empty() && !e->lds_ok) return fail(ECSG_EINVAL, id + "interface too large for the LDS Jacobi of a batch");
         }
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));

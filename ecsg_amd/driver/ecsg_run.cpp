@@ -19,6 +19,9 @@
 //                       (code convention c44 = 2*C44) instead of s11/s13
 //   vy_mirror_all=y     mirror vy across the y walls on every plane
 //   sxz_solver=0|1      vx-sxz interface system: Jacobi, or Chebyshev-accelerated
+//   ii_vz_xend=y        imode=0: the working copy's form of the x-end vz ghost
+//                       edge (update_ghost_vz.cpp:98-107) instead of the
+//                       committed one the golden II traces were made with
 //   graph=y|n           replay the time step from a HIP graph (default y)
 //   dry_run=y           read and check every input, print the mapped source and
 //                       receiver positions, and stop before any device work
@@ -283,6 +286,7 @@ int run(int argc, char **argv)
     cfg.max_steps = nsteps;
     cfg.vy_mirror_all = P.getb("vy_mirror_all", false) ? 1 : 0;
     cfg.sxz_solver = (int)P.geti("sxz_solver", 0);
+    cfg.ii_vz_xend = P.getb("ii_vz_xend", false) ? 1 : 0;
 
     Ctx C;
     if (ecsg_create(&cfg, &C.h) != ECSG_OK || !C.h) throw Fail(std::string("ecsg_create: ") + (C.h ? ecsg_last_error(C.h) : "failed"));

@@ -56,7 +56,8 @@ typedef struct {
     double dt;                          /* time step (main.cpp:51) */
     int radius;                         /* stencil radius; 1 supported (main.cpp:48) */
     int top_bc;                         /* 0 rigid, 1 free surface (main.cpp:60) */
-    int imode;                          /* 1 = energy-conserving interface (main.cpp:54) */
+    int imode;                          /* 1 = energy-conserving interface (ECI), 0 = the
+                                           "intuitive" interpolation (II, ratio 2 only), main.cpp:54 */
     int check_energy;                   /* main.cpp:45 */
     int precision;                      /* ECSG_F32 or ECSG_F64 */
     int max_steps;                      /* capacity of the trace / energy buffers */
@@ -68,6 +69,13 @@ typedef struct {
                                            reference (update_ghost_sxz.cpp:314-350); 1 = the same
                                            sweeps with Chebyshev acceleration (same fixed point and
                                            exit test, fewer sweeps) */
+    int ii_vz_xend;                     /* imode 0 only: the x-end edge of the fine vz ghosts
+                                           (update_ghost_vz.cpp:98-107).  0: left unwritten, as the
+                                           committed version of that line (whose stores fall outside
+                                           the field) -- the form the reference's golden II traces
+                                           were made with; 1: the working copy's form, including its
+                                           read of vz2 at the fine index nx1-1 (flat Field storage:
+                                           the element one coarse row length further on) */
 } ecsg_config;
 
 /* Creates the device state: wavefields (InitSim, init_del_sim.cpp:3-29),
@@ -101,11 +109,11 @@ int ecsg_set_receivers(ecsg_ctx *ctx, int n, const int *loc);
 int ecsg_update_vel(ecsg_ctx *ctx, int g);            /* UpdateVel, update_vel.cpp:4-249; main.cpp:179-180 */
 int ecsg_record_receivers(ecsg_ctx *ctx);             /* main.cpp:183-184 */
 int ecsg_add_source(ecsg_ctx *ctx);                   /* main.cpp:194-195 */
-int ecsg_update_ghost_vz(ecsg_ctx *ctx, int k);       /* UpdateGhostVZ, update_ghost_vz.cpp:124-250; main.cpp:206-209 */
+int ecsg_update_ghost_vz(ecsg_ctx *ctx, int k);       /* UpdateGhostVZ, update_ghost_vz.cpp:124-250 (imode 0: 36-123); main.cpp:206-209 */
 int ecsg_compute_energy(ecsg_ctx *ctx, double *e);    /* ComputeEnergy, compute_energy.cpp:34-54; main.cpp:217-225 */
 int ecsg_update_stress(ecsg_ctx *ctx, int g);         /* UpdateStress, update_stress.cpp:4-311; main.cpp:228-229 */
-int ecsg_update_ghost_sxz(ecsg_ctx *ctx, int k);      /* UpdateGhostSXZ + Jacobi2D, update_ghost_sxz.cpp:86-350; main.cpp:242-246 */
-int ecsg_update_ghost_syz(ecsg_ctx *ctx, int k);      /* UpdateGhostSYZ, update_ghost_syz.cpp:151-240; main.cpp:247-250 */
+int ecsg_update_ghost_sxz(ecsg_ctx *ctx, int k);      /* UpdateGhostSXZ + Jacobi2D, update_ghost_sxz.cpp:86-350 (imode 0: 36-84); main.cpp:242-246 */
+int ecsg_update_ghost_syz(ecsg_ctx *ctx, int k);      /* UpdateGhostSYZ, update_ghost_syz.cpp:151-240 (imode 0: 23-150); main.cpp:247-250 */
 int ecsg_end_step(ecsg_ctx *ctx);                     /* niter++ of main.cpp:165 */
 
 /* Interface residuals (compute_res_*, compute_res.cpp:4-95) with the factor 4

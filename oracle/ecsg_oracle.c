@@ -18,6 +18,9 @@
  *   UpdateGhostSXZ (mode 1), Jacobi2D, Norm2D
  *                                      update_ghost_sxz.cpp:86-297, 314-350, 357-388
  *   UpdateGhostSYZ (mode 1) .......... update_ghost_syz.cpp:151-240
+ *   UpdateGhostVZ/SXZ/SYZ (mode 0, II)  update_ghost_vz.cpp:36-123,
+ *                                      update_ghost_sxz.cpp:36-84,
+ *                                      update_ghost_syz.cpp:23-150
  *   ComputeEnergy / TriMulFields ..... compute_energy.cpp:34-54, field.cpp:482-518
  *   compute_res_* .................... compute_res.cpp:4-95
  *   time loop / phase order .......... main.cpp:165-261
@@ -134,6 +137,7 @@ typedef struct {
 typedef struct {
     int ng, radius, top_bc, imode, eci_mode, check_energy;
     int vy_mirror_all;   /* 1: mirror vy across the y walls on every z plane */
+    int ii_xend;         /* imode 0: form of the x-end vz ghost edge (ghost_vz_ii) */
     real dt;
     grid_t g[MAXG];
     iface_t itf[MAXG];
@@ -213,7 +217,7 @@ void *orc_create(int ng, const int *n, const double *h, int radius, int top_bc, 
         iface_t *I = &s->itf[k];
         int r = F->n[0] / C->n[0];
         if (r < 2 || F->n[0] != r * C->n[0] || F->n[1] != r * C->n[1]) return NULL;
-        if (eci_mode == 0 && r != 2) return NULL;
+        if ((eci_mode == 0 || imode == 0) && r != 2) return NULL;
         I->r = r;
         long nx2 = C->n[0], ny2 = C->n[1];
         I->gx = (real *)calloc((size_t)((nx2 + 1) * (ny2 + 1)), sizeof(real));
@@ -626,6 +630,142 @@ static void update_stress(osim *s, int g)
         for (int i1 = 1; i1 < ny; i1++)
             for (int ir = 1; ir < radius; ir++)
                 for (int i0 = 1; i0 < nx; i0++) AT(sxz, i0, i1, nz - 1 + ir) = AT(sxz, i0, i1, nz - 1 - ir + 1);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Interface, imode 0: the reference's "intuitive" (II) interpolation, */
+/* ratio 2 only, term for term.  Every formula reads the fine planes   */
+/* nz1-2 / nz1-1 and the coarse plane 0 only, so the loop order does   */
+/* not matter.                                                         */
+
+/* UpdateGhostVZ mode 0, update_ghost_vz.cpp:36-123.
+ * The x-end edge (fine x = nx1-1, update_ghost_vz.cpp:98-107) has two forms:
+ *   ii_xend 0: not written.  This is the committed version of that line
+ *              (`int i1 = vz1.GetEnd(0);`, which leaves i0 = 0 and shadows
+ *              i1, so its stores go to y indices beyond the field -- outside
+ *              the allocation for the reference flow's grids -- and are not
+ *              reproduced); the reference's golden II traces were made with
+ *              it and are matched by this form;
+ *   ii_xend 1: the working copy's form (`i0 = vz1.GetEnd(0);`), which reads
+ *              vz2 at x index nx1-1, beyond the coarse row: Field storage is
+ *              one flat array (field.cpp:123-136), so the read lands in a
+ *              following row.  AT() computes the same flat offset (the
+ *              oracle's fields have the reference's lengths and halo). */
+static void ghost_vz_ii(osim *s, int k)
+{
+    grid_t *F = &s->g[k], *C = &s->g[k + 1];
+    fld *vz1 = &F->sim[VZ], *vz2 = &C->sim[VZ];
+    const int nx2 = C->n[0], ny2 = C->n[1], nz1 = F->n[2];
+#pragma omp parallel for schedule(static)
+    for (int i1 = 1; i1 < ny2; i1++)
+        for (int i0 = 0; i0 < nx2; i0++)
+            AT(vz2, i0, i1, -1) = 0.25 * (AT(vz1, 2 * i0, 2 * i1, nz1 - 2) + AT(vz1, 2 * i0 + 1, 2 * i1, nz1 - 2)
+                                          + AT(vz1, 2 * i0, 2 * i1, nz1 - 1) + AT(vz1, 2 * i0 + 1, 2 * i1, nz1 - 1));
+#pragma omp parallel for schedule(static)
+    for (int i1 = 0; i1 < ny2; i1++)
+        for (int i0 = 0; i0 < nx2 - 1; i0++) {
+            const real a = AT(vz2, i0, i1, 0), b = AT(vz2, i0 + 1, i1, 0);
+            const real c = AT(vz2, i0, i1 + 1, 0), d = AT(vz2, i0 + 1, i1 + 1, 0);
+            AT(vz1, 2 * i0 + 1, 2 * i1, nz1) = AT(vz1, 2 * i0 + 1, 2 * i1, nz1 - 1) / 3 + a / 2 + b / 6;
+            AT(vz1, 2 * i0 + 2, 2 * i1, nz1) = AT(vz1, 2 * i0 + 2, 2 * i1, nz1 - 1) / 3 + a / 6 + b / 2;
+            AT(vz1, 2 * i0 + 1, 2 * i1 + 1, nz1) = AT(vz1, 2 * i0 + 1, 2 * i1 + 1, nz1 - 1) / 3 + a / 4 + b / 12 + c / 4 + d / 12;
+            AT(vz1, 2 * i0 + 2, 2 * i1 + 1, nz1) = AT(vz1, 2 * i0 + 2, 2 * i1 + 1, nz1 - 1) / 3 + a / 12 + b / 4 + c / 12 + d / 4;
+        }
+    /* extrapolated x edges (option 1 of the reference) */
+    const int e0 = fld_end(vz1, 0);
+#pragma omp parallel for schedule(static)
+    for (int i1 = 0; i1 < ny2; i1++) {
+        int i0 = 0;
+        AT(vz1, i0, 2 * i1, nz1) = AT(vz1, i0, 2 * i1, nz1 - 1) * 2 / 3 - AT(vz1, i0 + 1, 2 * i1, nz1 - 1) / 3
+                                   + AT(vz2, i0, i1, 0) * 2 / 3;
+        AT(vz1, i0, 2 * i1 + 1, nz1) = AT(vz1, i0, 2 * i1 + 1, nz1 - 1) * 2 / 3 - AT(vz1, i0 + 1, 2 * i1, nz1 - 1) / 3
+                                       + AT(vz2, i0, i1, 0) / 3 + AT(vz2, i0, i1 + 1, 0) / 3;
+        if (!s->ii_xend) continue;
+        i0 = e0;  /* a fine index, used on vz2 as the reference does */
+        AT(vz1, i0, 2 * i1, nz1) = AT(vz1, i0, 2 * i1, nz1 - 1) * 2 / 3 - AT(vz1, i0 - 1, 2 * i1, nz1 - 1) / 3
+                                   + AT(vz2, i0, i1, 0) * 2 / 3;
+        AT(vz1, i0, 2 * i1 + 1, nz1) = AT(vz1, i0, 2 * i1 + 1, nz1 - 1) * 2 / 3 - AT(vz1, i0 - 1, 2 * i1, nz1 - 1) / 3
+                                       + AT(vz2, i0, i1, 0) / 3 + AT(vz2, i0, i1 + 1, 0) / 3;
+    }
+}
+
+/* UpdateGhostSXZ mode 0, update_ghost_sxz.cpp:36-84 */
+static void ghost_sxz_ii(osim *s, int k)
+{
+    grid_t *F = &s->g[k], *C = &s->g[k + 1];
+    fld *sxz1 = &F->sim[SXZ], *sxz2 = &C->sim[SXZ];
+    const int nx2 = C->n[0], ny2 = C->n[1], nz1 = F->n[2];
+#pragma omp parallel for schedule(static)
+    for (int i1 = 1; i1 < ny2; i1++)
+        for (int i0 = 1; i0 < nx2; i0++)
+            AT(sxz2, i0, i1, -1) = 0.5 * (AT(sxz1, 2 * i0, 2 * i1, nz1 - 2) + AT(sxz1, 2 * i0, 2 * i1, nz1 - 1));
+#pragma omp parallel for schedule(static)
+    for (int i1 = 0; i1 <= ny2; i1++)
+        for (int i0 = 0; i0 <= nx2; i0++) {
+            if (i0 >= 1 && i0 < nx2 && i1 >= 1 && i1 < ny2)
+                AT(sxz1, 2 * i0, 2 * i1, nz1) = AT(sxz1, 2 * i0, 2 * i1, nz1 - 1) / 3 + AT(sxz2, i0, i1, 0) * 2 / 3.0;
+            if (i0 < nx2 && i1 >= 1 && i1 < ny2)
+                AT(sxz1, 2 * i0 + 1, 2 * i1, nz1) = AT(sxz1, 2 * i0 + 1, 2 * i1, nz1 - 1) / 3 + AT(sxz2, i0, i1, 0) / 3
+                                                    + AT(sxz2, i0 + 1, i1, 0) / 3;
+            if (i0 >= 1 && i0 < nx2 && i1 < ny2)
+                AT(sxz1, 2 * i0, 2 * i1 + 1, nz1) = AT(sxz1, 2 * i0, 2 * i1 + 1, nz1 - 1) / 3 + AT(sxz2, i0, i1, 0) / 3
+                                                    + AT(sxz2, i0, i1 + 1, 0) / 3;
+            if (i0 < nx2 && i1 < ny2)
+                AT(sxz1, 2 * i0 + 1, 2 * i1 + 1, nz1) = AT(sxz1, 2 * i0 + 1, 2 * i1 + 1, nz1 - 1) / 3
+                                                        + AT(sxz2, i0, i1, 0) / 6 + AT(sxz2, i0, i1 + 1, 0) / 6
+                                                        + AT(sxz2, i0 + 1, i1, 0) / 6 + AT(sxz2, i0 + 1, i1 + 1, 0) / 6;
+        }
+}
+
+/* UpdateGhostSYZ mode 0, update_ghost_syz.cpp:23-150 */
+static void ghost_syz_ii(osim *s, int k)
+{
+    grid_t *F = &s->g[k], *C = &s->g[k + 1];
+    fld *syz1 = &F->sim[SYZ], *syz2 = &C->sim[SYZ];
+    const int nx2 = C->n[0], ny2 = C->n[1], nz1 = F->n[2];
+#pragma omp parallel for schedule(static)
+    for (int i1 = 0; i1 <= fld_end(syz2, 1); i1++)
+        for (int i0 = 0; i0 <= fld_end(syz2, 0); i0++)
+            AT(syz2, i0, i1, -1) = 0.125 * (AT(syz1, 2 * i0, 2 * i1, nz1 - 2) + AT(syz1, 2 * i0 + 1, 2 * i1, nz1 - 2)
+                                            + AT(syz1, 2 * i0, 2 * i1 + 1, nz1 - 2) + AT(syz1, 2 * i0 + 1, 2 * i1 + 1, nz1 - 2)
+                                            + AT(syz1, 2 * i0, 2 * i1, nz1 - 1) + AT(syz1, 2 * i0 + 1, 2 * i1, nz1 - 1)
+                                            + AT(syz1, 2 * i0, 2 * i1 + 1, nz1 - 1) + AT(syz1, 2 * i0 + 1, 2 * i1 + 1, nz1 - 1));
+#pragma omp parallel for schedule(static)
+    for (int i1 = 0; i1 < ny2 - 1; i1++)
+        for (int i0 = 0; i0 < nx2 - 1; i0++) {
+            const real a = AT(syz2, i0, i1, 0), b = AT(syz2, i0 + 1, i1, 0);
+            const real c = AT(syz2, i0, i1 + 1, 0), d = AT(syz2, i0 + 1, i1 + 1, 0);
+            AT(syz1, 2 * i0 + 1, 2 * i1 + 1, nz1) = AT(syz1, 2 * i0 + 1, 2 * i1 + 1, nz1 - 1) / 3 + 0.375 * a + 0.125 * b + 0.125 * c + d / 24;
+            AT(syz1, 2 * i0 + 2, 2 * i1 + 1, nz1) = AT(syz1, 2 * i0 + 2, 2 * i1 + 1, nz1 - 1) / 3 + 0.125 * a + 0.375 * b + c / 24 + 0.125 * d;
+            AT(syz1, 2 * i0 + 1, 2 * i1 + 2, nz1) = AT(syz1, 2 * i0 + 1, 2 * i1 + 2, nz1 - 1) / 3 + 0.125 * a + b / 24 + 0.375 * c + 0.125 * d;
+            AT(syz1, 2 * i0 + 2, 2 * i1 + 2, nz1) = AT(syz1, 2 * i0 + 2, 2 * i1 + 2, nz1 - 1) / 3 + a / 24 + 0.125 * b + 0.125 * c + 0.375 * d;
+        }
+    /* extrapolated edges (option 1 of the reference) */
+    const int xe = fld_end(syz1, 0), ye = fld_end(syz1, 1);
+#pragma omp parallel for schedule(static)
+    for (int i0 = 0; i0 <= xe; i0++) {
+        for (int side = 0; side < 2; side++) {
+            const int j = side ? ye : 0, jn = side ? ye - 1 : 1;
+            real v;
+            if (i0 == 0)
+                v = AT(syz1, i0, j, nz1 - 1) - AT(syz1, i0 + 1, j, nz1 - 1) / 3 - AT(syz1, i0, jn, nz1 - 1) / 3
+                    + AT(syz2, i0 / 2, j / 2, 0) * 2 / 3;
+            else if (i0 == xe)
+                v = AT(syz1, i0, j, nz1 - 1) - AT(syz1, i0 - 1, j, nz1 - 1) / 3 - AT(syz1, i0, jn, nz1 - 1) / 3
+                    + AT(syz2, i0 / 2, j / 2, 0) * 2 / 3;
+            else
+                v = AT(syz1, i0, j, nz1 - 1) / 3 + AT(syz1, i0 - 1 + 2 * (i0 % 2), j, nz1 - 1) / 3
+                    - AT(syz1, i0, jn, nz1 - 1) / 3 + AT(syz2, i0 / 2, j / 2, 0) * 2 / 3;
+            AT(syz1, i0, j, nz1) = v;
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int i1 = 1; i1 < ye; i1++) {
+        AT(syz1, 0, i1, nz1) = AT(syz1, 0, i1, nz1 - 1) / 3 + AT(syz1, 0, i1 - 1 + 2 * (i1 % 2), nz1 - 1) / 3
+                               - AT(syz1, 1, i1, nz1 - 1) / 3 + AT(syz2, 0, i1 / 2, 0) * 2 / 3;
+        AT(syz1, xe, i1, nz1) = AT(syz1, xe, i1, nz1 - 1) / 3 + AT(syz1, xe, i1 - 1 + 2 * (i1 % 2), nz1 - 1) / 3
+                                - AT(syz1, xe - 1, i1, nz1 - 1) / 3 + AT(syz2, xe / 2, i1 / 2, 0) * 2 / 3;
     }
 }
 
@@ -1217,19 +1357,25 @@ int orc_update_stress(void *h, int g) { update_stress((osim *)h, g); return 0; }
 int orc_ghost_vz(void *h, int k)
 {
     osim *s = (osim *)h;
-    if (s->eci_mode == 0) ghost_vz_ref(s, k); else ghost_vz_gen(s, k);
+    if (s->imode == 0) ghost_vz_ii(s, k);
+    else if (s->eci_mode == 0) ghost_vz_ref(s, k);
+    else ghost_vz_gen(s, k);
     return 0;
 }
 int orc_ghost_sxz(void *h, int k)
 {
     osim *s = (osim *)h;
-    if (s->eci_mode == 0) ghost_sxz_ref(s, k); else ghost_sxz_gen(s, k);
+    if (s->imode == 0) ghost_sxz_ii(s, k);
+    else if (s->eci_mode == 0) ghost_sxz_ref(s, k);
+    else ghost_sxz_gen(s, k);
     return 0;
 }
 int orc_ghost_syz(void *h, int k)
 {
     osim *s = (osim *)h;
-    if (s->eci_mode == 0) ghost_syz_ref(s, k); else ghost_syz_gen(s, k);
+    if (s->imode == 0) ghost_syz_ii(s, k);
+    else if (s->eci_mode == 0) ghost_syz_ref(s, k);
+    else ghost_syz_gen(s, k);
     return 0;
 }
 int orc_residuals(void *h, int which, double *out) { residuals((osim *)h, which, out); return 0; }
@@ -1303,6 +1449,7 @@ int orc_run(void *h, int nsteps)
 }
 
 int orc_set_vy_mirror_all(void *h, int flag) { ((osim *)h)->vy_mirror_all = flag ? 1 : 0; return 0; }
+int orc_set_ii_xend(void *h, int flag) { ((osim *)h)->ii_xend = flag ? 1 : 0; return 0; }
 
 int orc_steps_done(void *h) { return ((osim *)h)->steps_done; }
 int orc_set_steps_done(void *h, int n) { ((osim *)h)->steps_done = n; return 0; }

@@ -54,6 +54,7 @@ def _load(prec: str):
     lib.orc_get_sxz_system.argtypes = [vp, i, P(C.c_double), P(C.c_double)]
     lib.orc_fd_coeff.argtypes = [i, P(C.c_double)]
     lib.orc_set_vy_mirror_all.argtypes = [vp, i]
+    lib.orc_set_ii_xend.argtypes = [vp, i]
     return lib
 
 
@@ -77,7 +78,7 @@ class Oracle:
     """
 
     def __init__(self, model, prec: str = "f64", eci_mode: int = None, check_energy: bool = True,
-                 max_steps: int = None, vy_mirror_all: bool = False):
+                 max_steps: int = None, vy_mirror_all: bool = False, ii_xend: bool = False):
         self.L = lib(prec)
         self.model = model
         ratios = model.ratios()
@@ -94,6 +95,7 @@ class Oracle:
             raise RuntimeError("orc_create failed (sizes/ratio)")
         self.check_energy = check_energy
         self.L.orc_set_vy_mirror_all(self.h, int(vy_mirror_all))
+        self.L.orc_set_ii_xend(self.h, int(ii_xend))
         for g in range(ng):
             self.set_params(g)
         if model.src_trace is not None and len(model.src_trace):

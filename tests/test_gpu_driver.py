@@ -132,3 +132,20 @@ def test_driver_reference_flow_golden(tmp_path):
     for j in (0, 3, 6):
         scale = (res[j + 1] * res[j + 2]).max() * m.grids[1].nodes
         assert np.abs(res[j]).max() <= 1e-9 * scale, j
+
+
+@pytest.mark.timeout(600)
+def test_driver_reference_flow_ii_golden(tmp_path):
+    """The II flow (tests/test1-cmp-eci-ii/ii/SConstruct:173-184: the same
+    command line with imode=0) through the driver: rec_out against the golden
+    II traces."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    gold = np.load(os.path.join(here, "golden", "reference_traces.npz"))
+    m = reference_flow(1, eci=False)
+    args = write_flow_inputs(m, str(tmp_path))
+    assert "imode=0" in args
+    _run(args + ["num_threads=16", "check_energy=y"], str(tmp_path), timeout=600)
+    kv, tr = read_rsf(str(tmp_path / "rec_out.rsf"))
+    err = rel(tr.astype(np.float64), gold["rec_ii_test1"])
+    print("driver golden II test1: relL2 vs II %.3e" % err)
+    assert err <= 1e-4

@@ -75,6 +75,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_layout_matches_header():
-    # ecsg_config: int, int[8][3], (pad), double[8][3], double, 8 ints
+    # ecsg_config: int, int[8][3], (pad), double[8][3], double, 9 ints, (pad to 8)
     import ctypes as C
-    assert C.sizeof(ecsg_amd.EcsgConfig) == 4 + 96 + 4 + 192 + 8 + 8 * 4
+    assert C.sizeof(ecsg_amd.EcsgConfig) == 4 + 96 + 4 + 192 + 8 + 9 * 4 + 4
+    assert ecsg_amd.EcsgConfig.ii_vz_xend.offset == 4 + 96 + 4 + 192 + 8 + 8 * 4

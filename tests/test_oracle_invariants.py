@@ -221,3 +221,86 @@ def test_fp32_build_tracks_fp64():
     a.run(m.nsteps)
     b.run(m.nsteps)
     assert rel(b.traces(), a.traces()) <= 1e-4
+
+
+@pytest.mark.parametrize("xend", [False, True])
+def test_ii_vz_x_end_edge(xend):
+    """imode 0, the x-end edge of the fine vz ghosts (update_ghost_vz.cpp:
+    98-107).  ii_xend 0 leaves it untouched.  ii_xend 1 reads vz2 at the fine
+    index nx1-1: in the reference's flat Field storage (x rows of nx2+2
+    values, halo 1, field.cpp:85-98,123-136) that is x = nx1-1-(nx2+2) of the
+    next y row -- checked here from the layout, not from the oracle's own
+    indexing."""
+    m = composite((2,), coarse=(6, 4, 4), nz_fine=(4,), iso=True, nsteps=2, seed=5)
+    m.imode = 0
+    o = Oracle(m, eci_mode=0, check_energy=False, ii_xend=xend)
+    rng = np.random.default_rng(7)
+    for g in range(2):
+        o.set(g, 2, rng.standard_normal(o.dims(g, 2)[0]))
+    before = o.get(0, 2)
+    vz2 = o.get(1, 2)                       # [z+1, y+1, x+1]
+    o.ghost_vz(0)
+    after = o.get(0, 2)
+    nx2, ny2 = m.grids[1].n[:2]
+    nz1 = m.grids[0].n[2]
+    xe = 2 * nx2 - 1
+    assert vz2.shape[2] == nx2 + 2          # the coarse vz row length
+    for i1 in range(ny2):
+        for row in (2 * i1, 2 * i1 + 1):
+            got = after[nz1 + 1, row + 1, xe + 1]
+            if not xend:
+                assert got == before[nz1 + 1, row + 1, xe + 1]
+                continue
+            xa, ya = xe - (nx2 + 2), i1 + 1    # flat-storage position of vz2(xe, i1, 0)
+            a = vz2[1, ya + 1, xa + 1]
+            e0 = before[nz1, 2 * i1 + 1, xe]   # vz1(xe-1, 2*i1, nz1-1)
+            x = before[nz1, row + 1, xe + 1]   # vz1(xe, row, nz1-1)
+            if row == 2 * i1:
+                want = x * 2 / 3 - e0 / 3 + a * 2 / 3
+            else:
+                want = x * 2 / 3 - e0 / 3 + a / 3 + vz2[1, ya + 2, xa + 1] / 3
+            assert got == want
+    # every other point of the fine ghost plane is written by the x-end-free formulas
+    o.close()
+
+
+def test_ii_ghosts_are_second_order_interpolants():
+    """A field linear in x, y and z is reproduced exactly (to rounding) by the
+    II interior formulas: coarse ghosts average fine values at the ghost
+    position, fine ghosts interpolate fine plane nz1-1 and coarse plane 0."""
+    m = composite((2,), coarse=(6, 5, 4), nz_fine=(4,), iso=True, nsteps=2, seed=6)
+    m.imode = 0
+    o = Oracle(m, eci_mode=0, check_energy=False)
+    hf, hc = m.grids[0].h[0], m.grids[1].h[0]
+    nz1 = m.grids[0].n[2]
+    from ecsg_amd.model import FIELD_POS
+
+    def lin(x, y, z):
+        return 1.0 + 0.3 * x - 0.2 * y + 0.7 * z
+
+    for c in (2, 6, 7):
+        pos = FIELD_POS[c]
+        for g, h in ((0, hf), (1, hc)):
+            shape = o.dims(g, c)[0]
+            z0 = 0.0 if g == 0 else nz1 * hf
+            k, j, i = np.meshgrid(*[np.arange(n) - 1 for n in shape], indexing="ij")
+            o.set(g, c, lin((i + 0.5 * pos[0]) * h, (j + 0.5 * pos[1]) * h, z0 + (k + 0.5 * pos[2]) * h))
+    o.ghost_vz(0)
+    o.ghost_sxz(0)
+    o.ghost_syz(0)
+    for c, (cx0, cx1, cy0, cy1) in ((2, (1, -1, 0, -1)), (6, (1, -1, 1, -1)), (7, (2, -2, 2, -2))):
+        pos = FIELD_POS[c]
+        f = o.get(0, c)
+        k = nz1 + 1  # plane nz1 (halo offset 1)
+        ny, nx = f.shape[1], f.shape[2]
+        j, i = np.meshgrid(np.arange(ny) - 1, np.arange(nx) - 1, indexing="ij")
+        want = lin((i + 0.5 * pos[0]) * hf, (j + 0.5 * pos[1]) * hf, (nz1 + 0.5 * pos[2]) * hf)
+        sl = (slice(1 + cy0, ny + cy1 - 1), slice(1 + cx0, nx + cx1 - 1))
+        assert np.abs(f[k][sl] - want[sl]).max() <= 1e-12, c
+        g2 = o.get(1, c)
+        ny2, nx2 = g2.shape[1], g2.shape[2]
+        j2, i2 = np.meshgrid(np.arange(ny2) - 1, np.arange(nx2) - 1, indexing="ij")
+        want2 = lin((i2 + 0.5 * pos[0]) * hc, (j2 + 0.5 * pos[1]) * hc, nz1 * hf + (-1 + 0.5 * pos[2]) * hc)
+        sl2 = (slice(2, ny2 - 2), slice(2, nx2 - 2))
+        assert np.abs(g2[0][sl2] - want2[sl2]).max() <= 1e-12, c
+    o.close()
