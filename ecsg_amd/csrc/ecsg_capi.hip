@@ -7,6 +7,7 @@
 #include "ecsg_batch.hpp"
 #include "ecsg_fused.hpp"
 #include "ecsg_ii.hpp"
+#include "ecsg_radius.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -108,6 +109,10 @@ struct DevGrid {
     // buffer of the fused step; the fields of the current state are set `cur`
     int cur = 0;
     T *alt = nullptr;
+    // radius > 1 (single grid): coefficients, mirror loops, stencil workgroups
+    RCoef<T> rco{};
+    MirrorSet mva{}, mvb{}, mst{};
+    int rblk = 0;
     T *fset(int k) const { return k ? alt : base; }
     T *fields() const { return fset(cur); }
     T *arr(int a) const { return a < 9 ? fields() + (long)a * astride : base + (long)a * astride; }
@@ -199,7 +204,9 @@ public:
     {
         cfg = c;
         if (c.num_grids < 1 || c.num_grids > ECSG_MAX_GRIDS) return fail(ECSG_EINVAL, "num_grids out of range");
-        if (c.radius != 1) return fail(ECSG_EINVAL, "only radius 1 is implemented on the device");
+        if (c.radius < 1 || c.radius > RMAX) return fail(ECSG_EINVAL, "radius must be 1..6");
+        if (c.radius > 1 && c.num_grids != 1)
+            return fail(ECSG_EINVAL, "radius > 1 runs on a single grid (the interface formulas are radius 1)");
         if (c.imode != 0 && c.imode != 1) return fail(ECSG_EINVAL, "imode must be 0 (II) or 1 (ECI)");
         if (c.top_bc != 0 && c.top_bc != 1) return fail(ECSG_EINVAL, "top_bc must be 0 or 1");
         if (!(c.dt > 0)) return fail(ECSG_EINVAL, "dt must be positive");
@@ -217,15 +224,16 @@ public:
             for (int k = 0; k < 3; k++) {
                 d.n[k] = c.n[g][k];
                 d.h[k] = c.h[g][k];
-                if (d.n[k] < 2) return fail(ECSG_EINVAL, "each grid needs at least 2 cells per dimension");
+                if (d.n[k] < 2 || d.n[k] < c.radius) return fail(ECSG_EINVAL, "each grid needs at least max(2, radius) cells per dimension");
                 if (!(d.h[k] > 0)) return fail(ECSG_EINVAL, "grid steps must be positive");
             }
             const long line = 128 / (long)sizeof(T);
-            const long xoff = line;  // ix = 0 starts a 128-byte line
-            d.P = ((xoff + d.n[0] + 2) + line - 1) / line * line;
-            d.S2 = d.P * (d.n[1] + 3);
-            d.o = d.S2 + d.P + xoff;
-            d.astride = ((d.S2 * (d.n[2] + 3)) + line - 1) / line * line;
+            const long xoff = line;  // ix = 0 starts a 128-byte line (line >= radius)
+            const long H = c.radius; // halo: indices [-H, n+H] in every dimension
+            d.P = ((xoff + d.n[0] + 1 + H) + line - 1) / line * line;
+            d.S2 = d.P * (d.n[1] + 1 + 2 * H);
+            d.o = H * d.S2 + H * d.P + xoff;
+            d.astride = ((d.S2 * (d.n[2] + 1 + 2 * H)) + line - 1) / line * line;
             if (int rc = dalloc(&d.base, (size_t)d.astride * 19)) return rc;
             GridDev<T> &v = d.dev;
             for (int a = 0; a < 9; a++) v.f[a] = d.arr(a);
@@ -245,6 +253,10 @@ public:
             // vy y-wall mirror planes: [0, nz-1] as the reference loops, or all
             v.zy = c.vy_mirror_all ? d.n[2] : d.n[2] - 1;
             set_tiling(d, tile_by, tile_zc);
+            if (c.radius > 1) {
+                setup_radius(d, c, top, bottom);
+                v.nblk = d.rblk;  // energy partials are per radius-R workgroup
+            }
             // energy partials sized for the largest tiling (BY = 1, ZC = 1)
             const size_t maxblk = (size_t)((d.n[0] + BX) / BX) * (d.n[1] + 1) * (d.n[2] + 1);
             if (int rc = dalloc(&d.ke, maxblk)) return rc;
@@ -324,6 +336,125 @@ public:
         if (int rc = dalloc(&d_rec, 4)) return rc;
         if (int rc = dalloc(&d_traces, 1)) return rc;
         return ECSG_OK;
+    }
+
+    // ----- radius > 1 (single grid) ----------------------------------------
+    static void add_loop(MirrorSet &m, int f, int axis, int dbase, int dstep, int sbase, int sstep, int a0, int a1,
+                         int b0, int b1, int nir, int sign)
+    {
+        if (nir < 1 || a1 < a0 || b1 < b0) return;
+        MirrorLoop &L = m.m[m.n];
+        L.f = f; L.axis = axis; L.dbase = dbase; L.dstep = dstep; L.sbase = sbase; L.sstep = sstep;
+        L.a0 = a0; L.a1 = a1; L.b0 = b0; L.b1 = b1; L.nir = nir; L.sign = sign;
+        m.start[m.n + 1] = m.start[m.n] + (long)(a1 - a0 + 1) * (b1 - b0 + 1) * nir;
+        m.n++;
+    }
+
+    // Coefficients and the ghost-mirror loops of update_vel.cpp:73-246 and
+    // update_stress.cpp:135-309, transcribed with their index ranges.  Axis
+    // pairs (a, b): mirrored x -> (y, z), y -> (x, z), z -> (x, y).
+    static void setup_radius(DevGrid<T> &d, const ecsg_config &c, int top, int bottom)
+    {
+        const int R = c.radius, nx = d.n[0], ny = d.n[1], nz = d.n[2];
+        double fd[RMAX];
+        fd_table(R, fd);
+        for (int k = 0; k < 3; k++)
+            for (int ir = 0; ir < R; ir++) d.rco.c[k][ir] = (T)(fd[ir] * c.dt / d.h[k]);
+        const int vtop = (top && c.top_bc != 1) ? 1 : 0;  // update_vel.cpp:17-18
+        const int zy = c.vy_mirror_all ? nz : nz - 1;
+        MirrorSet &A = d.mva, &B = d.mvb, &S = d.mst;
+        A = MirrorSet{}; B = MirrorSet{}; S = MirrorSet{};
+        // velocities, z walls (update_vel.cpp:75-131) then x walls (:135-189)
+        if (vtop) {
+            add_loop(A, 2, 2, 0, -1, -1, 1, 0, nx - 1, 0, ny, R - c.top_bc, -1);
+            add_loop(A, 0, 2, 0, -1, 0, 1, 0, nx - 1, 0, ny, R - 1, -1);
+            add_loop(A, 1, 2, 0, -1, 0, 1, 0, nx - 1, 0, ny - 1, R - 1, -1);
+        }
+        if (bottom) {
+            add_loop(A, 2, 2, nz - 1, 1, nz, -1, 0, nx - 1, 0, ny, R, -1);
+            add_loop(A, 0, 2, nz, 1, nz, -1, 0, nx - 1, 0, ny, R - 1, -1);
+            add_loop(A, 1, 2, nz, 1, nz, -1, 0, nx - 1, 0, ny - 1, R - 1, -1);
+        }
+        add_loop(A, 2, 0, 0, -1, -1, 1, 0, ny, 0, nz - 1, R, -1);
+        add_loop(A, 0, 0, 0, -1, 0, 1, 0, ny, 0, nz, R - 1, -1);
+        add_loop(A, 1, 0, 0, -1, -1, 1, 0, ny, 0, nz, R, -1);
+        add_loop(A, 2, 0, nx - 1, 1, nx, -1, 0, ny, 0, nz - 1, R, -1);
+        add_loop(A, 0, 0, nx, 1, nx, -1, 0, ny, 0, nz, R - 1, -1);
+        add_loop(A, 1, 0, nx - 1, 1, nx, -1, 0, ny, 0, nz, R, -1);
+        // velocities, y walls (update_vel.cpp:191-246)
+        add_loop(B, 2, 1, 0, -1, 0, 1, 0, nx - 1, 0, nz - 1, R - 1, -1);
+        add_loop(B, 0, 1, 0, -1, 0, 1, 0, nx, 0, nz - 1, R - 1, -1);
+        add_loop(B, 1, 1, 0, -1, -1, 1, 0, nx - 1, 0, zy, R, -1);
+        add_loop(B, 2, 1, ny, 1, ny, -1, 0, nx - 1, 0, nz - 1, R - 1, -1);
+        add_loop(B, 0, 1, ny, 1, ny, -1, 0, nx, 0, nz - 1, R - 1, -1);
+        add_loop(B, 1, 1, ny - 1, 1, ny, -1, 0, nx - 1, 0, zy, R, -1);
+        // stresses (update_stress.cpp:135-309)
+        const int flag = (top && c.top_bc) ? 1 : 0;
+        add_loop(S, 3, 0, 0, -1, -1, 1, 0, ny, 0, nz, R - 1, 1);
+        add_loop(S, 7, 0, 0, -1, 0, 1, 0, ny, 0, nz - 1, R - 1, 1);
+        add_loop(S, 8, 0, 0, -1, 0, 1, 0, ny - 1, 0, nz, R - 1, 1);
+        add_loop(S, 3, 0, nx - 1, 1, nx, -1, 0, ny, 0, nz, R - 1, 1);
+        add_loop(S, 7, 0, nx, 1, nx, -1, 0, ny, 0, nz - 1, R - 1, 1);
+        add_loop(S, 8, 0, nx, 1, nx, -1, 0, ny - 1, 0, nz, R - 1, 1);
+        add_loop(S, 4, 1, 0, -1, 0, 1, 0, nx - 1, 0, nz, R - 1, 1);
+        add_loop(S, 6, 1, 0, -1, -1, 1, 0, nx - 1, 0, nz - 1, R - 1, 1);
+        add_loop(S, 8, 1, 0, -1, -1, 1, 0, nx, 0, nz, R - 1, 1);
+        add_loop(S, 4, 1, ny, 1, ny, -1, 0, nx - 1, 0, nz, R - 1, 1);
+        add_loop(S, 6, 1, ny - 1, 1, ny, -1, 0, nx - 1, 0, nz - 1, R - 1, 1);
+        add_loop(S, 8, 1, ny - 1, 1, ny, -1, 0, nx, 0, nz, R - 1, 1);
+        if (top) {
+            add_loop(S, 5, 2, 0, -1, 0, 1, 0, nx - 1, 1, ny - 1, R - 1, 1);
+            add_loop(S, 6, 2, 0, -1, -1, 1, 0, nx - 1, 0, ny - 1, R - 1 + flag, flag ? -1 : 1);
+            add_loop(S, 7, 2, 0, -1, -1, 1, 1, nx - 1, 1, ny - 1, R - 1 + flag, flag ? -1 : 1);
+        }
+        if (bottom) {
+            add_loop(S, 5, 2, nz, 1, nz, -1, 0, nx - 1, 1, ny - 1, R - 1, 1);
+            add_loop(S, 6, 2, nz - 1, 1, nz, -1, 0, nx - 1, 0, ny - 1, R - 1, 1);
+            add_loop(S, 7, 2, nz - 1, 1, nz, -1, 1, nx - 1, 1, ny - 1, R - 1, 1);
+        }
+        d.rblk = ((nx + BX) / BX) * ((ny + RBY) / RBY) * ((nz + RZC) / RZC);
+    }
+
+    void mirror_launch(DevGrid<T> &d, const MirrorSet &m)
+    {
+        const long n = m.start[m.n];
+        if (n > 0) hipLaunchKernelGGL((k_mirror<T>), dim3((n + 255) / 256), dim3(256), 0, s, d.dev, m);
+    }
+
+    template <int R>
+    void vel_r(DevGrid<T> &d, bool en, const int *srcp)
+    {
+        const dim3 blk(BX, RBY);
+        if (en) hipLaunchKernelGGL((k_vel_r<T, R, true>), dim3(d.rblk), blk, 0, s, d.dev, d.rco, d.ke, srcp, d_src_old);
+        else hipLaunchKernelGGL((k_vel_r<T, R, false>), dim3(d.rblk), blk, 0, s, d.dev, d.rco, d.ke, srcp, d_src_old);
+    }
+    template <int R>
+    void stress_r(DevGrid<T> &d, bool en)
+    {
+        const dim3 blk(BX, RBY);
+        if (en) hipLaunchKernelGGL((k_stress_r<T, R, true>), dim3(d.rblk), blk, 0, s, d.dev, d.rco, d.se);
+        else hipLaunchKernelGGL((k_stress_r<T, R, false>), dim3(d.rblk), blk, 0, s, d.dev, d.rco, d.se);
+    }
+    // stencil updates only (no mirrors), radius > 1
+    void vel_radius(DevGrid<T> &d, bool en, const int *srcp)
+    {
+        switch (cfg.radius) {
+        case 2: vel_r<2>(d, en, srcp); break;
+        case 3: vel_r<3>(d, en, srcp); break;
+        case 4: vel_r<4>(d, en, srcp); break;
+        case 5: vel_r<5>(d, en, srcp); break;
+        default: vel_r<6>(d, en, srcp); break;
+        }
+    }
+    void stress_radius(DevGrid<T> &d, bool en)
+    {
+        switch (cfg.radius) {
+        case 2: stress_r<2>(d, en); break;
+        case 3: stress_r<3>(d, en); break;
+        case 4: stress_r<4>(d, en); break;
+        case 5: stress_r<5>(d, en); break;
+        default: stress_r<6>(d, en); break;
+        }
     }
 
     // stencil tiling: BX x by threads per workgroup, zc planes per thread
@@ -582,6 +713,7 @@ public:
     int set_fused(int on) override
     {
         if (on && cfg.imode == 0 && !I.empty()) return fail(ECSG_EINVAL, "the fused pass runs the energy-conserving interface (imode 1) only");
+        if (on && cfg.radius != 1) return fail(ECSG_EINVAL, "the fused pass is radius 1");
         HIPCHK(hipStreamSynchronize(s));
         if (on)
             for (auto &d : G)
@@ -803,6 +935,14 @@ public:
         const bool en = cfg.check_energy != 0;
         const int *srcp = (en && src_type == 0 && g == src_g && src_len > 0) ? d_src_loc : nullptr;
         ev_begin();
+        if (cfg.radius > 1) {
+            vel_radius(d, en, srcp);
+            mirror_launch(d, d.mva);  // z and x walls
+            mirror_launch(d, d.mvb);  // y walls, after the x mirrors read the old y-halo row
+            ev_end(g == 0 ? 0 : 2);
+            HIPCHK(hipGetLastError());
+            return ECSG_OK;
+        }
         switch (d.by) {
         case 1: vel_launch<1>(d, en, srcp); break;
         case 2: vel_launch<2>(d, en, srcp); break;
@@ -818,6 +958,13 @@ public:
     {
         DevGrid<T> &d = G[g];
         ev_begin();
+        if (cfg.radius > 1) {
+            stress_radius(d, cfg.check_energy != 0);
+            mirror_launch(d, d.mst);
+            ev_end(g == 0 ? 1 : 3);
+            HIPCHK(hipGetLastError());
+            return ECSG_OK;
+        }
         switch (d.by) {
         case 1: stress_launch<1>(d, cfg.check_energy != 0); break;
         case 2: stress_launch<2>(d, cfg.check_energy != 0); break;
@@ -1159,7 +1306,8 @@ public:
             DevGrid<T> &d = G[g];
             T *save = e_save;
             HIPCHK(hipMemcpyAsync(save, d.arr(3), sizeof(T) * d.astride * 6, hipMemcpyDeviceToDevice, s));
-            switch (d.by) {
+            if (cfg.radius > 1) stress_radius(d, true);
+            else switch (d.by) {
             case 1: stress_launch<1>(d, true); break;
             case 2: stress_launch<2>(d, true); break;
             case 8: stress_launch<8>(d, true); break;
@@ -1289,7 +1437,7 @@ public:
         const DevGrid<T> &d = G[g];
         static const int p0[3] = {0, 0, 0};
         const int *pos = sel < 9 ? SIM_POS[sel] : (sel == 9 ? p0 : ST_POS[sel - 10]);
-        const int r = sel < 9 ? 1 : 0;
+        const int r = sel < 9 ? cfg.radius : 0;
         for (int k = 0; k < 3; k++) len[k] = pos[k] ? d.n[k] + 2 * r : d.n[k] + 1 + 2 * r;
         return r;
     }
@@ -1599,6 +1747,7 @@ public:
         if (adj_ready) return ECSG_OK;
         if (int rc = check_ready()) return rc;
         if (cfg.imode == 0 && !I.empty()) return fail(ECSG_EINVAL, "the adjoint is implemented for the energy-conserving interface (imode 1)");
+        if (cfg.radius != 1) return fail(ECSG_EINVAL, "the adjoint is implemented for radius 1");
         adj_free();
         AG.resize(G.size());
         AI.resize(I.size());
@@ -2131,6 +2280,7 @@ public:
     {
         if (int rc = check_ready()) return rc;
         if (reps < 1) return fail(ECSG_EINVAL, "reps must be >= 1");
+        if (cfg.radius != 1) return fail(ECSG_EINVAL, "the tiling autotuner is for the radius-1 stencils");
         HIPCHK(hipStreamSynchronize(s));
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
@@ -2256,6 +2406,7 @@ public:
             if (e->cfg.check_energy) return fail(ECSG_EINVAL, id + "batched models run without the energy check");
             if (e->cfg.sxz_solver != 0) return fail(ECSG_EINVAL, id + "batched models use the Jacobi sweeps (sxz_solver 0)");
             if (e->cfg.imode != 1 && !e->I.empty()) return fail(ECSG_EINVAL, id + "batched composite models use the energy-conserving interface (imode 1)");
+            if (e->cfg.radius != 1) return fail(ECSG_EINVAL, id + "batched models are radius 1");
             if (!e->I.empty() && !e->lds_ok) return fail(ECSG_EINVAL, id + "interface too large for the LDS Jacobi of a batch");
         }
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));

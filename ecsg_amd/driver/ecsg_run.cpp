@@ -3,7 +3,8 @@
 // libecsg_hip.so.
 //
 // Command line (main.cpp:31-145): key=value pairs, booleans y/n.
-//   num_threads (accepted, unused), check_energy, radius (1), dt [ms],
+//   num_threads (accepted, unused), check_energy, radius (1..6; above 1 on a
+//   single grid), dt [ms],
 //   imode (1 ECI), verb, top_bc, num_grids, movie_dt, movie_nt,
 //   movie_v{x,y,z}_flag; files buoy_<i>, s11_<i>, s13_<i> (i = 1..num_grids,
 //   init_del_par_field.cpp:21-123), src (init_src.cpp:16-45), rec_in
@@ -174,7 +175,7 @@ int run(int argc, char **argv)
     const bool check_energy = P.getb("check_energy", false);
     const int radius = (int)P.geti("radius", 0);
     if (radius <= 0) throw Fail("radius must be given and positive");
-    if (radius != 1) throw Fail("radius " + std::to_string(radius) + " is not supported (radius 1 only)");
+    if (radius > 6) throw Fail("radius " + std::to_string(radius) + " is not supported (FD coefficients exist for 1..6)");
     const double dt = P.getd("dt", -1.0);
     if (!(dt > 0)) throw Fail("dt must be given and positive");
     const int imode = (int)P.geti("imode", 1);

@@ -54,7 +54,8 @@ typedef struct {
     int n[ECSG_MAX_GRIDS][3];           /* cells per grid (Grid::size, grid.hpp:8) */
     double h[ECSG_MAX_GRIDS][3];        /* grid steps (Grid::step) */
     double dt;                          /* time step (main.cpp:51) */
-    int radius;                         /* stencil radius; 1 supported (main.cpp:48) */
+    int radius;                         /* stencil radius 1..6 (main.cpp:48, init_del_fdcoeff.cpp:38-85);
+                                           above 1 on a single grid only */
     int top_bc;                         /* 0 rigid, 1 free surface (main.cpp:60) */
     int imode;                          /* 1 = energy-conserving interface (ECI), 0 = the
                                            "intuitive" interpolation (II, ratio 2 only), main.cpp:54 */

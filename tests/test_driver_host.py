@@ -71,7 +71,7 @@ def test_dry_run_generic_positions(exe, tmp_path):
 @pytest.mark.parametrize("drop,extra,message", [
     ("dt=", [], "dt must be given"),
     ("radius=", [], "radius must be given"),
-    ("", ["radius=2"], "radius 2 is not supported"),
+    ("", ["radius=7"], "radius 7 is not supported"),
     ("movie_nt=", [], "movie_nt must be given"),
     ("", ["check_energy=maybe"], "not a boolean"),
     ("", ["num_grids=3"], "buoy_3"),

@@ -304,3 +304,76 @@ def test_ii_ghosts_are_second_order_interpolants():
         sl2 = (slice(2, ny2 - 2), slice(2, nx2 - 2))
         assert np.abs(g2[0][sl2] - want2[sl2]).max() <= 1e-12, c
     o.close()
+
+
+def _mode_model(n, R, rho=2000.0, vp=3000.0, vs=1500.0, h=10.0):
+    nx, ny, nz = n
+    shape = (nz + 1, ny + 1, nx + 1)
+    mu, lam = rho * vs * vs, rho * vp * vp - 2 * rho * vs * vs
+    s11 = np.full(shape, lam + 2 * mu)
+    s13 = np.full(shape, lam)
+    buoy = np.full(shape, 1.0 / rho)
+    g = Grid((nx, ny, nz), (h, h, h), buoy, [s11, s13])
+    dt = 0.5 * h / (vp * math.sqrt(3))
+    return Model(grids=[g], dt=dt, nsteps=4, iso=True, top_bc=0, radius=R,
+                 src_trace=np.zeros(0, np.float32), rec_loc=np.zeros((0, 4), np.int32))
+
+
+@pytest.mark.parametrize("R", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_radius_standing_mode_closed_form(R, axis):
+    """Radius-R stencils and their wall mirrors (update_vel.cpp,
+    update_stress.cpp, coefficients init_del_fdcoeff.cpp:38-85) on a 1-D
+    standing P mode between rigid walls: the normal stress cos(k x) on its
+    nodes and the velocity sin(k x) on its own nodes evolve as
+        B' = B - b A G,  A' = A + c B' G,  G = 2 sum_r coeff_r sin((2r+1) k h / 2),
+    exactly (to rounding), away from the other walls' reach (R cells a step)."""
+    W = 8 * R + 6  # cells across the mode: the other walls reach 2R cells a step
+    n = [W, W, W]
+    n[axis] = 24
+    m = _mode_model(tuple(n), R)
+    o = Oracle(m, eci_mode=1, check_energy=False)
+    L = n[axis]
+    h = m.grids[0].h[axis]
+    k = 3 * math.pi / (L * h)
+    vsel, ssel = (0, 3) if axis == 0 else (1, 4) if axis == 1 else (2, 5)
+    # coordinates along the axis: velocity staggered except for vx, stress nodes the other way
+    vstag = axis != 0
+
+    def coord(shape, stag):
+        idx = np.arange(shape[2 - axis]) - R
+        return (idx + (0.5 if stag else 0.0)) * h
+
+    def along(a, shape):
+        sh = [1, 1, 1]
+        sh[2 - axis] = shape[2 - axis]
+        return np.broadcast_to(a.reshape(sh), shape).copy()
+
+    sshape, _ = o.dims(0, ssel)
+    o.set(0, ssel, along(np.cos(k * coord(sshape, not vstag)), sshape))
+    fd = fd_coeff(R)
+    c = [f * m.dt / h for f in fd]
+    G = 2 * sum(c[r] * math.sin((2 * r + 1) * k * h / 2) for r in range(R))
+    rho = 2000.0
+    cmod = m.grids[0].stiff[0][0, 0, 0]
+    A, B = 1.0, 0.0
+    nsteps = 2
+    for _ in range(nsteps):
+        o.run(1)
+        B = B - (1.0 / rho) * A * G
+        A = A + cmod * B * G
+    v, sg = o.get(0, vsel), o.get(0, ssel)
+    vshape = v.shape
+    want_v = B * along(np.sin(k * coord(vshape, vstag)), vshape)
+    want_s = A * along(np.cos(k * coord(sshape, not vstag)), sshape)
+    # the centre, out of reach of the other walls after nsteps steps
+    lo, hi = R + 2 * R * nsteps + 1, R + W - 2 * R * nsteps - 1
+    sl = [slice(lo, hi)] * 3
+    sl[2 - axis] = slice(R, vshape[2 - axis] - R)  # every node of the mode axis, walls included
+    sl = tuple(sl)
+    assert np.abs(v[sl] - want_v[sl]).max() <= 1e-12 * abs(B), (R, axis)
+    sl_s = list(sl)
+    sl_s[2 - axis] = slice(R, sshape[2 - axis] - R)
+    sl_s = tuple(sl_s)
+    assert np.abs(sg[sl_s] - want_s[sl_s]).max() <= 1e-12 * abs(A), (R, axis)
+    o.close()
