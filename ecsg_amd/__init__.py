@@ -33,7 +33,7 @@ ABI_SYMBOLS = (
     "ecsg_reset", "ecsg_set_checkpointing", "ecsg_gradient", "ecsg_get_gradient", "ecsg_adjoint_step",
     "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune", "ecsg_set_fused",
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
-    "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs",
+    "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -91,6 +91,7 @@ def load_library(path: Optional[str] = None):
     L.ecsg_compute_energy.argtypes = [vp, P(d)]
     L.ecsg_compute_res.argtypes = [vp, i, P(d)]
     L.ecsg_field_avgabs.argtypes = [vp, i, i, P(d)]
+    L.ecsg_sbp_residuals.argtypes = [vp, i, P(d)]
     L.ecsg_step_n.argtypes = [vp, i, i]
     L.ecsg_field_dims.argtypes = [vp, i, i, P(i)]
     L.ecsg_get_field.argtypes = [vp, i, i, P(d)]
@@ -223,6 +224,12 @@ class Simulation:
     def residuals(self, which: int) -> np.ndarray:
         out = np.zeros(max(1, self.model.num_grids - 1))
         self._chk(self.L.ecsg_compute_res(self.h, which, _dp(out)))
+        return out
+
+    def sbp_residuals(self, g: int = 0) -> np.ndarray:
+        """TestSBP (test_sbp.cpp:5-199): the seven summation-by-parts residuals of grid g."""
+        out = np.zeros(7)
+        self._chk(self.L.ecsg_sbp_residuals(self.h, g, _dp(out)))
         return out
 
     def avg_abs(self, g: int, sel: int) -> float:

@@ -8,6 +8,7 @@
 #include "ecsg_fused.hpp"
 #include "ecsg_ii.hpp"
 #include "ecsg_radius.hpp"
+#include "ecsg_diag.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -63,6 +64,7 @@ public:
     virtual int end_step() = 0;
     virtual int residual(int which, double *out) = 0;
     virtual int avg_abs(int g, int sel, double *out) = 0;
+    virtual int sbp(int g, double *out) = 0;
     virtual int step_n(int n, int use_graph) = 0;
     virtual int steps_done() const = 0;
     virtual int sync() = 0;
@@ -183,7 +185,7 @@ public:
             hipFree(f.Bs); hipFree(f.A); hipFree(f.absp); hipFree(f.X0); hipFree(f.X1);
             hipFree(f.jpart); hipFree(f.stats);
         }
-        hipFree(d_rec); hipFree(d_traces); hipFree(d_step); hipFree(d_energy); hipFree(d_fix); hipFree(d_diag); hipFree(e_save);
+        hipFree(d_rec); hipFree(d_traces); hipFree(d_step); hipFree(d_energy); hipFree(d_fix); hipFree(d_diag); hipFree(e_save); hipFree(d_sbp);
         hipFree(d_src_old); hipFree(d_src_loc); hipFree(d_src_tr);
         adj_free();
         for (auto e : evs) hipEventDestroy(e);
@@ -1349,6 +1351,32 @@ public:
         HIPCHK(hipStreamSynchronize(s));
         return ECSG_OK;
     }
+
+    // test_sbp.cpp:5-199 on grid g: seven residuals
+    double *d_sbp = nullptr;
+    int sbp(int g, double *out) override
+    {
+        if (int rc = check_ready()) return rc;
+        if (g < 0 || g >= (int)G.size()) return fail(ECSG_EINVAL, "grid index out of range");
+        DevGrid<T> &d = G[g];
+        const int nblk = ((d.n[0] + BX) / BX) * ((d.n[1] + SBP_BY) / SBP_BY) * ((d.n[2] + SBP_ZC) / SBP_ZC);
+        const size_t need = 7 * (size_t)nblk + 8;
+        if (!d_sbp || sbp_n < need) {
+            HIPCHK(hipStreamSynchronize(s));
+            hipFree(d_sbp);
+            d_sbp = nullptr;
+            sbp_n = 0;
+            if (int rc = dalloc(&d_sbp, need)) return rc;
+            sbp_n = need;
+        }
+        hipLaunchKernelGGL((k_sbp_part<T>), dim3(nblk), dim3(BX, SBP_BY), 0, s, d.dev, cfg.radius, d_sbp, nblk);
+        hipLaunchKernelGGL(k_sbp_final, dim3(7), dim3(64), 0, s, (const double *)d_sbp, nblk, d_sbp + 7 * (size_t)nblk);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, d_sbp + 7 * (size_t)nblk, 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return ECSG_OK;
+    }
+    size_t sbp_n = 0;
 
     static constexpr int DIAG_PARTS = 1024;
     static constexpr int DIAG_N = DIAG_PARTS + 8;
@@ -2702,6 +2730,7 @@ int ecsg_update_ghost_syz(ecsg_ctx *ctx, int k) { CTX_CHECK(ctx); return ret(ctx
 int ecsg_end_step(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->end_step()); }
 int ecsg_compute_res(ecsg_ctx *ctx, int which, double *out) { CTX_CHECK(ctx); return ret(ctx, ctx->e->residual(which, out)); }
 int ecsg_field_avgabs(ecsg_ctx *ctx, int g, int sel, double *out) { CTX_CHECK(ctx); return ret(ctx, ctx->e->avg_abs(g, sel, out)); }
+int ecsg_sbp_residuals(ecsg_ctx *ctx, int g, double out[7]) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sbp(g, out)); }
 int ecsg_step_n(ecsg_ctx *ctx, int nsteps, int use_graph) { CTX_CHECK(ctx); return ret(ctx, ctx->e->step_n(nsteps, use_graph)); }
 int ecsg_steps_done(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->steps_done() : -1; }
 int ecsg_synchronize(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sync()); }

@@ -24,6 +24,10 @@
 //                       edge (update_ghost_vz.cpp:98-107) instead of the
 //                       committed one the golden II traces were made with
 //   graph=y|n           replay the time step from a HIP graph (default y)
+//   sbp=y               print the summation-by-parts residuals of grid 1 after
+//                       each stress update (TestSBP, test_sbp.cpp, as the
+//                       committed main.cpp calls it) and write them to
+//                       sbp_trace (n1 = steps, n2 = 7)
 //   dry_run=y           read and check every input, print the mapped source and
 //                       receiver positions, and stop before any device work
 // Errors are reported on stderr with exit status 1 where the reference asserts.
@@ -187,6 +191,7 @@ int run(int argc, char **argv)
     if (prec != "f64" && prec != "f32") throw Fail("precision must be f64 or f32");
     const bool aniso = P.getb("aniso", false);
     const bool use_graph = P.getb("graph", true);
+    const bool sbp = P.getb("sbp", false);
 
     const double movie_dt = P.getd("movie_dt", -1.0);
     if (!(movie_dt > 0)) throw Fail("movie_dt must be given and positive");
@@ -355,8 +360,9 @@ int run(int argc, char **argv)
     const int ni = num_grids - 1;
     const bool residuals = verb && ni > 0;
     std::vector<std::vector<float>> res(9 * (size_t)std::max(ni, 0));  // res_vec[k][j] (init_del_res_trace.cpp)
+    std::vector<float> sbp_tr;                                          // [7][steps]
     const auto t0 = std::chrono::steady_clock::now();
-    if (!verb) {
+    if (!verb && !sbp) {
         // whole steps, graph-replayed, stopping at the movie frames
         int done = 0;
         size_t next = 0;
@@ -386,7 +392,7 @@ int run(int argc, char **argv)
         };
         size_t next = 0;
         for (int niter = 0; niter < nsteps; niter++) {
-            std::cerr << "niter = " << niter << std::endl;
+            if (verb) std::cerr << "niter = " << niter << std::endl;
             for (int i = 0; i < num_grids; i++) C.chk(ecsg_update_vel(C.h, i), "UpdateVel");
             C.chk(ecsg_record_receivers(C.h), "receivers");
             if (residuals) {
@@ -398,6 +404,16 @@ int run(int argc, char **argv)
             if (residuals) res_pair(2, 6, ECSG_VZ, ECSG_SZZ);  // compute_res.cpp:67-95
             if (check_energy) C.chk(ecsg_compute_energy(C.h, nullptr), "energy");
             for (int i = 0; i < num_grids; i++) C.chk(ecsg_update_stress(C.h, i), "UpdateStress");
+            if (sbp) {
+                double r7[7];
+                C.chk(ecsg_sbp_residuals(C.h, 0, r7), "SBP residuals");
+                std::cerr << "SBP_res1..7 =";
+                for (int k = 0; k < 7; k++) {
+                    std::cerr << " " << r7[k];
+                    sbp_tr.push_back((float)r7[k]);
+                }
+                std::cerr << std::endl;
+            }
             for (int k = 0; k < ni; k++) {
                 C.chk(ecsg_update_ghost_sxz(C.h, k), "UpdateGhostSXZ");
                 C.chk(ecsg_update_ghost_syz(C.h, k), "UpdateGhostSYZ");
@@ -439,6 +455,16 @@ int run(int argc, char **argv)
         o.putq("label1", "Time"); o.putq("unit1", "ms");
         o.put("n2", n_rec);
         o.write(std::vector<float>(tr.begin(), tr.begin() + (size_t)n_rec * nsteps));
+    }
+    if (sbp) {
+        rsf::Output o;
+        o.path = P.file("sbp_trace");
+        o.put("n1", nsteps);
+        o.put("n2", 7);
+        std::vector<float> t((size_t)7 * nsteps);
+        for (int n = 0; n < nsteps; n++)
+            for (int k = 0; k < 7; k++) t[(size_t)k * nsteps + n] = sbp_tr[(size_t)n * 7 + k];
+        o.write(t);
     }
     if (residuals) {
         rsf::Output o;

@@ -128,6 +128,14 @@ int ecsg_compute_res(ecsg_ctx *ctx, int which, double *out);
  * fixed order. */
 int ecsg_field_avgabs(ecsg_ctx *ctx, int g, int sel, double *out);
 
+/* TestSBP (test_sbp.cpp:5-199; called after UpdateStress in the committed
+ * main.cpp): the seven summation-by-parts residuals of grid g,
+ * (u, D s)_u + (D u, s)_s for (vx, sxz, z), (vx, sxx, x), (vx, sxy, y),
+ * (vy, syz, z), (vz, szz, z), (vz, sxz, x), (vz, syz, y), with the
+ * reference's boundary weights and index ranges.  Summed in FP64 on the
+ * device in a fixed order. */
+int ecsg_sbp_residuals(ecsg_ctx *ctx, int g, double out[7]);
+
 /* Runs nsteps full steps of main.cpp:165-261 on the device (receivers and
  * energy recorded on the device), captured once into a hipGraph and replayed
  * when use_graph != 0. */

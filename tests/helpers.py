@@ -98,3 +98,69 @@ def rel(a, b):
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def _sbp_w(n_lo, n_hi, idx):
+    return (1 - 0.5 * (idx == n_lo)) * (1 - 0.5 * (idx == n_hi))
+def sbp_residuals(F, n, r=1):
+    """The seven summation-by-parts sums of TestSBP (test_sbp.cpp:5-199) from
+    the 9 wavefields F (Field layout with halo r, [z][y][x]) of a grid of n
+    cells, in numpy.  Reads follow the Field's flat storage, so an index past
+    a row continues in the next one, as the reference's do.  Returns the sums
+    and the sums of the absolute terms."""
+    nx, ny, nz = n
+    def at(a, xs, ys, zs):
+        # flat offsets in the Field layout (reads past a row continue in the next one)
+        L2, L1, L0 = a.shape
+        zz, yy, xx = np.meshgrid(np.asarray(zs) + r, np.asarray(ys) + r, np.asarray(xs) + r, indexing="ij")
+        return a.ravel()[(zz * L1 + yy) * L0 + xx]
+    vx, vy, vz, sxx, syy, szz, syz, sxz, sxy = [F[c] for c in range(9)]
+    X = lambda lo, hi: np.arange(lo, hi + 1)
+    def W(xs, ys, zs, wx, wy, wz):
+        gx = _sbp_w(xs[0], xs[-1], xs) if wx else np.ones(len(xs))
+        gy = _sbp_w(ys[0], ys[-1], ys) if wy else np.ones(len(ys))
+        gz = _sbp_w(zs[0], zs[-1], zs) if wz else np.ones(len(zs))
+        return gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    res, mag = [], []
+    def two(tA, tB):
+        res.append(tA.sum() + tB.sum()); mag.append(np.abs(tA).sum() + np.abs(tB).sum())
+    # ranges
+    vxr = (X(0, nx), X(0, ny), X(0, nz)); vyr = (X(0, nx - 1), X(0, ny - 1), X(0, nz)); vzr = (X(0, nx - 1), X(0, ny), X(0, nz - 1))
+    snr = (X(0, nx - 1), X(0, ny), X(0, nz)); syzr = (X(0, nx - 1), X(0, ny - 1), X(0, nz - 1))
+    sxzr = (X(0, nx), X(0, ny), X(0, nz - 1)); sxyr = (X(0, nx), X(0, ny - 1), X(0, nz))
+    x, y, z = vxr
+    m = W(x, y, z, 1, 1, 1)
+    A = m * at(vx, x, y, z) * (at(sxz, x, y, z) - at(sxz, x, y, z - 1))
+    x, y, z = sxzr
+    B = W(x, y, z, 1, 1, 0) * (at(vx, x, y, z + 1) - at(vx, x, y, z)) * at(sxz, x, y, z)
+    two(A, B)
+    x, y, z = vxr
+    A = m * at(vx, x, y, z) * (at(sxx, x, y, z) - at(sxx, x - 1, y, z))
+    x, y, z = snr
+    B = W(x, y, z, 0, 1, 1) * (at(vx, x + 1, y, z) - at(vx, x, y, z)) * at(sxx, x, y, z)
+    two(A, B)
+    x, y, z = vxr
+    A = m * at(vx, x, y, z) * (at(sxy, x, y, z) - at(sxy, x, y - 1, z))
+    x, y, z = sxyr
+    B = W(x, y, z, 1, 0, 1) * (at(vx, x, y + 1, z) - at(vx, x, y, z)) * at(sxy, x, y, z)
+    two(A, B)
+    x, y, z = vyr
+    A = W(x, y, z, 0, 0, 1) * at(vy, x, y, z) * (at(syz, x, y, z) - at(syz, x, y, z - 1))
+    x, y, z = syzr
+    B = (at(vy, x, y, z + 1) - at(vy, x, y, z)) * at(syz, x, y, z)
+    two(A, B)
+    x, y, z = vzr
+    wy_ = W(x, y, z, 0, 1, 0)
+    A = wy_ * at(vz, x, y, z) * (at(szz, x, y, z + 1) - at(szz, x, y, z))
+    x2, y2, z2 = snr
+    B = W(x2, y2, z2, 0, 1, 1) * (at(vz, x2, y2, z2) - at(vz, x2, y2, z2 - 1)) * at(szz, x2, y2, z2)
+    two(A, B)
+    A = wy_ * at(vz, x, y, z) * (at(sxz, x + 1, y, z) - at(sxz, x, y, z))
+    x2, y2, z2 = sxzr
+    B = W(x2, y2, z2, 1, 1, 0) * (at(vz, x2, y2, z2) - at(vz, x2 - 1, y2, z2)) * at(sxz, x2, y2, z2)
+    two(A, B)
+    A = wy_ * at(vz, x, y, z) * (at(syz, x, y + 1, z) - at(syz, x, y, z))
+    x2, y2, z2 = syzr
+    B = (at(vz, x2, y2, z2) - at(vz, x2, y2 - 1, z2)) * at(syz, x2, y2, z2)
+    two(A, B)
+    return np.array(res), np.array(mag)

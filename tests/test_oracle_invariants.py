@@ -377,3 +377,27 @@ def test_radius_standing_mode_closed_form(R, axis):
     sl_s = tuple(sl_s)
     assert np.abs(sg[sl_s] - want_s[sl_s]).max() <= 1e-12 * abs(A), (R, axis)
     o.close()
+
+
+@pytest.mark.parametrize("top_bc", [0, 1])
+@pytest.mark.parametrize("iso", [False, True])
+def test_sbp_identities_vanish_on_single_grids(top_bc, iso):
+    """The summation-by-parts identities of test_sbp.cpp hold to rounding on
+    the states the scheme produces on a single grid (rigid walls, rigid or
+    free top), and the interface flux of a composite grid shows in the z ones
+    (vx-sxz, vy-syz, vz-szz) of its fine grid."""
+    from helpers import sbp_residuals
+    m = single(n=(14, 13, 12), nsteps=30, seed=3, top_bc=top_bc, src_type=1, iso=iso)
+    o = Oracle(m, eci_mode=1, check_energy=False)
+    o.run(30)
+    r, mag = sbp_residuals({c: o.get(0, c) for c in range(9)}, m.grids[0].n)
+    assert np.all(mag > 0)
+    assert np.all(np.abs(r) <= 1e-13 * mag), r / mag
+    o.close()
+    m = composite((3,), nsteps=30, seed=4)
+    o = Oracle(m, eci_mode=1, check_energy=False)
+    o.run(30)
+    r, mag = sbp_residuals({c: o.get(0, c) for c in range(9)}, m.grids[0].n)
+    assert np.all(np.abs(r[[1, 2, 5, 6]]) <= 1e-13 * mag[[1, 2, 5, 6]])
+    assert np.all(np.abs(r[[0, 3, 4]]) >= 1e-4 * mag[[0, 3, 4]])
+    o.close()
