@@ -34,6 +34,7 @@ ABI_SYMBOLS = (
     "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune", "ecsg_set_fused",
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
+    "ecsg_checkpoint_interval",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -102,6 +103,7 @@ def load_library(path: Optional[str] = None):
     L.ecsg_profile_kernels.argtypes = [vp, i, P(d), P(i)]
     L.ecsg_reset.argtypes = [vp]
     L.ecsg_set_checkpointing.argtypes = [vp, i, i]
+    L.ecsg_checkpoint_interval.argtypes = [vp]
     L.ecsg_gradient.argtypes = [vp, i, P(d), P(d)]
     L.ecsg_get_gradient.argtypes = [vp, i, i, P(d)]
     L.ecsg_adjoint_step.argtypes = [vp]
@@ -282,7 +284,13 @@ class Simulation:
         self._chk(self.L.ecsg_reset(self.h))
 
     def set_checkpointing(self, every: int = 50, fmt: str = "native"):
+        """every = 0: the largest interval that fits in free device memory."""
         self._chk(self.L.ecsg_set_checkpointing(self.h, int(every), CKPT_FORMATS[fmt]))
+
+    @property
+    def checkpoint_interval(self) -> int:
+        """The interval the last gradient call used."""
+        return int(self.L.ecsg_checkpoint_interval(self.h))
 
     def gradient(self, nsteps: int, observed: np.ndarray) -> float:
         """Misfit 1/2 sum (traces - observed)^2 over nsteps steps from rest and

@@ -21,6 +21,14 @@
 
 namespace ecsg {
 
+// f[q] += v (gradient accumulators and adjoint fields; plain loads and
+// stores: the stencils' cache hints made these sweeps slower)
+template <typename T>
+__device__ __forceinline__ void acc_st(T *f, long q, T v)
+{
+    f[q] += v;
+}
+
 template <typename T>
 struct AdjGridDev {
     T *lam[9];          // adjoint vx vy vz sxx syy szz syz sxz sxy
@@ -152,9 +160,9 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
         const T ex_xm = shfl_upw(ex);
         const T hxz_xp = shfl_dn(hxz), hxy_xp = shfl_dn(hxy);
         if (out) {
-            if (ix >= 0 && iy >= 0 && iz >= 0) a.lam[0][q] += (ex_xm - ex) + cz * (hxz_m - hxz) + cy * (hxym - hxy);
-            if (iz >= 0) a.lam[1][q] += (ey - eyp) + cz * (hyz_m - hyz) + cx * (hxy - hxy_xp);
-            if (iy >= 0) a.lam[2][q] += (ez - ez_n) + cy * (hyzm - hyz) + cx * (hxz - hxz_xp);
+            if (ix >= 0 && iy >= 0 && iz >= 0) acc_st(a.lam[0], q, (ex_xm - ex) + cz * (hxz_m - hxz) + cy * (hxym - hxy));
+            if (iz >= 0) acc_st(a.lam[1], q, (ey - eyp) + cz * (hyz_m - hyz) + cx * (hxy - hxy_xp));
+            if (iy >= 0) acc_st(a.lam[2], q, (ez - ez_n) + cy * (hyzm - hyz) + cx * (hxz - hxz_xp));
             // gradient w.r.t. the staged stiffnesses at this stress point
             if (ix >= 0 && iy >= 0 && iz >= 0) {
                 const T *fvx = a.fv[0], *fvy = a.fv[1], *fvz = a.fv[2];
@@ -165,29 +173,29 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
                     if (a.free_top && iz == 0) {
                         const T c33 = a.c[2][q], c23 = a.c[3][q], c13 = a.c[4][q];
                         const T u = c13 / c33, v = c23 / c33;
-                        a.gc[0][q] += l1 * X;
-                        a.gc[1][q] += l2 * Y;
-                        a.gc[5][q] += l1 * Y + l2 * X;
-                        a.gc[4][q] += l1 * (T(-2) * u * X - v * Y) + l2 * (-v * X);
-                        a.gc[3][q] += l1 * (-u * Y) + l2 * (-u * X - T(2) * v * Y);
-                        a.gc[2][q] += l1 * (u * u * X + u * v * Y) + l2 * (v * u * X + v * v * Y);
+                        acc_st(a.gc[0], q, l1 * X);
+                        acc_st(a.gc[1], q, l2 * Y);
+                        acc_st(a.gc[5], q, l1 * Y + l2 * X);
+                        acc_st(a.gc[4], q, l1 * (T(-2) * u * X - v * Y) + l2 * (-v * X));
+                        acc_st(a.gc[3], q, l1 * (-u * Y) + l2 * (-u * X - T(2) * v * Y));
+                        acc_st(a.gc[2], q, l1 * (u * u * X + u * v * Y) + l2 * (v * u * X + v * v * Y));
                     } else {
                         const T l3 = lzz[q];
                         const T Z = cz * (fvz[q] - fvz[q - S2]);
-                        a.gc[0][q] += l1 * X;
-                        a.gc[1][q] += l2 * Y;
-                        a.gc[2][q] += l3 * Z;
-                        a.gc[3][q] += l2 * Z + l3 * Y;
-                        a.gc[4][q] += l1 * Z + l3 * X;
-                        a.gc[5][q] += l1 * Y + l2 * X;
+                        acc_st(a.gc[0], q, l1 * X);
+                        acc_st(a.gc[1], q, l2 * Y);
+                        acc_st(a.gc[2], q, l3 * Z);
+                        acc_st(a.gc[3], q, l2 * Z + l3 * Y);
+                        acc_st(a.gc[4], q, l1 * Z + l3 * X);
+                        acc_st(a.gc[5], q, l1 * Y + l2 * X);
                     }
                 }
                 if (ix <= nx - 1 && iy <= ny - 1 && iz <= nz - 1)
-                    a.gc[6][q] += lyz[q] * (T(0.5) * (cz * (fvy[q + S2] - fvy[q]) + cy * (fvz[q + P] - fvz[q])));
+                    acc_st(a.gc[6], q, lyz[q] * (T(0.5) * (cz * (fvy[q + S2] - fvy[q]) + cy * (fvz[q + P] - fvz[q]))));
                 if (iz <= nz - 1)
-                    a.gc[7][q] += lxz[q] * (T(0.5) * (cz * (fvx[q + S2] - fvx[q]) + cx * (fvz[q] - fvz[q - 1])));
+                    acc_st(a.gc[7], q, lxz[q] * (T(0.5) * (cz * (fvx[q + S2] - fvx[q]) + cx * (fvz[q] - fvz[q - 1]))));
                 if (iy <= ny - 1)
-                    a.gc[8][q] += lxy[q] * (T(0.5) * (cy * (fvx[q + P] - fvx[q]) + cx * (fvy[q] - fvy[q - 1])));
+                    acc_st(a.gc[8], q, lxy[q] * (T(0.5) * (cy * (fvx[q + P] - fvx[q]) + cx * (fvy[q] - fvy[q - 1]))));
             }
         }
         hyz_m = hyz; hxz_m = hxz;
@@ -249,24 +257,24 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
         if (out) {
             if (iz >= 0) {
                 if (ix <= nx - 1) {
-                    a.lam[3][q] += cx * (wx - wx_xp);
-                    a.lam[4][q] += cy * (wym - wy);
-                    a.lam[5][q] += cz * (wz_m - wz);
+                    acc_st(a.lam[3], q, cx * (wx - wx_xp));
+                    acc_st(a.lam[4], q, cy * (wym - wy));
+                    acc_st(a.lam[5], q, cz * (wz_m - wz));
                 }
-                if (iy <= ny - 1) a.lam[8][q] += cy * (wx - wxp) + cx * (wy_xm - wy);
+                if (iy <= ny - 1) acc_st(a.lam[8], q, cy * (wx - wxp) + cx * (wy_xm - wy));
             }
-            a.lam[7][q] += cz * (wx - wx_n) + cx * (wz_xm - wz);
-            if (ix <= nx - 1 && iy <= ny - 1) a.lam[6][q] += cz * (wy - wy_n) + cy * (wz - wzp);
+            acc_st(a.lam[7], q, cz * (wx - wx_n) + cx * (wz_xm - wz));
+            if (ix <= nx - 1 && iy <= ny - 1) acc_st(a.lam[6], q, cz * (wy - wy_n) + cy * (wz - wzp));
             // gradient w.r.t. buoyancy at this velocity point, forward s^n
             if (iz >= 0) {
                 const T *sxx = a.fs[0], *syy = a.fs[1], *szz = a.fs[2], *syz = a.fs[3], *sxz = a.fs[4], *sxy = a.fs[5];
                 const bool zin = iz >= a.top_eff && iz <= nz - a.bottom;
                 if (ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && zin)
-                    a.gb[0][q] += lvx[q] * (cx * (sxx[q] - sxx[q - 1]) + cy * (sxy[q] - sxy[q - P]) + cz * (sxz[q] - sxz[q - S2]));
+                    acc_st(a.gb[0], q, lvx[q] * (cx * (sxx[q] - sxx[q - 1]) + cy * (sxy[q] - sxy[q - P]) + cz * (sxz[q] - sxz[q - S2])));
                 if (ix <= nx - 1 && iy <= ny - 1 && zin)
-                    a.gb[1][q] += lvy[q] * (cx * (sxy[q + 1] - sxy[q]) + cz * (syz[q] - syz[q - S2]) + cy * (syy[q + P] - syy[q]));
+                    acc_st(a.gb[1], q, lvy[q] * (cx * (sxy[q + 1] - sxy[q]) + cz * (syz[q] - syz[q - S2]) + cy * (syy[q + P] - syy[q])));
                 if (ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && iz <= nz - 1)
-                    a.gb[2][q] += lvz[q] * (cx * (sxz[q + 1] - sxz[q]) + cz * (szz[q + S2] - szz[q]) + cy * (syz[q] - syz[q - P]));
+                    acc_st(a.gb[2], q, lvz[q] * (cx * (sxz[q + 1] - sxz[q]) + cz * (szz[q + S2] - szz[q]) + cy * (syz[q] - syz[q - P])));
             }
         }
         wz_m = wz;

@@ -19,7 +19,7 @@ namespace ecsg {
 template <typename T>
 struct IfWork {
     IfaceDev<T> a;
-    double *B, *M, *Dp, *U;  // vz-szz right-hand side and prefactored elimination
+    double *B, *M, *Rp, *U;  // vz-szz right-hand side, prefactored elimination, pivot reciprocals
     double *Bs, *absp;       // vx-sxz right-hand side and per-block |B| sums
 };
 
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(64) void k_gvz_solve_batch(const IfWork<T> *__restr
 {
     const int m = batch_find(start, n, blockIdx.x);
     const IfWork<T> &W = w[m];
-    gvz_solve_body(W.a, (blockIdx.x - start[m]) * 64 + (int)threadIdx.x, W.B, W.M, W.Dp, W.U);
+    gvz_solve_body(W.a, (blockIdx.x - start[m]) * 64 + (int)threadIdx.x, W.B, W.M, W.Rp, W.U);
 }
 
 template <typename T>

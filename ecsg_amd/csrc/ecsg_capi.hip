@@ -77,6 +77,7 @@ public:
     virtual int profile(int nsteps, double ms[6], int cnt[6]) = 0;
     virtual int reset() = 0;
     virtual int set_checkpointing(int every, int format) = 0;
+    virtual int checkpoint_interval() const = 0;
     virtual int gradient(int nsteps, const double *obs, double *misfit) = 0;
     virtual int get_gradient(int g, int sel, double *out) = 0;
     virtual int adjoint_step() = 0;
@@ -123,6 +124,7 @@ struct DevGrid {
 struct DevIface {
     int r = 0, nx2 = 0, ny2 = 0, nz1 = 0;
     double *B = nullptr, *M = nullptr, *Dp = nullptr, *U = nullptr;  // vz system
+    double *Rp = nullptr;                                            // 1 / Dp
     double *Bs = nullptr, *A = nullptr, *absp = nullptr;             // sxz system
     double *X0 = nullptr, *X1 = nullptr, *jpart = nullptr;
     long long *stats = nullptr;
@@ -181,7 +183,7 @@ public:
             hipFree(g.se);
         }
         for (auto &f : I) {
-            hipFree(f.B); hipFree(f.M); hipFree(f.Dp); hipFree(f.U);
+            hipFree(f.B); hipFree(f.M); hipFree(f.Dp); hipFree(f.U); hipFree(f.Rp);
             hipFree(f.Bs); hipFree(f.A); hipFree(f.absp); hipFree(f.X0); hipFree(f.X1);
             hipFree(f.jpart); hipFree(f.stats);
         }
@@ -279,6 +281,7 @@ public:
             if (int rc = dalloc(&f.B, nl)) return rc;
             if (int rc = dalloc(&f.M, nl)) return rc;
             if (int rc = dalloc(&f.Dp, nl)) return rc;
+            if (int rc = dalloc(&f.Rp, nl)) return rc;
             if (int rc = dalloc(&f.U, nl)) return rc;
             if (int rc = dalloc(&f.Bs, nn)) return rc;
             if (int rc = dalloc(&f.A, 9 * nn)) return rc;
@@ -620,6 +623,11 @@ public:
         }
         HIPCHK(hipMemcpy(f.M, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(f.Dp, Dp.data(), sizeof(double) * Dp.size(), hipMemcpyHostToDevice));
+        // the forward back substitution multiplies by the pivot reciprocals
+        // (one dependent multiply per row instead of a division)
+        std::vector<double> Rp(Dp.size());
+        for (size_t q = 0; q < Dp.size(); q++) Rp[q] = 1.0 / Dp[q];
+        HIPCHK(hipMemcpy(f.Rp, Rp.data(), sizeof(double) * Rp.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(f.U, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice));
         // vx-sxz: 9 bands on interior coarse nodes; buoyancy at primary nodes
         const int n0 = nx2 - 1, n1 = ny2 - 1;
@@ -806,7 +814,7 @@ public:
             const long nl = (long)f.nx2 * (f.ny2 - 1);
             ev_begin();
             hipLaunchKernelGGL((k_gvz_rhs<T>), dim3((nl + 255) / 256), dim3(256), 0, s, a, f.B);
-            hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Dp, f.U);
+            hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Rp, f.U);
             const long nf = (long)f.r * f.nx2 * (f.r * f.ny2 - 1);
             hipLaunchKernelGGL((k_gvz_fine<T>), dim3((nf + 255) / 256), dim3(256), 0, s, a);
             ev_end(4);
@@ -1004,7 +1012,7 @@ public:
         const long nl = (long)f.nx2 * (f.ny2 - 1);
         ev_begin();
         hipLaunchKernelGGL((k_gvz_rhs<T>), dim3((nl + 255) / 256), dim3(256), 0, s, a, f.B);
-        hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Dp, f.U);
+        hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Rp, f.U);
         const long nf = (long)f.r * f.nx2 * (f.r * f.ny2 - 1);
         hipLaunchKernelGGL((k_gvz_fine<T>), dim3((nf + 255) / 256), dim3(256), 0, s, a);
         ev_end(4);
@@ -2003,26 +2011,55 @@ public:
 
     int set_checkpointing(int every, int format) override
     {
-        if (every < 1) return fail(ECSG_EINVAL, "checkpoint interval must be >= 1");
+        if (every < 0) return fail(ECSG_EINVAL, "checkpoint interval must be >= 0 (0: sized from free device memory)");
         if (format < 0 || format > 2) return fail(ECSG_EINVAL, "checkpoint format must be 0 (native), 1 (fp16) or 2 (bf16)");
         ck_every = every;
         ck_fmt = format;
         return ECSG_OK;
     }
 
-    int ck_prepare(int nck)
+    // Checkpoint interval for ck_every == 0: the largest K whose ring (K+1
+    // states) and checkpoints (one per segment but the last) fit in the free
+    // device memory, less a reserve.  Larger K means fewer recomputed steps:
+    // the last segment is never recomputed.
+    int auto_interval(int N, int *K_out)
+    {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        double state = 0, ckb = 0;
+        for (const DevGrid<T> &d : G) {
+            state += 9.0 * (double)d.astride * sizeof(T);
+            ckb += 9.0 * (double)d.astride * (ck_fmt == 0 ? sizeof(T) : sizeof(unsigned short));
+        }
+        for (const DevIface &f : I) ckb += 8.0 * (f.nx2 + 1) * (f.ny2 + 1);
+        // what this handle already holds for the ring and checkpoints is reusable
+        double avail = (double)fr + (double)ring_alloc * state + (double)ck_alloc * ckb;
+        avail -= std::max(4.0e9, 0.05 * (double)tot);
+        int K = 1;
+        for (int k = N; k >= 1; k--) {
+            const int nseg = (N + k - 1) / k;
+            if ((k + 1) * state + std::max(nseg - 1, 1) * ckb <= avail) { K = k; break; }
+        }
+        *K_out = K;
+        return ECSG_OK;
+    }
+    int last_K = 0;
+    int checkpoint_interval() const override { return last_K; }
+
+    int ck_prepare(int nck, int K)
     {
         const int ng = (int)G.size(), ni = (int)I.size();
-        if (ck_every + 1 > ring_alloc) {
+        if (K + 1 > ring_alloc) {
             for (auto p : ring) dfree(p);
             ring.clear();
-            for (int q = 0; q <= ck_every; q++)
+            ring_alloc = 0;
+            for (int q = 0; q <= K; q++)
                 for (int g = 0; g < ng; g++) {
                     T *p = nullptr;
                     if (int rc = dalloc(&p, (size_t)9 * G[g].astride)) return rc;
                     ring.push_back(p);
                 }
-            ring_alloc = ck_every + 1;
+            ring_alloc = K + 1;
         }
         if (nck > ck_alloc || ck_fmt != ck_alloc_fmt) {
             for (auto p : ck_nat) dfree(p);
@@ -2128,16 +2165,33 @@ public:
         if (nrec > 0 && !obs) return fail(ECSG_EINVAL, "gradient: null observed traces");
         if (int rc = adj_build()) return rc;
         if (int rc = build_adj_receivers()) return rc;
-        const int K = ck_every;
+        int K = ck_every;
+        if (K == 0)
+            if (int rc = auto_interval(N, &K)) return rc;
+        K = std::min(K, N);
+        last_K = K;
         const int nseg = (N + K - 1) / K;
-        if (int rc = ck_prepare(nseg)) return rc;
+        if (int rc = ck_prepare(std::max(nseg - 1, 1), K)) return rc;
         if (int rc = reset()) return rc;
-        // forward pass with checkpoints
+        // forward pass with checkpoints; the last segment fills the ring
+        // directly (the states the backward pass needs), so it is never
+        // recomputed
         for (int j = 0; j < nseg; j++) {
-            if (int rc = ck_save(j)) return rc;
+            const bool last = j == nseg - 1;
             const int len = std::min(K, N - j * K);
-            for (int t = 0; t < len; t++)
-                if (int rc = enqueue_step()) return rc;
+            if (!last) {
+                if (int rc = ck_save(j)) return rc;
+                for (int t = 0; t < len; t++)
+                    if (int rc = enqueue_step()) return rc;
+            } else {
+                if (int rc = ring_save(0)) return rc;
+                for (int t = 0; t < len; t++) {
+                    set_store(t + 1);
+                    const int rc = enqueue_step();
+                    set_store(-1);
+                    if (rc) return rc;
+                }
+            }
         }
         HIPCHK(hipStreamSynchronize(s));
         // residuals and misfit
@@ -2159,14 +2213,16 @@ public:
         // backward over the segments
         for (int j = nseg - 1; j >= 0; j--) {
             const int len = std::min(K, N - j * K);
-            if (int rc = ck_restore(j, j * K)) return rc;
-            if (int rc = ring_save(0)) return rc;
-            for (int t = 0; t < len; t++) {
-                // the step's kernels also write every value into ring slot t+1
-                set_store(t + 1);
-                const int rc = enqueue_step();
-                set_store(-1);
-                if (rc) return rc;
+            if (j < nseg - 1) {
+                if (int rc = ck_restore(j, j * K)) return rc;
+                if (int rc = ring_save(0)) return rc;
+                for (int t = 0; t < len; t++) {
+                    // the step's kernels also write every value into ring slot t+1
+                    set_store(t + 1);
+                    const int rc = enqueue_step();
+                    set_store(-1);
+                    if (rc) return rc;
+                }
             }
             for (int t = len - 1; t >= 0; t--)
                 if (int rc = adj_step_impl(j * K + t, t + 1, t, true)) return rc;
@@ -2462,7 +2518,7 @@ public:
                 DevIface &f = e->I[k];
                 IfWork<T> w{};
                 w.a = e->iface_dev(k);
-                w.B = f.B; w.M = f.M; w.Dp = f.Dp; w.U = f.U; w.Bs = f.Bs; w.absp = f.absp;
+                w.B = f.B; w.M = f.M; w.Rp = f.Rp; w.U = f.U; w.Bs = f.Bs; w.absp = f.absp;
                 ifs.push_back(w);
                 const long r = f.r;
                 const long nb[5] = {((long)f.nx2 * (f.ny2 - 1) + 255) / 256, ((long)f.nx2 + 63) / 64,
@@ -2760,6 +2816,7 @@ int ecsg_set_checkpointing(ecsg_ctx *ctx, int every, int format)
     CTX_CHECK(ctx);
     return ret(ctx, ctx->e->set_checkpointing(every, format));
 }
+int ecsg_checkpoint_interval(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->checkpoint_interval() : 0; }
 int ecsg_gradient(ecsg_ctx *ctx, int nsteps, const double *observed, double *misfit)
 {
     CTX_CHECK(ctx);

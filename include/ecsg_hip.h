@@ -184,9 +184,15 @@ enum { ECSG_CKPT_NATIVE = 0, ECSG_CKPT_FP16 = 1, ECSG_CKPT_BF16 = 2 };
 /* Zero every wavefield, interface warm start, trace and the step counter. */
 int ecsg_reset(ecsg_ctx *ctx);
 
-/* Checkpoint interval (default 50) and storage: native precision, or IEEE
- * half / bfloat16 of each array divided by its maximum magnitude. */
+/* Checkpoint interval (default 50; 0: the largest interval whose ring of
+ * recomputed states and checkpoints fit in the free device memory) and
+ * storage: native precision, or IEEE half / bfloat16 of each array divided
+ * by its maximum magnitude.  The last segment of a gradient is never
+ * recomputed (the forward pass leaves its states in the ring). */
 int ecsg_set_checkpointing(ecsg_ctx *ctx, int every, int format);
+
+/* The checkpoint interval the last ecsg_gradient call used (0 before any). */
+int ecsg_checkpoint_interval(const ecsg_ctx *ctx);
 
 /* Runs the forward pass and the adjoint pass; misfit receives J.  After the
  * call the forward state is that of an intermediate step: ecsg_reset before
