@@ -28,6 +28,19 @@ __device__ __forceinline__ void acc_st(T *f, long q, T v)
 {
     f[q] += v;
 }
+#ifndef ECSG_GRAD_NT
+#define ECSG_GRAD_NT 1
+#endif
+// the same for the gradient-only sweeps (ECSG_GRAD_NT: with the cache hints)
+template <typename T>
+__device__ __forceinline__ void acc_g(T *f, long q, T v)
+{
+#if ECSG_GRAD_NT
+    stc(f, q, ldc(f, q) + v);
+#else
+    f[q] += v;
+#endif
+}
 
 template <typename T>
 struct AdjGridDev {
@@ -84,8 +97,8 @@ __global__ __launch_bounds__(256) void k_adj_coef(GridDev<T> g, T *out, long ast
 
 
 // ---------------------------------------------------------------------------
-// Transpose of UpdateStress (stress stencil) on the velocities, with the
-// gradient w.r.t. the nine staged stiffnesses.  Forward:
+// Transpose of UpdateStress (stress stencil) on the velocities (the gradient
+// w.r.t. the nine staged stiffnesses is k_adj_grad's).  Forward:
 //   normal:  sxx += c11 cx dvx + c12 cy dvy + c13 cz dvz (free row: condensed moduli)
 //   syz += 1/2 c44 (cz dz vy + cy dy vz), sxz += 1/2 c55 (...), sxy += 1/2 c66 (...)
 // The adjoint weights e = C lam (normal) and h = 1/2 c lam (shear) live on
@@ -163,40 +176,6 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
             if (ix >= 0 && iy >= 0 && iz >= 0) acc_st(a.lam[0], q, (ex_xm - ex) + cz * (hxz_m - hxz) + cy * (hxym - hxy));
             if (iz >= 0) acc_st(a.lam[1], q, (ey - eyp) + cz * (hyz_m - hyz) + cx * (hxy - hxy_xp));
             if (iy >= 0) acc_st(a.lam[2], q, (ez - ez_n) + cy * (hyzm - hyz) + cx * (hxz - hxz_xp));
-            // gradient w.r.t. the staged stiffnesses at this stress point
-            if (ix >= 0 && iy >= 0 && iz >= 0) {
-                const T *fvx = a.fv[0], *fvy = a.fv[1], *fvz = a.fv[2];
-                if (ix <= nx - 1) {
-                    const T l1 = lxx[q], l2 = lyy[q];
-                    const T X = cx * (fvx[q + 1] - fvx[q]);
-                    const T Y = cy * (fvy[q] - fvy[q - P]);
-                    if (a.free_top && iz == 0) {
-                        const T c33 = a.c[2][q], c23 = a.c[3][q], c13 = a.c[4][q];
-                        const T u = c13 / c33, v = c23 / c33;
-                        acc_st(a.gc[0], q, l1 * X);
-                        acc_st(a.gc[1], q, l2 * Y);
-                        acc_st(a.gc[5], q, l1 * Y + l2 * X);
-                        acc_st(a.gc[4], q, l1 * (T(-2) * u * X - v * Y) + l2 * (-v * X));
-                        acc_st(a.gc[3], q, l1 * (-u * Y) + l2 * (-u * X - T(2) * v * Y));
-                        acc_st(a.gc[2], q, l1 * (u * u * X + u * v * Y) + l2 * (v * u * X + v * v * Y));
-                    } else {
-                        const T l3 = lzz[q];
-                        const T Z = cz * (fvz[q] - fvz[q - S2]);
-                        acc_st(a.gc[0], q, l1 * X);
-                        acc_st(a.gc[1], q, l2 * Y);
-                        acc_st(a.gc[2], q, l3 * Z);
-                        acc_st(a.gc[3], q, l2 * Z + l3 * Y);
-                        acc_st(a.gc[4], q, l1 * Z + l3 * X);
-                        acc_st(a.gc[5], q, l1 * Y + l2 * X);
-                    }
-                }
-                if (ix <= nx - 1 && iy <= ny - 1 && iz <= nz - 1)
-                    acc_st(a.gc[6], q, lyz[q] * (T(0.5) * (cz * (fvy[q + S2] - fvy[q]) + cy * (fvz[q + P] - fvz[q]))));
-                if (iz <= nz - 1)
-                    acc_st(a.gc[7], q, lxz[q] * (T(0.5) * (cz * (fvx[q + S2] - fvx[q]) + cx * (fvz[q] - fvz[q - 1]))));
-                if (iy <= ny - 1)
-                    acc_st(a.gc[8], q, lxy[q] * (T(0.5) * (cy * (fvx[q + P] - fvx[q]) + cx * (fvy[q] - fvy[q - 1]))));
-            }
         }
         hyz_m = hyz; hxz_m = hxz;
         ex = ex_n; ey = ey_n; ez = ez_n; hyz = hyz_n; hxz = hxz_n; hxy = hxy_n;
@@ -204,9 +183,72 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
     }
 }
 
+// Gradient w.r.t. the nine staged stiffnesses at the stress points:
+// lam_s (after the transposed stress mirrors) times the derivative of the
+// stress update w.r.t. each coefficient, from this step's forward velocities.
+// Own sweep (64 x GBY threads, GZC planes per thread, the forward velocities'
+// z neighbours carried in registers): apart from the transposed stencil its
+// register needs are small, and lam_s is not written by k_adj_stress.
+constexpr int GBY = 4;
+constexpr int GZC = 8;
+
+template <typename T>
+__global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
+{
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int gx = (nx + 64) / 64, gy = (ny + GBY) / GBY;
+    const int tile = xcd_tile(blockIdx.x, gx * gy * ((nz + GZC) / GZC));
+    const int bx = tile % gx, by = (tile / gx) % gy, bz = tile / (gx * gy);
+    const int ix = bx * 64 + threadIdx.x, iy = by * GBY + threadIdx.y;
+    if (ix > nx || iy > ny) return;
+    const int z0 = bz * GZC, z1 = min(z0 + GZC, nz + 1);
+    const long P = a.P, S2 = a.S2;
+    const T cx = a.cx, cy = a.cy, cz = a.cz;
+    const T *lxx = a.lam[3], *lyy = a.lam[4], *lzz = a.lam[5], *lyz = a.lam[6], *lxz = a.lam[7], *lxy = a.lam[8];
+    const T *fvx = a.fv[0], *fvy = a.fv[1], *fvz = a.fv[2];
+    long q = a.o + (long)z0 * S2 + (long)iy * P + ix;
+    T fvz_m = fvz[q - S2], fvx_c = fvx[q], fvy_c = fvy[q];
+    for (int iz = z0; iz < z1; iz++, q += S2) {
+        const T fvx_n = fvx[q + S2], fvy_n = fvy[q + S2], fvz_c = fvz[q];
+        if (ix <= nx - 1) {
+            const T l1 = lxx[q], l2 = lyy[q];
+            const T X = cx * (fvx[q + 1] - fvx_c);
+            const T Y = cy * (fvy_c - fvy[q - P]);
+            if (a.free_top && iz == 0) {
+                const T c33 = a.c[2][q], c23 = a.c[3][q], c13 = a.c[4][q];
+                const T u = c13 / c33, v = c23 / c33;
+                acc_g(a.gc[0], q, l1 * X);
+                acc_g(a.gc[1], q, l2 * Y);
+                acc_g(a.gc[5], q, l1 * Y + l2 * X);
+                acc_g(a.gc[4], q, l1 * (T(-2) * u * X - v * Y) + l2 * (-v * X));
+                acc_g(a.gc[3], q, l1 * (-u * Y) + l2 * (-u * X - T(2) * v * Y));
+                acc_g(a.gc[2], q, l1 * (u * u * X + u * v * Y) + l2 * (v * u * X + v * v * Y));
+            } else {
+                const T l3 = lzz[q];
+                const T Z = cz * (fvz_c - fvz_m);
+                acc_g(a.gc[0], q, l1 * X);
+                acc_g(a.gc[1], q, l2 * Y);
+                acc_g(a.gc[2], q, l3 * Z);
+                acc_g(a.gc[3], q, l2 * Z + l3 * Y);
+                acc_g(a.gc[4], q, l1 * Z + l3 * X);
+                acc_g(a.gc[5], q, l1 * Y + l2 * X);
+            }
+        }
+        if (ix <= nx - 1 && iy <= ny - 1 && iz <= nz - 1)
+            acc_g(a.gc[6], q, lyz[q] * (T(0.5) * (cz * (fvy_n - fvy_c) + cy * (fvz[q + P] - fvz_c))));
+        if (iz <= nz - 1)
+            acc_g(a.gc[7], q, lxz[q] * (T(0.5) * (cz * (fvx_n - fvx_c) + cx * (fvz_c - fvz[q - 1]))));
+        if (iy <= ny - 1)
+            acc_g(a.gc[8], q, lxy[q] * (T(0.5) * (cy * (fvx[q + P] - fvx_c) + cx * (fvy_c - fvy[q - 1]))));
+        fvz_m = fvz_c;
+        fvx_c = fvx_n;
+        fvy_c = fvy_n;
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Transpose of UpdateVel (velocity stencil) on the stresses, with the gradient
-// w.r.t. buoyancy.  Forward:
+// Transpose of UpdateVel (velocity stencil) on the stresses (the gradient
+// w.r.t. buoyancy is k_adj_gradb's).  Forward:
 //   vx += b (cx dx sxx + cy dy sxy + cz dz sxz)                on the vx update set
 //   vy += mean_xy(b) (cx dx sxy + cz dz syz + cy dy syy)       on the vy update set
 //   vz += mean_xz(b) (cx dx sxz + cz dz szz + cy dy syz)       on the vz update set
@@ -265,20 +307,44 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
             }
             acc_st(a.lam[7], q, cz * (wx - wx_n) + cx * (wz_xm - wz));
             if (ix <= nx - 1 && iy <= ny - 1) acc_st(a.lam[6], q, cz * (wy - wy_n) + cy * (wz - wzp));
-            // gradient w.r.t. buoyancy at this velocity point, forward s^n
-            if (iz >= 0) {
-                const T *sxx = a.fs[0], *syy = a.fs[1], *szz = a.fs[2], *syz = a.fs[3], *sxz = a.fs[4], *sxy = a.fs[5];
-                const bool zin = iz >= a.top_eff && iz <= nz - a.bottom;
-                if (ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && zin)
-                    acc_st(a.gb[0], q, lvx[q] * (cx * (sxx[q] - sxx[q - 1]) + cy * (sxy[q] - sxy[q - P]) + cz * (sxz[q] - sxz[q - S2])));
-                if (ix <= nx - 1 && iy <= ny - 1 && zin)
-                    acc_st(a.gb[1], q, lvy[q] * (cx * (sxy[q + 1] - sxy[q]) + cz * (syz[q] - syz[q - S2]) + cy * (syy[q + P] - syy[q])));
-                if (ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && iz <= nz - 1)
-                    acc_st(a.gb[2], q, lvz[q] * (cx * (sxz[q + 1] - sxz[q]) + cz * (szz[q + S2] - szz[q]) + cy * (syz[q] - syz[q - P])));
-            }
         }
         wz_m = wz;
         wx = wx_n; wy = wy_n; wz = wz_n; wxp = wxp_n; wzp = wzp_n; wym = wym_n;
+    }
+}
+
+// Gradient w.r.t. buoyancy at the velocity points (vx node, vy xy mean, vz
+// xz mean): lam_v (after the transposed velocity mirrors) times the stress
+// differences of this step's forward stresses s^n.  Own sweep, like
+// k_adj_grad; lam_v is not written by k_adj_vel.
+template <typename T>
+__global__ __launch_bounds__(64 * GBY) void k_adj_gradb(AdjGridDev<T> a)
+{
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int gx = (nx + 64) / 64, gy = (ny + GBY) / GBY;
+    const int tile = xcd_tile(blockIdx.x, gx * gy * ((nz + GZC) / GZC));
+    const int bx = tile % gx, by = (tile / gx) % gy, bz = tile / (gx * gy);
+    const int ix = bx * 64 + threadIdx.x, iy = by * GBY + threadIdx.y;
+    if (ix > nx || iy > ny) return;
+    const int z0 = bz * GZC, z1 = min(z0 + GZC, nz + 1);
+    const long P = a.P, S2 = a.S2;
+    const T cx = a.cx, cy = a.cy, cz = a.cz;
+    const T *lvx = a.lam[0], *lvy = a.lam[1], *lvz = a.lam[2];
+    const T *sxx = a.fs[0], *syy = a.fs[1], *szz = a.fs[2], *syz = a.fs[3], *sxz = a.fs[4], *sxy = a.fs[5];
+    long q = a.o + (long)z0 * S2 + (long)iy * P + ix;
+    T sxz_m = sxz[q - S2], syz_m = syz[q - S2], szz_c = szz[q];
+    for (int iz = z0; iz < z1; iz++, q += S2) {
+        const T sxz_c = sxz[q], syz_c = syz[q], szz_n = szz[q + S2];
+        const bool zin = iz >= a.top_eff && iz <= nz - a.bottom;
+        if (ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && zin)
+            acc_g(a.gb[0], q, lvx[q] * (cx * (sxx[q] - sxx[q - 1]) + cy * (sxy[q] - sxy[q - P]) + cz * (sxz_c - sxz_m)));
+        if (ix <= nx - 1 && iy <= ny - 1 && zin)
+            acc_g(a.gb[1], q, lvy[q] * (cx * (sxy[q + 1] - sxy[q]) + cz * (syz_c - syz_m) + cy * (syy[q + P] - syy[q])));
+        if (ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && iz <= nz - 1)
+            acc_g(a.gb[2], q, lvz[q] * (cx * (sxz[q + 1] - sxz_c) + cz * (szz_n - szz_c) + cy * (syz_c - syz[q - P])));
+        sxz_m = sxz_c;
+        syz_m = syz_c;
+        szz_c = szz_n;
     }
 }
 

@@ -516,6 +516,7 @@ public:
 
     int set_params(int g, const double *buoy, const double *const stiff[9], int iso) override
     {
+        coef_ok = false;
         if (g < 0 || g >= (int)G.size()) return fail(ECSG_EINVAL, "grid index out of range");
         if (!buoy || !stiff) return fail(ECSG_EINVAL, "null parameter array");
         DevGrid<T> &d = G[g];
@@ -1657,6 +1658,7 @@ public:
         dfree(d_rloc); dfree(d_rstart); dfree(d_rrec); dfree(d_resid);
         ck_free();
         adj_ready = false;
+        coef_ok = false;
     }
 
     template <typename P>
@@ -1927,6 +1929,8 @@ public:
             launch_mirror(g, AG[g].sm);
             AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
             hipLaunchKernelGGL((k_adj_stress<T>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
+            const int gb = ((a.nx + 64) / 64) * ((a.ny + GBY) / GBY) * ((a.nz + GZC) / GZC);
+            hipLaunchKernelGGL((k_adj_grad<T>), dim3(gb), dim3(64, GBY), 0, s, a);
         }
         // vz-szz (tridiagonal)
         for (int k = ni - 1; k >= 0; k--) {
@@ -1951,14 +1955,20 @@ public:
             launch_mirror(g, AG[g].vB);
             AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
             hipLaunchKernelGGL((k_adj_vel<T>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
+            const int gb = ((a.nx + 64) / 64) * ((a.ny + GBY) / GBY) * ((a.nz + GZC) / GZC);
+            hipLaunchKernelGGL((k_adj_gradb<T>), dim3(gb), dim3(64, GBY), 0, s, a);
         }
         HIPCHK(hipGetLastError());
         return ECSG_OK;
     }
     int resid_steps = 0;
 
+    // masked coefficients: rebuilt only after the parameters change
+    bool coef_ok = false;
     int adj_coef()
     {
+        if (coef_ok) return ECSG_OK;
+        coef_ok = true;
         for (int g = 0; g < (int)G.size(); g++) {
             const DevGrid<T> &d = G[g];
             const long np = (long)(d.n[0] + 1) * (d.n[1] + 1) * (d.n[2] + 1);
