@@ -69,6 +69,7 @@ def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_step
     sim.sync()
     el = time.perf_counter() - t
     stats = [sim.jacobi_stats(k) for k in range(m.num_grids - 1)]
+    overlap = sim.overlap_mode
     kern = None
     if profile:
         # per-kernel durations: further steps with each launch bracketed by
@@ -77,7 +78,7 @@ def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_step
         sim.sync()
     tr = sim.traces()
     sim.close()
-    return el, kern, stats, tr
+    return el, kern, stats, tr, overlap
 
 
 def adjoint_run(m, steps: int, every: int, fmt: str, precision: str = "f32"):
@@ -173,16 +174,31 @@ def profiled_traffic(workload: str):
 
 
 def copy_calibration():
-    """A plain 1 GiB device-to-device copy (torch) so that FETCH_SIZE /
-    WRITE_SIZE of a kernel with known traffic can scale the stencil counters."""
+    """1 GiB device copies of known size, so that FETCH_SIZE / WRITE_SIZE of
+    the stencil kernels can be scaled: torch's copy (plain accesses), the
+    library's FP32 copy with the stencils' non-temporal loads and stores
+    (hints 3) and without them (hints 0), disjoint and in place (each value
+    read and written back, as the stencils update their fields); three
+    launches each."""
     import torch
+    import ecsg_amd
+    L = ecsg_amd.load_library()
     n = (1 << 30) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
+    for in_place in (False, True):
+        for hints in (0, 3):
+            for _ in range(3):
+                rc = L.ecsg_copy_probe(dst.data_ptr(), dst.data_ptr() if in_place else src.data_ptr(), n, hints, None)
+                if rc != 0:
+                    raise RuntimeError("ecsg_copy_probe failed (%d)" % rc)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+    assert bool(torch.equal(dst, src))
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3,
+                      "kernels": ["__amd_rocclr_copyBuffer", "k_copy_probe<0, false>", "k_copy_probe<3, false>",
+                                  "k_copy_probe<0, true>", "k_copy_probe<3, true>"]}), flush=True)
 
 
 def main():
@@ -192,7 +208,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="three_grid")
     ap.add_argument("--fp64-steps", type=int, default=500)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=50)
@@ -223,24 +239,30 @@ def main():
     log("[bench] model %s: %d grids, %.1f M nodes, dt=%.4g, built in %.1fs"
         % (m.name, m.num_grids, nodes / 1e6, m.dt, time.time() - t0))
 
-    el, kern, stats, _ = gpu_run(m, "f32", args.warmup, args.steps, profile=args.prof_steps > 0,
-                                 prof_steps=args.prof_steps, solver=args.solver)
+    el, kern, stats, _, overlap = gpu_run(m, "f32", args.warmup, args.steps, profile=args.prof_steps > 0,
+                                          prof_steps=args.prof_steps, solver=args.solver)
     gpts = nodes * args.steps / el / 1e9
     ms_step = el / args.steps * 1e3
     log("[bench] f32: %.3f s for %d steps -> %.2f Gpt/s, %.3f ms/step" % (el, args.steps, gpts, ms_step))
     ms, cnt = kern if kern else ([0.0] * 6, [0] * 6)
     g0 = m.grids[0]
-    n0 = g0.nodes
+    # the timed launches of grid 0: all its planes, or, with the interface
+    # overlap, its interior planes (velocities 0..nz-1; stresses 0..nz-1 in
+    # mode 1, 0..nz-2 in mode 2)
+    planes0 = {0: g0.n[2] + 1, 1: g0.n[2], 2: g0.n[2] - 1}[overlap]
+    planes0v = g0.n[2] + 1 if overlap == 0 else g0.n[2]
+    n0 = (g0.n[0] + 1) * (g0.n[1] + 1) * planes0
+    n0v = (g0.n[0] + 1) * (g0.n[1] + 1) * planes0v
     t_stress = ms[1] * 1e-3
     t_vel = ms[0] * 1e-3
     achieved = BYTES_STRESS_F32 * n0 / t_stress if t_stress > 0 else 0.0
-    vel_bw = BYTES_VEL_F32 * n0 / t_vel if t_vel > 0 else 0.0
+    vel_bw = BYTES_VEL_F32 * n0v / t_vel if t_vel > 0 else 0.0
     log("[bench] kernel avg ms: vel0 %.4f stress0 %.4f (launches %d/%d), interface %.4f (x%d), point %.4f"
         % (ms[0], ms[1], cnt[0], cnt[1], ms[4], cnt[4], ms[5]))
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
-        el64, kern64, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=args.prof_steps > 0,
+        el64, kern64, _, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=args.prof_steps > 0,
                                      prof_steps=args.prof_steps, solver=args.solver)
         g64 = nodes * args.fp64_steps / el64 / 1e9
         fp64 = {"value": round(g64, 3), "unit": "Gpt/s", "steps": args.fp64_steps,
@@ -297,11 +319,12 @@ def main():
                    "nodes": nodes, "ratio": m.ratios(), "receivers": int(len(m.rec_loc)),
                    "energy_check": False, "sxz_solver": args.solver, "parallelism": "single GPU"},
         "hbm_frac_step": round(BYTES_STEP_F32 * gpts * 1e9 / HBM_PEAK, 4),
-        "roofline": {"bound": "hbm", "kernel": "k_stress<float> on the %dx%dx%d-node grid" % tuple(x + 1 for x in g0.n),
+        "roofline": {"bound": "hbm", "kernel": "k_stress<float> on planes 0..%d of the %dx%dx%d-node grid"
+                     % ((planes0 - 1,) + tuple(x + 1 for x in g0.n)),
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": profiled_traffic(args.workload),
                      "bytes_per_launch": BYTES_STRESS_F32 * n0, "avg_launch_ms": round(ms[1], 5),
-                     "vel_kernel_GBps": round(vel_bw / 1e9, 1)},
+                     "vel_kernel_GBps": round(vel_bw / 1e9, 1), "interface_overlap": overlap},
         "fp64": fp64,
         "adjoint": adjoint,
         "batch": batch,

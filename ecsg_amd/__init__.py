@@ -34,7 +34,7 @@ ABI_SYMBOLS = (
     "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune", "ecsg_set_fused",
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
-    "ecsg_checkpoint_interval",
+    "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_set_overlap", "ecsg_overlap_active",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -94,6 +94,9 @@ def load_library(path: Optional[str] = None):
     L.ecsg_field_avgabs.argtypes = [vp, i, i, P(d)]
     L.ecsg_sbp_residuals.argtypes = [vp, i, P(d)]
     L.ecsg_step_n.argtypes = [vp, i, i]
+    L.ecsg_copy_probe.argtypes = [vp, vp, C.c_uint64, i, vp]
+    L.ecsg_set_overlap.argtypes = [vp, i]
+    L.ecsg_overlap_active.argtypes = [vp]
     L.ecsg_field_dims.argtypes = [vp, i, i, P(i)]
     L.ecsg_get_field.argtypes = [vp, i, i, P(d)]
     L.ecsg_set_field.argtypes = [vp, i, i, P(d)]
@@ -328,6 +331,21 @@ class Simulation:
     def set_fused(self, on: bool = True):
         """Fused velocity+stress pass in step() (identical results)."""
         self._chk(self.L.ecsg_set_fused(self.h, int(bool(on))))
+
+    def set_overlap(self, on: bool = True):
+        """Interface ghost updates on a second stream beside the stencil
+        planes away from the interfaces (default on; identical results)."""
+        self._chk(self.L.ecsg_set_overlap(self.h, int(bool(on))))
+
+    @property
+    def overlap_active(self) -> bool:
+        return self.overlap_mode > 0
+
+    @property
+    def overlap_mode(self) -> int:
+        """0: serial step; 1: interface overlap; 2: overlap with grid 0's
+        stress interior also beside the sxz/syz ghost solves."""
+        return int(self.L.ecsg_overlap_active(self.h))
 
     def autotune(self, reps: int = 5):
         """Per-grid stencil tiling chosen by timing (state preserved):

@@ -85,6 +85,8 @@ public:
     virtual int adjoint_stats(long long *sweeps, double *ms) = 0;
     virtual int autotune(int reps, int *by_out, int *zc_out) = 0;
     virtual int set_fused(int on) = 0;
+    virtual int set_overlap(int on) = 0;
+    virtual int overlap_active() const = 0;
 };
 
 #define HIPCHK(x)                                                                                  \
@@ -171,6 +173,13 @@ public:
     bool prof = false;
     std::vector<hipEvent_t> evs;
     std::vector<int> ev_kind;
+    // interface updates on a second stream beside the stencil planes that do
+    // not touch the interfaces (enqueue_step_overlap)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ov_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int overlap_on = 1;
+    bool ov_open = false;  // s has not yet waited for the last sxz/syz ghost update
+    std::vector<int> h_rec;  // receiver locations (grid, ix, iy, iz)
 
     ~Engine() override
     {
@@ -191,6 +200,9 @@ public:
         hipFree(d_src_old); hipFree(d_src_loc); hipFree(d_src_tr);
         adj_free();
         for (auto e : evs) hipEventDestroy(e);
+        for (auto e : ov_ev)
+            if (e) hipEventDestroy(e);
+        if (s2) hipStreamDestroy(s2);
         if (s) hipStreamDestroy(s);
     }
 
@@ -222,6 +234,11 @@ public:
         if (tile_by != 1 && tile_by != 2 && tile_by != 4 && tile_by != 8) return fail(ECSG_EINVAL, "tile height must be 1, 2, 4 or 8");
         if (tile_zc < 1) return fail(ECSG_EINVAL, "z chunk must be >= 1");
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        if (const char *e = getenv("ECSG_OVERLAP")) overlap_on = atoi(e) != 0;
+        if (c.num_grids > 1) {
+            HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            for (auto &e : ov_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
         G.resize(c.num_grids);
         for (int g = 0; g < c.num_grids; g++) {
             DevGrid<T> &d = G[g];
@@ -472,6 +489,8 @@ public:
         v.gy = (d.n[1] + 1 + by - 1) / by;
         const int gz = (d.n[2] + 1 + zc - 1) / zc;
         v.nblk = v.gx * v.gy * gz;
+        v.zb = 0;
+        v.ze = d.n[2] + 1;
         d.by = by;
     }
 
@@ -705,6 +724,7 @@ public:
         d_rec = nullptr; d_traces = nullptr;
         if (int rc = dalloc(&d_rec, std::max(4 * n, 4))) return rc;
         if (n) HIPCHK(hipMemcpy(d_rec, loc, sizeof(int) * 4 * n, hipMemcpyHostToDevice));
+        h_rec.assign(loc ? loc : nullptr, loc ? loc + 4 * n : nullptr);
         if (int rc = dalloc(&d_traces, (size_t)std::max(n, 1) * cfg.max_steps)) return rc;
         nrec = n;
         invalidate_graph();
@@ -721,6 +741,15 @@ public:
 
     // ----- fused step ------------------------------------------------------
     bool fused_ok() const { return use_fused && !cfg.check_energy; }
+    int set_overlap(int on) override
+    {
+        HIPCHK(hipStreamSynchronize(s));
+        overlap_on = on ? 1 : 0;
+        invalidate_graph();
+        return ECSG_OK;
+    }
+    int overlap_active() const override { return fused_ok() ? 0 : overlap_mode(); }
+
     int set_fused(int on) override
     {
         if (on && cfg.imode == 0 && !I.empty()) return fail(ECSG_EINVAL, "the fused pass runs the energy-conserving interface (imode 1) only");
@@ -924,20 +953,51 @@ public:
     }
 
     template <int BYv>
-    void vel_launch(DevGrid<T> &d, bool en, const int *srcp)
+    void vel_launch(const GridDev<T> &v, double *ke, bool en, const int *srcp)
     {
         if (en)
-            hipLaunchKernelGGL((k_vel<T, true, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.ke, srcp, d_src_old);
+            hipLaunchKernelGGL((k_vel<T, true, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, ke, srcp, d_src_old);
         else
-            hipLaunchKernelGGL((k_vel<T, false, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.ke, srcp, d_src_old);
+            hipLaunchKernelGGL((k_vel<T, false, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, ke, srcp, d_src_old);
     }
     template <int BYv>
-    void stress_launch(DevGrid<T> &d, bool en)
+    void stress_launch(const GridDev<T> &v, double *se, bool en)
     {
         if (en)
-            hipLaunchKernelGGL((k_stress<T, true, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.se);
+            hipLaunchKernelGGL((k_stress<T, true, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, se);
         else
-            hipLaunchKernelGGL((k_stress<T, false, BYv>), dim3(d.dev.nblk), dim3(BX, BYv), 0, s, d.dev, d.se);
+            hipLaunchKernelGGL((k_stress<T, false, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, se);
+    }
+    // radius-1 stencil of grid g over the z planes [zb, ze) only (no energy)
+    int stencil_part(int g, int zb, int ze, bool vel)
+    {
+        if (ze <= zb) return ECSG_OK;
+        DevGrid<T> &d = G[g];
+        GridDev<T> v = d.dev;
+        v.zb = zb;
+        v.ze = ze;
+        v.nblk = v.gx * v.gy * ((ze - zb + v.zc - 1) / v.zc);
+        ev_begin();
+        if (vel) {
+            switch (d.by) {
+            case 1: vel_launch<1>(v, d.ke, false, nullptr); break;
+            case 2: vel_launch<2>(v, d.ke, false, nullptr); break;
+            case 8: vel_launch<8>(v, d.ke, false, nullptr); break;
+            default: vel_launch<4>(v, d.ke, false, nullptr); break;
+            }
+        } else {
+            switch (d.by) {
+            case 1: stress_launch<1>(v, d.se, false); break;
+            case 2: stress_launch<2>(v, d.se, false); break;
+            case 8: stress_launch<8>(v, d.se, false); break;
+            default: stress_launch<4>(v, d.se, false); break;
+            }
+        }
+        // kinds 0 / 1: the interior launch of grid 0, the long one
+        const bool main_part = g == 0 && zb == 0;
+        ev_end(main_part ? (vel ? 0 : 1) : (vel ? 2 : 3));
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
     }
 
     int launch_vel(int g)
@@ -955,10 +1015,10 @@ public:
             return ECSG_OK;
         }
         switch (d.by) {
-        case 1: vel_launch<1>(d, en, srcp); break;
-        case 2: vel_launch<2>(d, en, srcp); break;
-        case 8: vel_launch<8>(d, en, srcp); break;
-        default: vel_launch<4>(d, en, srcp); break;
+        case 1: vel_launch<1>(d.dev, d.ke, en, srcp); break;
+        case 2: vel_launch<2>(d.dev, d.ke, en, srcp); break;
+        case 8: vel_launch<8>(d.dev, d.ke, en, srcp); break;
+        default: vel_launch<4>(d.dev, d.ke, en, srcp); break;
         }
         ev_end(g == 0 ? 0 : 2);
         HIPCHK(hipGetLastError());
@@ -977,10 +1037,10 @@ public:
             return ECSG_OK;
         }
         switch (d.by) {
-        case 1: stress_launch<1>(d, cfg.check_energy != 0); break;
-        case 2: stress_launch<2>(d, cfg.check_energy != 0); break;
-        case 8: stress_launch<8>(d, cfg.check_energy != 0); break;
-        default: stress_launch<4>(d, cfg.check_energy != 0); break;
+        case 1: stress_launch<1>(d.dev, d.se, cfg.check_energy != 0); break;
+        case 2: stress_launch<2>(d.dev, d.se, cfg.check_energy != 0); break;
+        case 8: stress_launch<8>(d.dev, d.se, cfg.check_energy != 0); break;
+        default: stress_launch<4>(d.dev, d.se, cfg.check_energy != 0); break;
         }
         ev_end(g == 0 ? 1 : 3);
         HIPCHK(hipGetLastError());
@@ -1202,9 +1262,136 @@ public:
         return ECSG_OK;
     }
 
-    // one time step, main.cpp:165-261
-    int enqueue_step()
+    bool overlap_ok() const
     {
+        return overlap_on && G.size() > 1 && cfg.radius == 1 && cfg.imode == 1 && !cfg.check_energy;
+    }
+    // the launches of a scope go to s2 (the interface stream) instead of s
+    struct OnSecond {
+        Engine *e;
+        explicit OnSecond(Engine *x) : e(x) { std::swap(e->s, e->s2); }
+        ~OnSecond() { std::swap(e->s, e->s2); }
+    };
+    // s waits for the sxz/syz ghosts of the last overlapped step
+    int ov_join()
+    {
+        if (!ov_open) return ECSG_OK;
+        ov_open = false;
+        HIPCHK(hipStreamWaitEvent(s, ov_ev[3], 0));
+        return ECSG_OK;
+    }
+
+    // planes [a, b) of grid g excluded: the planes of [0, nz] below a and
+    // from b on (all of them when a >= b)
+    int stencil_rest(int g, int a, int b, bool vel)
+    {
+        const int n1 = G[g].n[2] + 1;
+        if (a >= b) return stencil_part(g, 0, n1, vel);
+        if (int rc = stencil_part(g, 0, a, vel)) return rc;
+        return stencil_part(g, b, n1, vel);
+    }
+    // Planes of grid g away from its interfaces.  Velocity interior [vlo,
+    // vhi): every plane but z = 0 below an interface and z = nz above one.
+    // Stress interior [slo, shi): also without z = 1 below and z = nz-1
+    // above an interface (their stencils read the velocity edge planes).
+    int vlo(int g) const { return g > 0 ? 1 : 0; }
+    int vhi(int g) const { return g + 1 < (int)G.size() ? G[g].n[2] : G[g].n[2] + 1; }
+    int slo(int g) const { return g > 0 ? 2 : 0; }
+    int shi(int g) const { return g + 1 < (int)G.size() ? G[g].n[2] - 1 : G[g].n[2] + 1; }
+    // the source and every receiver lie on stress-interior planes: the
+    // receiver/source kernel may run before the velocity edge planes
+    bool points_interior() const
+    {
+        if (src_len > 0 && src_g >= 0 && (src_loc[2] < slo(src_g) || src_loc[2] >= shi(src_g))) return false;
+        for (size_t q = 0; q + 3 < h_rec.size(); q += 4)
+            if (h_rec[q + 3] < slo(h_rec[q]) || h_rec[q + 3] >= shi(h_rec[q])) return false;
+        return true;
+    }
+    int overlap_mode() const { return !overlap_ok() ? 0 : points_interior() ? 2 : 1; }
+
+    // One time step (main.cpp:165-261) with the interface updates on s2.
+    // An interface reads and writes, in its upper (fine) grid, only the
+    // planes z = nz-1 .. nz (and the ghost plane below), in its lower
+    // (coarse) grid only z = -1 .. 0; a stencil at plane z reads planes
+    // z-1 .. z+1.  So the velocity update of every plane but z = nz of an
+    // upper and z = 0 of a lower grid runs beside the previous step's
+    // sxz/syz ghost solves, and so does the stress update of grid 0's planes
+    // z = 0 .. nz-2 when the source and the receivers lie on such interior
+    // planes (mode 2: their kernel then runs right after the interior
+    // velocities); the other stress interiors run beside the vz ghost
+    // solves, and the planes next to each interface after the ghosts they
+    // read.  Every entry sees the same operations in the same order as in
+    // the serial step.  With keep_open the step ends without s waiting for
+    // its sxz/syz ghosts: the next step's interior velocities may start
+    // first.
+    int enqueue_step_overlap(bool keep_open)
+    {
+        const int ng = (int)G.size();
+        const bool early = overlap_mode() == 2;
+        // coarse grids first: their short launches run before the previous
+        // step's Jacobi solve is resident, which slows them several times
+        for (int g = ng - 1; g >= 0; g--)
+            if (int rc = stencil_part(g, vlo(g), vhi(g), true)) return rc;
+        if (early) {
+            if (int rc = launch_recsrc()) return rc;
+            if (int rc = stencil_part(0, slo(0), shi(0), false)) return rc;
+        }
+        if (int rc = ov_join()) return rc;
+        for (int g = 0; g < ng; g++)
+            if (int rc = stencil_rest(g, vlo(g), vhi(g), true)) return rc;
+        if (!early)
+            if (int rc = launch_recsrc()) return rc;
+        HIPCHK(hipEventRecord(ov_ev[0], s));
+        {
+            OnSecond on2(this);
+            HIPCHK(hipStreamWaitEvent(s, ov_ev[0], 0));
+            for (int k = 0; k + 1 < ng; k++)
+                if (int rc = launch_ghost_vz(k)) return rc;
+            HIPCHK(hipEventRecord(ov_ev[1], s));
+        }
+        for (int g = early ? 1 : 0; g < ng; g++)
+            if (int rc = stencil_part(g, slo(g), shi(g), false)) return rc;
+        HIPCHK(hipStreamWaitEvent(s, ov_ev[1], 0));
+        for (int g = 0; g < ng; g++)
+            if (int rc = stencil_rest(g, slo(g), shi(g), false)) return rc;
+        HIPCHK(hipEventRecord(ov_ev[2], s));
+        {
+            OnSecond on2(this);
+            HIPCHK(hipStreamWaitEvent(s, ov_ev[2], 0));
+            if (int rc = enqueue_ghost_stress()) return rc;
+            HIPCHK(hipEventRecord(ov_ev[3], s));
+        }
+        ov_open = true;
+        return keep_open ? ECSG_OK : ov_join();
+    }
+
+    // UpdateGhostSXZ of every interface (main.cpp:242-246): right-hand
+    // sides, then the Jacobi solves of up to MAXI interfaces side by side;
+    // then UpdateGhostSYZ
+    int enqueue_ghost_stress()
+    {
+        const int ng = (int)G.size();
+        if (ng > 1) {
+            ev_begin();
+            for (int k = 0; k + 1 < ng; k++)
+                if (int rc = launch_sxz_rhs(k)) return rc;
+            if (lds_ok && cfg.sxz_solver == 0) {
+                if (int rc = launch_jacobi_lds(false)) return rc;
+            } else {
+                for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
+                    if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
+            }
+            ev_end(4);
+        }
+        for (int k = 0; k + 1 < ng; k++)
+            if (int rc = launch_ghost_syz(k)) return rc;
+        return ECSG_OK;
+    }
+
+    // one time step, main.cpp:165-261
+    int enqueue_step(bool keep_open = false)
+    {
+        if (overlap_ok()) return enqueue_step_overlap(keep_open);
         const int ng = (int)G.size();
         for (int g = 0; g < ng; g++)
             if (int rc = launch_vel(g)) return rc;
@@ -1222,23 +1409,7 @@ public:
             }
             return ECSG_OK;
         }
-        // UpdateGhostSXZ of every interface (main.cpp:242-246): right-hand
-        // sides, then the Jacobi solves of up to MAXI interfaces side by side
-        if (ng > 1) {
-            ev_begin();
-            for (int k = 0; k + 1 < ng; k++)
-                if (int rc = launch_sxz_rhs(k)) return rc;
-            if (lds_ok && cfg.sxz_solver == 0) {
-                if (int rc = launch_jacobi_lds(false)) return rc;
-            } else {
-                for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
-                    if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
-            }
-            ev_end(4);
-        }
-        for (int k = 0; k + 1 < ng; k++)
-            if (int rc = launch_ghost_syz(k)) return rc;
-        return ECSG_OK;
+        return enqueue_ghost_stress();
     }
 
     // ----- per-phase entry points (the host drives the order) ---------------
@@ -1319,10 +1490,10 @@ public:
             HIPCHK(hipMemcpyAsync(save, d.arr(3), sizeof(T) * d.astride * 6, hipMemcpyDeviceToDevice, s));
             if (cfg.radius > 1) stress_radius(d, true);
             else switch (d.by) {
-            case 1: stress_launch<1>(d, true); break;
-            case 2: stress_launch<2>(d, true); break;
-            case 8: stress_launch<8>(d, true); break;
-            default: stress_launch<4>(d, true); break;
+            case 1: stress_launch<1>(d.dev, d.se, true); break;
+            case 2: stress_launch<2>(d.dev, d.se, true); break;
+            case 8: stress_launch<8>(d.dev, d.se, true); break;
+            default: stress_launch<4>(d.dev, d.se, true); break;
             }
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(d.arr(3), save, sizeof(T) * d.astride * 6, hipMemcpyDeviceToDevice, s));
@@ -1449,7 +1620,7 @@ public:
             }
         }
         for (int i = 0; i < n; i++)
-            if (int rc = enqueue_step()) return rc;
+            if (int rc = enqueue_step(i + 1 < n)) return rc;
         host_step += n;
         return ECSG_OK;
     }
@@ -1558,7 +1729,7 @@ public:
         if (host_step + nsteps > cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded");
         HIPCHK(hipStreamSynchronize(s));
         // pre-create the events: launches per step times two
-        size_t per_step = 2 * (2 * G.size() + 2 + 3 * I.size() + (cfg.check_energy ? 1 : 0));
+        size_t per_step = 2 * (6 * G.size() + 2 + 3 * I.size() + (cfg.check_energy ? 1 : 0));
         while (evs.size() < per_step * (size_t)nsteps) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
@@ -1569,7 +1740,7 @@ public:
         prof = true;
         if (fused_ok() && sets_dirty) sync_sets();
         for (int i = 0; i < nsteps; i++)
-            if (int rc = fused_ok() ? enqueue_fused_step() : enqueue_step()) { prof = false; return rc; }
+            if (int rc = fused_ok() ? enqueue_fused_step() : enqueue_step(i + 1 < nsteps)) { prof = false; return rc; }
         host_step += nsteps;
         prof = false;
         HIPCHK(hipStreamSynchronize(s));
@@ -2564,6 +2735,8 @@ public:
                 v.gx = (d.n[0] + 1 + BX - 1) / BX;
                 v.gy = (d.n[1] + 1 + by - 1) / by;
                 v.nblk = v.gx * v.gy * ((d.n[2] + 1 + zc - 1) / zc);
+                v.zb = 0;
+                v.ze = d.n[2] + 1;
                 gs.push_back(v);
                 st.push_back(acc);
                 acc += v.nblk;
@@ -2746,6 +2919,20 @@ static int ret(ecsg_ctx *c, int rc)
     return rc;
 }
 
+// Copy of known size for scaling the HBM byte counters; H = 3 uses the
+// stencils' non-temporal loads and stores, H = 0 plain ones.  With
+// InPlace (src == dst) every value is read and written back by one lane, the
+// access pattern of the stencils' updated fields.
+template <int H, bool InPlace>
+__global__ __launch_bounds__(256) void k_copy_probe(float *dst, const float *src, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if constexpr (H == 3) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+        else dst[i] = src[i];
+    }
+}
+
 extern "C" {
 
 const char *ecsg_version(void) { return "ecsg_hip 0.1 (gfx950)"; }
@@ -2797,7 +2984,24 @@ int ecsg_end_step(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->end_s
 int ecsg_compute_res(ecsg_ctx *ctx, int which, double *out) { CTX_CHECK(ctx); return ret(ctx, ctx->e->residual(which, out)); }
 int ecsg_field_avgabs(ecsg_ctx *ctx, int g, int sel, double *out) { CTX_CHECK(ctx); return ret(ctx, ctx->e->avg_abs(g, sel, out)); }
 int ecsg_sbp_residuals(ecsg_ctx *ctx, int g, double out[7]) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sbp(g, out)); }
+int ecsg_copy_probe(float *dst, const float *src, uint64_t n, int hints, void *stream)
+{
+    if (!dst || !src || (hints != 0 && hints != 3)) return ECSG_EINVAL;
+    if (n == 0) return ECSG_OK;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 256 * 64);
+    hipStream_t st = (hipStream_t)stream;
+    if (src == dst) {
+        if (hints == 3) k_copy_probe<3, true><<<blocks, 256, 0, st>>>(dst, src, n);
+        else k_copy_probe<0, true><<<blocks, 256, 0, st>>>(dst, src, n);
+    } else {
+        if (hints == 3) k_copy_probe<3, false><<<blocks, 256, 0, st>>>(dst, src, n);
+        else k_copy_probe<0, false><<<blocks, 256, 0, st>>>(dst, src, n);
+    }
+    return hipGetLastError() == hipSuccess ? ECSG_OK : ECSG_EHIP;
+}
 int ecsg_step_n(ecsg_ctx *ctx, int nsteps, int use_graph) { CTX_CHECK(ctx); return ret(ctx, ctx->e->step_n(nsteps, use_graph)); }
+int ecsg_set_overlap(ecsg_ctx *ctx, int on) { CTX_CHECK(ctx); return ret(ctx, ctx->e->set_overlap(on)); }
+int ecsg_overlap_active(const ecsg_ctx *ctx) { return ctx && ctx->e ? ctx->e->overlap_active() : 0; }
 int ecsg_steps_done(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->steps_done() : -1; }
 int ecsg_synchronize(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sync()); }
 int ecsg_field_dims(const ecsg_ctx *ctx, int g, int sel, int len[3])

@@ -43,6 +43,7 @@ struct GridDev {
     int iso;        // isotropic compliance formula for the energy
     int zy;         // last z plane on which vy is mirrored across the y walls
     int zc, gx, gy, nblk;  // tiling
+    int zb, ze;            // z planes [zb, ze) covered by a stencil launch
 };
 
 // Workgroup id -> tile id such that each XCD (ids dispatched round robin over
@@ -110,8 +111,8 @@ __device__ __forceinline__ void vel_tile(const GridDev<T> &g, const int tile, do
     const int ix = bx * BX + threadIdx.x;
     const int iy = by * BYv + threadIdx.y;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int z0 = bz * g.zc;
-    const int z1 = min(z0 + g.zc, nz + 1);
+    const int z0 = g.zb + bz * g.zc;
+    const int z1 = min(z0 + g.zc, g.ze);
     double acc = 0.0;
     if (ix <= nx && iy <= ny) {
         const long S2 = g.S2, P = g.P;
@@ -284,8 +285,8 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
     const int ix = bx * BX + threadIdx.x;
     const int iy = by * BYv + threadIdx.y;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int z0 = bz * g.zc;
-    const int z1 = min(z0 + g.zc, nz + 1);
+    const int z0 = g.zb + bz * g.zc;
+    const int z1 = min(z0 + g.zc, g.ze);
     double acc = 0.0;
     if (ix <= nx && iy <= ny) {
         const long S2 = g.S2, P = g.P;

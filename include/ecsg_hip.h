@@ -22,6 +22,8 @@
 #ifndef ECSG_HIP_H
 #define ECSG_HIP_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -160,11 +162,15 @@ int ecsg_get_energy(ecsg_ctx *ctx, double *out);
 int ecsg_jacobi_stats(ecsg_ctx *ctx, int k, long long *total, int *last, int *capped);
 
 /* Measurement hook for the benchmark: runs nsteps steps without a graph and
- * times each kernel launch with HIP events on the handle's stream.
+ * times each kernel launch with HIP events on the stream it runs on.
  * ms_out[kind] receives the average duration of kernel kind over the run and
  * count_out[kind] the number of launches: kind 0 = velocity stencil of grid 0,
  * 1 = stress stencil of grid 0, 2 = all velocity stencils, 3 = all stress
- * stencils, 4 = interface kernels, 5 = receivers/source/energy kernels. */
+ * stencils, 4 = interface kernels, 5 = receivers/source/energy kernels.
+ * With the interface overlap active (ecsg_overlap_active) kinds 0 and 1 are
+ * the launches over the interior planes of grid 0: velocities z = 0 .. nz-1,
+ * stresses z = 0 .. nz-1 (mode 1) or z = 0 .. nz-2 (mode 2); its other
+ * planes are launches of their own, counted in kinds 2 and 3 only. */
 int ecsg_profile_kernels(ecsg_ctx *ctx, int nsteps, double ms_out[6], int count_out[6]);
 
 /* --- Adjoint-state gradient.  No counterpart in the reference, which has no
@@ -213,6 +219,19 @@ int ecsg_set_adjoint_field(ecsg_ctx *ctx, int g, int sel, const double *in);
 /* Total Jacobi sweeps of the transposed vx-sxz solves since the last gradient. */
 int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps);
 
+/* Interface overlap (default on; no counterpart in the reference, whose
+ * step is serial): in ecsg_step_n, ecsg_profile_kernels and the gradient's
+ * forward steps of a composite model (radius 1, imode 1, energy check off)
+ * the interface ghost updates run on a second stream beside the stencil
+ * planes that do not touch the interfaces, and the planes next to each
+ * interface are updated after the ghosts they read.  Results are identical
+ * to the serial step.  ecsg_overlap_active returns the schedule the next
+ * steps use: 0 = serial; 1 = overlap; 2 = overlap with the stress interior
+ * of grid 0 also beside the sxz/syz ghost solves (the source and every
+ * receiver at least two planes away from each interface). */
+int ecsg_set_overlap(ecsg_ctx *ctx, int on);
+int ecsg_overlap_active(const ecsg_ctx *ctx);
+
 /* ecsg_step_n's velocity and stress updates in one fused pass (default off;
  * used when the energy check is off): the pass reads the state before the
  * step and writes the state after it into a second set of field arrays
@@ -245,6 +264,17 @@ int ecsg_batch_synchronize(ecsg_batch *b);
 /* one tiling for the whole batch, or the fastest of the candidates above */
 int ecsg_batch_set_tiling(ecsg_batch *b, int by, int zc);
 int ecsg_batch_autotune(ecsg_batch *b, int reps, int *by, int *zc);
+
+/* --- Measurement.  No counterpart in the reference. --- */
+/* Copies n FP32 values from src to dst (device pointers, disjoint) on stream
+ * (a hipStream_t, NULL = the null stream), one value per lane as the stencils
+ * access them: hints 0 = plain loads and stores, 3 = non-temporal loads and
+ * stores (the stencils' streaming hints).  src == dst reads and writes back
+ * every value in place; otherwise the ranges must not overlap.  A copy of
+ * known size scales the HBM byte counters (FETCH_SIZE / WRITE_SIZE) of
+ * kernels that use the same hints.  Asynchronous; returns ECSG_EINVAL for
+ * other hints or null pointers. */
+int ecsg_copy_probe(float *dst, const float *src, uint64_t n, int hints, void *stream);
 
 #ifdef __cplusplus
 }
