@@ -11,8 +11,10 @@ with all inputs resident in HBM.
 
 Output: one JSON line.  value = grid-point updates per second (sum over grids
 of nodes x steps / wall time).  roofline: the dominant kernel (stress stencil
-of the 1207x1207x81 grid) timed with HIP events on its own stream over the
-timed region, at 96 algorithmic bytes per node (24 FP32 array accesses).
+of the 1207x1207x81 grid; with the interface overlap, its launch over the
+planes away from the interface) timed with HIP events on its launch stream
+during the timed steps, at 96 algorithmic bytes per node (24 FP32 array
+accesses).
 cpu_baseline: the CPU restatement (oracle/, OpenMP) on the same workload for a
 bounded number of steps.
 
@@ -64,21 +66,21 @@ def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_step
     sim.step(warmup)
     sim.sync()
     log("[bench] %s warmup %d steps done" % (precision, warmup))
+    # the timed steps, with HIP events around the grid-0 stencil launches
     t = time.perf_counter()
-    sim.step(steps)
-    sim.sync()
+    live = sim.step_timed(steps)
     el = time.perf_counter() - t
     stats = [sim.jacobi_stats(k) for k in range(m.num_grids - 1)]
     overlap = sim.overlap_mode
-    kern = None
-    if profile:
-        # per-kernel durations: further steps with each launch bracketed by
-        # HIP events on the simulation's stream
-        kern = sim.profile(prof_steps)
+    if profile and prof_steps > 0:
+        # every kernel kind: further steps with each launch bracketed by events
+        ms, cnt = sim.profile(prof_steps)
         sim.sync()
+        log("[bench] %s profiled steps, kernel avg ms: vel0 %.4f stress0 %.4f, interface %.4f (x%d), point %.4f"
+            % (precision, ms[0], ms[1], ms[4], cnt[4], ms[5]))
     tr = sim.traces()
     sim.close()
-    return el, kern, stats, tr, overlap
+    return el, live, stats, tr, overlap
 
 
 def adjoint_run(m, steps: int, every: int, fmt: str, precision: str = "f32"):
@@ -211,7 +213,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--prof-steps", type=int, default=50)
+    ap.add_argument("--prof-steps", type=int, default=0,
+                    help="extra steps timing every kernel kind with events (log only)")
     ap.add_argument("--solver", default="jacobi", choices=["jacobi", "chebyshev"],
                     help="vx-sxz interface solver (jacobi = the reference's sweeps)")
     ap.add_argument("--adjoint-steps", type=int, default=100,
@@ -239,12 +242,11 @@ def main():
     log("[bench] model %s: %d grids, %.1f M nodes, dt=%.4g, built in %.1fs"
         % (m.name, m.num_grids, nodes / 1e6, m.dt, time.time() - t0))
 
-    el, kern, stats, _, overlap = gpu_run(m, "f32", args.warmup, args.steps, profile=args.prof_steps > 0,
+    el, live, stats, _, overlap = gpu_run(m, "f32", args.warmup, args.steps, profile=args.prof_steps > 0,
                                           prof_steps=args.prof_steps, solver=args.solver)
     gpts = nodes * args.steps / el / 1e9
     ms_step = el / args.steps * 1e3
     log("[bench] f32: %.3f s for %d steps -> %.2f Gpt/s, %.3f ms/step" % (el, args.steps, gpts, ms_step))
-    ms, cnt = kern if kern else ([0.0] * 6, [0] * 6)
     g0 = m.grids[0]
     # the timed launches of grid 0: all its planes, or, with the interface
     # overlap, its interior planes (velocities 0..nz-1; stresses 0..nz-1 in
@@ -253,23 +255,20 @@ def main():
     planes0v = g0.n[2] + 1 if overlap == 0 else g0.n[2]
     n0 = (g0.n[0] + 1) * (g0.n[1] + 1) * planes0
     n0v = (g0.n[0] + 1) * (g0.n[1] + 1) * planes0v
-    t_stress = ms[1] * 1e-3
-    t_vel = ms[0] * 1e-3
+    t_vel, t_stress = live[0] * 1e-3, live[1] * 1e-3
     achieved = BYTES_STRESS_F32 * n0 / t_stress if t_stress > 0 else 0.0
     vel_bw = BYTES_VEL_F32 * n0v / t_vel if t_vel > 0 else 0.0
-    log("[bench] kernel avg ms: vel0 %.4f stress0 %.4f (launches %d/%d), interface %.4f (x%d), point %.4f"
-        % (ms[0], ms[1], cnt[0], cnt[1], ms[4], cnt[4], ms[5]))
+    log("[bench] timed steps, grid-0 launches avg ms: vel %.4f stress %.4f" % live)
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
-        el64, kern64, _, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=args.prof_steps > 0,
-                                     prof_steps=args.prof_steps, solver=args.solver)
+        el64, live64, _, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=args.prof_steps > 0,
+                                        prof_steps=args.prof_steps, solver=args.solver)
         g64 = nodes * args.fp64_steps / el64 / 1e9
         fp64 = {"value": round(g64, 3), "unit": "Gpt/s", "steps": args.fp64_steps,
                 "ms_per_step": round(el64 / args.fp64_steps * 1e3, 4),
                 "hbm_frac_step": round(2 * BYTES_STEP_F32 * g64 * 1e9 / HBM_PEAK, 4),
-                "stress0_ms": round(kern64[0][1], 5) if kern64 else None,
-                "vel0_ms": round(kern64[0][0], 5) if kern64 else None}
+                "stress0_ms": round(live64[1], 5), "vel0_ms": round(live64[0], 5)}
         log("[bench] f64: %.2f Gpt/s" % g64)
 
     adjoint = None
@@ -323,7 +322,7 @@ def main():
                      % ((planes0 - 1,) + tuple(x + 1 for x in g0.n)),
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": profiled_traffic(args.workload),
-                     "bytes_per_launch": BYTES_STRESS_F32 * n0, "avg_launch_ms": round(ms[1], 5),
+                     "bytes_per_launch": BYTES_STRESS_F32 * n0, "avg_launch_ms": round(live[1], 5),
                      "vel_kernel_GBps": round(vel_bw / 1e9, 1), "interface_overlap": overlap},
         "fp64": fp64,
         "adjoint": adjoint,

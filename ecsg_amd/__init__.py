@@ -34,7 +34,7 @@ ABI_SYMBOLS = (
     "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune", "ecsg_set_fused",
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
-    "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_set_overlap", "ecsg_overlap_active",
+    "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -97,6 +97,7 @@ def load_library(path: Optional[str] = None):
     L.ecsg_copy_probe.argtypes = [vp, vp, C.c_uint64, i, vp]
     L.ecsg_set_overlap.argtypes = [vp, i]
     L.ecsg_overlap_active.argtypes = [vp]
+    L.ecsg_step_n_timed.argtypes = [vp, i, P(d)]
     L.ecsg_field_dims.argtypes = [vp, i, i, P(i)]
     L.ecsg_get_field.argtypes = [vp, i, i, P(d)]
     L.ecsg_set_field.argtypes = [vp, i, i, P(d)]
@@ -275,6 +276,14 @@ class Simulation:
         self._chk(self.L.ecsg_jacobi_stats(self.h, k, C.byref(t), C.byref(l), C.byref(c)))
         return t.value, l.value, c.value
 
+    def step_timed(self, nsteps: int):
+        """nsteps steps (no graph) timing the grid-0 velocity and stress
+        stencil launches with HIP events; returns their average ms and
+        blocks until done."""
+        ms = (C.c_double * 2)()
+        self._chk(self.L.ecsg_step_n_timed(self.h, int(nsteps), ms))
+        return ms[0], ms[1]
+
     def profile(self, nsteps: int):
         ms = (C.c_double * 6)()
         cnt = (C.c_int * 6)()
@@ -332,10 +341,11 @@ class Simulation:
         """Fused velocity+stress pass in step() (identical results)."""
         self._chk(self.L.ecsg_set_fused(self.h, int(bool(on))))
 
-    def set_overlap(self, on: bool = True):
+    def set_overlap(self, on=True):
         """Interface ghost updates on a second stream beside the stencil
-        planes away from the interfaces (default on; identical results)."""
-        self._chk(self.L.ecsg_set_overlap(self.h, int(bool(on))))
+        planes away from the interfaces (identical results).  True / False,
+        or None for the default: on when the finest grid has >= 2^23 nodes."""
+        self._chk(self.L.ecsg_set_overlap(self.h, -1 if on is None else int(bool(on))))
 
     @property
     def overlap_active(self) -> bool:

@@ -75,6 +75,7 @@ public:
     virtual int get_energy(double *out) = 0;
     virtual int jacobi_stats(int k, long long *tot, int *last, int *capped) = 0;
     virtual int profile(int nsteps, double ms[6], int cnt[6]) = 0;
+    virtual int step_timed(int nsteps, double ms[2]) = 0;
     virtual int reset() = 0;
     virtual int set_checkpointing(int every, int format) = 0;
     virtual int checkpoint_interval() const = 0;
@@ -171,13 +172,14 @@ public:
     bool graph_failed = false;
     // profiling
     bool prof = false;
+    bool live = false;  // step_timed: events around the grid-0 stencil launches only
     std::vector<hipEvent_t> evs;
     std::vector<int> ev_kind;
     // interface updates on a second stream beside the stencil planes that do
     // not touch the interfaces (enqueue_step_overlap)
     hipStream_t s2 = nullptr;
     hipEvent_t ov_ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    int overlap_on = 1;
+    int overlap_on = -1;  // 1 on, 0 off, -1 by size (OVERLAP_MIN_NODES)
     bool ov_open = false;  // s has not yet waited for the last sxz/syz ghost update
     std::vector<int> h_rec;  // receiver locations (grid, ix, iy, iz)
 
@@ -234,11 +236,7 @@ public:
         if (tile_by != 1 && tile_by != 2 && tile_by != 4 && tile_by != 8) return fail(ECSG_EINVAL, "tile height must be 1, 2, 4 or 8");
         if (tile_zc < 1) return fail(ECSG_EINVAL, "z chunk must be >= 1");
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        if (const char *e = getenv("ECSG_OVERLAP")) overlap_on = atoi(e) != 0;
-        if (c.num_grids > 1) {
-            HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-            for (auto &e : ov_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        }
+        if (const char *e = getenv("ECSG_OVERLAP")) overlap_on = atoi(e) != 0 ? 1 : 0;
         G.resize(c.num_grids);
         for (int g = 0; g < c.num_grids; g++) {
             DevGrid<T> &d = G[g];
@@ -744,7 +742,9 @@ public:
     int set_overlap(int on) override
     {
         HIPCHK(hipStreamSynchronize(s));
-        overlap_on = on ? 1 : 0;
+        overlap_on = on < 0 ? -1 : on ? 1 : 0;
+        if (overlap_ok())
+            if (int rc = ensure_s2()) return rc;
         invalidate_graph();
         return ECSG_OK;
     }
@@ -940,14 +940,15 @@ public:
         ev_next++;
         return e;
     }
-    void ev_begin()
+    // mark: time this launch also outside profile() (step_timed)
+    void ev_begin(bool mark = false)
     {
-        if (!prof) return;
+        if (!prof && !mark) return;
         hipEventRecord(next_event(), s);
     }
-    void ev_end(int kind)
+    void ev_end(int kind, bool mark = false)
     {
-        if (!prof) return;
+        if (!prof && !mark) return;
         hipEventRecord(next_event(), s);
         ev_kind.push_back(kind);
     }
@@ -977,7 +978,9 @@ public:
         v.zb = zb;
         v.ze = ze;
         v.nblk = v.gx * v.gy * ((ze - zb + v.zc - 1) / v.zc);
-        ev_begin();
+        // kinds 0 / 1: the interior launch of grid 0, the long one
+        const bool main_part = g == 0 && zb == 0;
+        ev_begin(live && main_part);
         if (vel) {
             switch (d.by) {
             case 1: vel_launch<1>(v, d.ke, false, nullptr); break;
@@ -993,9 +996,7 @@ public:
             default: stress_launch<4>(v, d.se, false); break;
             }
         }
-        // kinds 0 / 1: the interior launch of grid 0, the long one
-        const bool main_part = g == 0 && zb == 0;
-        ev_end(main_part ? (vel ? 0 : 1) : (vel ? 2 : 3));
+        ev_end(main_part ? (vel ? 0 : 1) : (vel ? 2 : 3), live && main_part);
         HIPCHK(hipGetLastError());
         return ECSG_OK;
     }
@@ -1005,12 +1006,13 @@ public:
         DevGrid<T> &d = G[g];
         const bool en = cfg.check_energy != 0;
         const int *srcp = (en && src_type == 0 && g == src_g && src_len > 0) ? d_src_loc : nullptr;
-        ev_begin();
+        const bool mark = live && g == 0;
+        ev_begin(mark);
         if (cfg.radius > 1) {
             vel_radius(d, en, srcp);
             mirror_launch(d, d.mva);  // z and x walls
             mirror_launch(d, d.mvb);  // y walls, after the x mirrors read the old y-halo row
-            ev_end(g == 0 ? 0 : 2);
+            ev_end(g == 0 ? 0 : 2, mark);
             HIPCHK(hipGetLastError());
             return ECSG_OK;
         }
@@ -1020,7 +1022,7 @@ public:
         case 8: vel_launch<8>(d.dev, d.ke, en, srcp); break;
         default: vel_launch<4>(d.dev, d.ke, en, srcp); break;
         }
-        ev_end(g == 0 ? 0 : 2);
+        ev_end(g == 0 ? 0 : 2, mark);
         HIPCHK(hipGetLastError());
         return ECSG_OK;
     }
@@ -1028,11 +1030,12 @@ public:
     int launch_stress(int g)
     {
         DevGrid<T> &d = G[g];
-        ev_begin();
+        const bool mark = live && g == 0;
+        ev_begin(mark);
         if (cfg.radius > 1) {
             stress_radius(d, cfg.check_energy != 0);
             mirror_launch(d, d.mst);
-            ev_end(g == 0 ? 1 : 3);
+            ev_end(g == 0 ? 1 : 3, mark);
             HIPCHK(hipGetLastError());
             return ECSG_OK;
         }
@@ -1042,7 +1045,7 @@ public:
         case 8: stress_launch<8>(d.dev, d.se, cfg.check_energy != 0); break;
         default: stress_launch<4>(d.dev, d.se, cfg.check_energy != 0); break;
         }
-        ev_end(g == 0 ? 1 : 3);
+        ev_end(g == 0 ? 1 : 3, mark);
         HIPCHK(hipGetLastError());
         return ECSG_OK;
     }
@@ -1259,12 +1262,21 @@ public:
     int check_ready()
     {
         if (!all_params()) return fail(ECSG_ESTATE, "parameters not set for every grid");
-        return ECSG_OK;
+        // a multi-step call that stopped on an error may have left the last
+        // ghost update unjoined
+        return ov_join();
     }
 
+    // Below this many nodes in the finest grid the overlap's extra launches
+    // and cross-stream waits cost more than the interface latency they hide
+    // (64 two-grid members of 0.4-3.5 M nodes, stepped one at a time with
+    // graphs, ran at half speed with it).
+    static constexpr long OVERLAP_MIN_NODES = 1L << 23;
     bool overlap_ok() const
     {
-        return overlap_on && G.size() > 1 && cfg.radius == 1 && cfg.imode == 1 && !cfg.check_energy;
+        if (G.size() < 2 || cfg.radius != 1 || cfg.imode != 1 || cfg.check_energy) return false;
+        if (overlap_on >= 0) return overlap_on != 0;
+        return (long)(G[0].n[0] + 1) * (G[0].n[1] + 1) * (G[0].n[2] + 1) >= OVERLAP_MIN_NODES;
     }
     // the launches of a scope go to s2 (the interface stream) instead of s
     struct OnSecond {
@@ -1272,6 +1284,16 @@ public:
         explicit OnSecond(Engine *x) : e(x) { std::swap(e->s, e->s2); }
         ~OnSecond() { std::swap(e->s, e->s2); }
     };
+    // the interface stream and its events, created when first needed (an
+    // idle second stream per handle would take a hardware queue from the
+    // handles that step side by side)
+    int ensure_s2()
+    {
+        if (s2) return ECSG_OK;
+        HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        for (auto &e : ov_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return ECSG_OK;
+    }
     // s waits for the sxz/syz ghosts of the last overlapped step
     int ov_join()
     {
@@ -1326,6 +1348,7 @@ public:
     // first.
     int enqueue_step_overlap(bool keep_open)
     {
+        if (int rc = ensure_s2()) return rc;
         const int ng = (int)G.size();
         const bool early = overlap_mode() == 2;
         // coarse grids first: their short launches run before the previous
@@ -1589,6 +1612,8 @@ public:
         if (n < 0) return fail(ECSG_EINVAL, "negative step count");
         if (host_step + n > cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded");
         if (fused_ok()) return step_fused(n, use_graph);
+        if (overlap_ok())  // before any capture
+            if (int rc = ensure_s2()) return rc;
         // The tiled interface Jacobi is a cooperative launch; its behaviour
         // inside a captured graph is not relied on, so composite models with
         // interfaces too large for the LDS solver run eagerly.
@@ -1629,6 +1654,7 @@ public:
 
     int sync() override
     {
+        if (int rc = ov_join()) return rc;
         HIPCHK(hipStreamSynchronize(s));
         for (int k = 0; k < (int)I.size(); k++) {
             long long st3[3];
@@ -1754,6 +1780,42 @@ public:
             if (kind == 1) { ms[3] += t; cnt[3]++; }
         }
         for (int k = 0; k < 6; k++) ms[k] = cnt[k] ? ms[k] / cnt[k] : 0;
+        ev_kind.clear();
+        ev_next = 0;
+        return ECSG_OK;
+    }
+
+    // nsteps steps as step_n without a graph, with HIP events (on the
+    // launch stream) around the grid-0 velocity and stress stencil launches
+    // only; ms: their average durations.  Blocks until the steps are done.
+    int step_timed(int nsteps, double ms[2]) override
+    {
+        if (int rc = check_ready()) return rc;
+        if (nsteps < 0) return fail(ECSG_EINVAL, "negative step count");
+        if (host_step + nsteps > cfg.max_steps) return fail(ECSG_EINVAL, "max_steps exceeded");
+        if (fused_ok()) return fail(ECSG_EINVAL, "step_timed runs the separate stencil passes (fused pass on)");
+        HIPCHK(hipStreamSynchronize(s));
+        while (evs.size() < (size_t)4 * nsteps) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            evs.push_back(e);
+        }
+        ev_next = 0;
+        ev_kind.clear();
+        live = true;
+        for (int i = 0; i < nsteps; i++)
+            if (int rc = enqueue_step(i + 1 < nsteps)) { live = false; return rc; }
+        live = false;
+        host_step += nsteps;
+        if (int rc = sync()) return rc;
+        double sum[2] = {0, 0};
+        int cnt[2] = {0, 0};
+        for (size_t q = 0; q < ev_kind.size(); q++) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, evs[2 * q], evs[2 * q + 1]));
+            if (ev_kind[q] == 0 || ev_kind[q] == 1) { sum[ev_kind[q]] += t; cnt[ev_kind[q]]++; }
+        }
+        for (int k = 0; k < 2; k++) ms[k] = cnt[k] ? sum[k] / cnt[k] : 0;
         ev_kind.clear();
         ev_next = 0;
         return ECSG_OK;
@@ -2363,12 +2425,12 @@ public:
             if (!last) {
                 if (int rc = ck_save(j)) return rc;
                 for (int t = 0; t < len; t++)
-                    if (int rc = enqueue_step()) return rc;
+                    if (int rc = enqueue_step(t + 1 < len)) return rc;
             } else {
                 if (int rc = ring_save(0)) return rc;
                 for (int t = 0; t < len; t++) {
                     set_store(t + 1);
-                    const int rc = enqueue_step();
+                    const int rc = enqueue_step(t + 1 < len);
                     set_store(-1);
                     if (rc) return rc;
                 }
@@ -2400,7 +2462,7 @@ public:
                 for (int t = 0; t < len; t++) {
                     // the step's kernels also write every value into ring slot t+1
                     set_store(t + 1);
-                    const int rc = enqueue_step();
+                    const int rc = enqueue_step(t + 1 < len);
                     set_store(-1);
                     if (rc) return rc;
                 }
@@ -3001,6 +3063,12 @@ int ecsg_copy_probe(float *dst, const float *src, uint64_t n, int hints, void *s
 }
 int ecsg_step_n(ecsg_ctx *ctx, int nsteps, int use_graph) { CTX_CHECK(ctx); return ret(ctx, ctx->e->step_n(nsteps, use_graph)); }
 int ecsg_set_overlap(ecsg_ctx *ctx, int on) { CTX_CHECK(ctx); return ret(ctx, ctx->e->set_overlap(on)); }
+int ecsg_step_n_timed(ecsg_ctx *ctx, int nsteps, double ms_out[2])
+{
+    CTX_CHECK(ctx);
+    if (!ms_out) return ECSG_EINVAL;
+    return ret(ctx, ctx->e->step_timed(nsteps, ms_out));
+}
 int ecsg_overlap_active(const ecsg_ctx *ctx) { return ctx && ctx->e ? ctx->e->overlap_active() : 0; }
 int ecsg_steps_done(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->steps_done() : -1; }
 int ecsg_synchronize(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sync()); }

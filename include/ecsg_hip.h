@@ -173,6 +173,13 @@ int ecsg_jacobi_stats(ecsg_ctx *ctx, int k, long long *total, int *last, int *ca
  * planes are launches of their own, counted in kinds 2 and 3 only. */
 int ecsg_profile_kernels(ecsg_ctx *ctx, int nsteps, double ms_out[6], int count_out[6]);
 
+/* Measurement hook for the benchmark's timed region: runs nsteps steps as
+ * ecsg_step_n without a graph, with HIP events on the launch stream around
+ * the grid-0 velocity and stress stencil launches only (the kinds 0 and 1 of
+ * ecsg_profile_kernels); ms_out receives their average durations.  Blocks
+ * until the steps are done. */
+int ecsg_step_n_timed(ecsg_ctx *ctx, int nsteps, double ms_out[2]);
+
 /* --- Adjoint-state gradient.  No counterpart in the reference, which has no
  * inversion path; built on the same time step as the entry points above. ---
  *
@@ -219,16 +226,17 @@ int ecsg_set_adjoint_field(ecsg_ctx *ctx, int g, int sel, const double *in);
 /* Total Jacobi sweeps of the transposed vx-sxz solves since the last gradient. */
 int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps);
 
-/* Interface overlap (default on; no counterpart in the reference, whose
- * step is serial): in ecsg_step_n, ecsg_profile_kernels and the gradient's
- * forward steps of a composite model (radius 1, imode 1, energy check off)
- * the interface ghost updates run on a second stream beside the stencil
- * planes that do not touch the interfaces, and the planes next to each
- * interface are updated after the ghosts they read.  Results are identical
- * to the serial step.  ecsg_overlap_active returns the schedule the next
- * steps use: 0 = serial; 1 = overlap; 2 = overlap with the stress interior
- * of grid 0 also beside the sxz/syz ghost solves (the source and every
- * receiver at least two planes away from each interface). */
+/* Interface overlap (no counterpart in the reference, whose step is
+ * serial): in ecsg_step_n, ecsg_profile_kernels and the gradient's forward
+ * steps of a composite model (radius 1, imode 1, energy check off) the
+ * interface ghost updates run on a second stream beside the stencil planes
+ * that do not touch the interfaces, and the planes next to each interface
+ * are updated after the ghosts they read.  Results are identical to the
+ * serial step.  on: 1 = on, 0 = off, -1 (default) = on when the finest grid
+ * has at least 2^23 nodes.  ecsg_overlap_active returns the schedule the
+ * next steps use: 0 = serial; 1 = overlap; 2 = overlap with the stress
+ * interior of grid 0 also beside the sxz/syz ghost solves (the source and
+ * every receiver at least two planes away from each interface). */
 int ecsg_set_overlap(ecsg_ctx *ctx, int on);
 int ecsg_overlap_active(const ecsg_ctx *ctx);
 

@@ -113,6 +113,8 @@ def test_overlap_modes():
 
 def _mode(m):
     s = ecsg_amd.Simulation(m, precision="f32", check_energy=False)
+    assert s.overlap_mode == 0  # by default only for finest grids of 2^23 nodes or more
+    s.set_overlap(True)
     k = s.overlap_mode
     s.close()
     return k
@@ -121,11 +123,26 @@ def _mode(m):
 def test_overlap_not_active_where_not_applicable():
     m = composite((3,), coarse=(6, 5, 4), nz_fine=(5,), nsteps=5, seed=4)
     s = ecsg_amd.Simulation(m, precision="f64", check_energy=True)
+    s.set_overlap(True)
     assert not s.overlap_active  # the energy check runs the serial step
     s.close()
     m1 = composite((), coarse=(8, 7, 6), nz_fine=(), nsteps=5, seed=1)
     s = ecsg_amd.Simulation(m1, precision="f64", check_energy=False)
+    s.set_overlap(True)
     assert not s.overlap_active  # one grid: no interfaces
+    s.close()
+
+
+def test_overlap_by_size():
+    """Default: on for a finest grid of 2^23 nodes or more (state only, no steps)."""
+    m = composite((3,), coarse=(100, 100, 3), nz_fine=(93,), nsteps=2, seed=12)
+    assert (m.grids[0].n[0] + 1) * (m.grids[0].n[1] + 1) * (m.grids[0].n[2] + 1) >= 1 << 23
+    s = ecsg_amd.Simulation(m, precision="f32", check_energy=False)
+    assert s.overlap_active
+    s.set_overlap(False)
+    assert not s.overlap_active
+    s.set_overlap(None)
+    assert s.overlap_active
     s.close()
 
 
@@ -145,3 +162,24 @@ def test_gradient_with_overlap_equals_serial():
     assert res[0][0] == res[1][0] and res[0][0] > 0
     for a, b in zip(res[0][1], res[1][1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("overlap", [False, True])
+def test_step_timed_equals_step(overlap):
+    """ecsg_step_n_timed advances the state as ecsg_step_n and times the
+    grid-0 stencil launches."""
+    nsteps = 12
+    m = _model("r2r3", nsteps)
+    outs = []
+    for timed in (False, True):
+        s = ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=nsteps + 2)
+        s.set_overlap(overlap)
+        if timed:
+            v, st = s.step_timed(nsteps)
+            assert v > 0 and st > 0
+        else:
+            s.step(nsteps)
+        s.sync()
+        outs.append(([s.get(g, c) for g in range(m.num_grids) for c in range(9)], s.traces()))
+        s.close()
+    _same(outs[1], outs[0], "timed")
