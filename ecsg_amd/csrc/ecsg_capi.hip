@@ -317,7 +317,7 @@ public:
                 return fail(ECSG_EINVAL, "interface " + std::to_string(k) + " too large for the resident Jacobi solve");
             f.coop = (int)std::max(1L, want);
             int pert = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pert, (const void *)k_jacobi_tiled<T>, JT, 0));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pert, (const void *)k_jacobi_tiled<T>, JTT, 0));
             max_tiled = pert * ncu;
             f.tiles = ((f.nx2 - 1 + TJ - 1) / TJ) * ((f.ny2 - 1 + TJ - 1) / TJ);
             const size_t np = std::max(4 * (size_t)f.coop + 1, (size_t)2 * f.tiles * 2 * SJ);
@@ -1163,7 +1163,7 @@ public:
         if (cfg.sxz_solver == 0) {
             // the reference's Jacobi sweeps, SJ per grid barrier in LDS tiles
             if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
-            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JT), args, 0, s));
+            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JTT), args, 0, s));
         } else {
             if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
             HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, 0, s));
@@ -2116,7 +2116,7 @@ public:
         jb.solve_only = 1;
         if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
         void *args[] = {(void *)&jb};
-        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JT), args, 0, s));
+        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JTT), args, 0, s));
         return ECSG_OK;
     }
 

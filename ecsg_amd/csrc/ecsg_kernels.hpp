@@ -975,16 +975,20 @@ constexpr int TJ = 32;                   // tile edge (unknowns)
 constexpr int SJ = ECSG_SJ;               // sweeps per grid barrier
 constexpr int EJ = TJ + 2 * SJ;          // LDS edge of tile + halo
 constexpr int RJ = TJ + 2 * (SJ - 1);    // edge of the first sweep's region
-constexpr int KJ = (RJ * RJ + JT - 1) / JT;  // region points per thread
+#ifndef ECSG_JTT
+#define ECSG_JTT 256
+#endif
+constexpr int JTT = ECSG_JTT;             // threads per tiled-Jacobi workgroup
+constexpr int KJ = (RJ * RJ + JTT - 1) / JTT;  // region points per thread
 
 template <typename T>
-__global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
+__global__ __launch_bounds__(JTT) void k_jacobi_tiled(JacBatch<T> jb)
 {
     namespace cg = cooperative_groups;
     cg::grid_group grid = cg::this_grid();
     __shared__ double xb[2][EJ * EJ];
     __shared__ double stash[SJ][TJ * TJ];
-    __shared__ double red[JT / 64][2 * SJ];
+    __shared__ double red[JTT / 64][2 * SJ];
     __shared__ double s_tol[MAXI];
     __shared__ double s_n2[MAXI][SJ], s_x2[MAXI][SJ];
     int me = 0;
@@ -1016,7 +1020,7 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
     int ti[KJ];          // tile index, -1 outside the tile
 #pragma unroll
     for (int k = 0; k < KJ; k++) {
-        const int p = tid + k * JT;
+        const int p = tid + k * JTT;
         li[k] = -1; sh[k] = 0; ti[k] = -1;
         if (p < RJ * RJ) {
             const int ri = p % RJ, rj = p / RJ;
@@ -1048,7 +1052,7 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
             if (k == me) active = !done_k[k];
         if (active) {
             // load tile + halo of the current iterate (zero outside the unknowns)
-            for (int p = tid; p < EJ * EJ; p += JT) {
+            for (int p = tid; p < EJ * EJ; p += JTT) {
                 const int ei = p % EJ, ej = p / EJ;
                 const int I = I0 - SJ + ei, Jg = J0 - SJ + ej;
                 double v = 0;
@@ -1101,11 +1105,11 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
             __syncthreads();
             if (tid < 2 * SJ) {
                 double v = 0;
-                for (int q = 0; q < JT / 64; q++) v += red[q][tid];
+                for (int q = 0; q < JTT / 64; q++) v += red[q][tid];
                 J.part[((long)(batch & 1) * J.nblk + lb) * 2 * SJ + tid] = v;
             }
             // the tile after SJ sweeps is the next batch's starting iterate
-            for (int p = tid; p < TJ * TJ; p += JT) {
+            for (int p = tid; p < TJ * TJ; p += JTT) {
                 const int I = I0 + p % TJ, Jg = J0 + p / TJ;
                 if (I <= n0 && Jg <= n1) Xn[(long)Jg * w + I] = stash[SJ - 1][p];
             }
@@ -1167,7 +1171,7 @@ __global__ __launch_bounds__(JT) void k_jacobi_tiled(JacBatch<T> jb)
     for (int k = 0; k < MAXI; k++)
         if (k == me) { iters = it_k[k]; stop = stop_k[k]; }
     // the tile at the exit sweep: next warm start and coarse ghosts sxz2(I,J,-1)
-    for (int p = tid; p < TJ * TJ; p += JT) {
+    for (int p = tid; p < TJ * TJ; p += JTT) {
         const int I = I0 + p % TJ, Jg = J0 + p / TJ;
         if (I <= n0 && Jg <= n1) {
             const double x = stash[stop - 1][p];
