@@ -204,9 +204,9 @@ __device__ __forceinline__ void vel_tile(const GridDev<T> &g, const int tile, do
 #undef PUT
             if (sv) {
                 // the entries this thread owns, updated or not
-                g.st[0][p] = vx_c;
-                if (ok_y) g.st[1][p] = vy_c;
-                if (ix <= nx - 1 && iz <= nz - 1) g.st[2][p] = vz_c;
+                stc(g.st[0], p, vx_c);
+                if (ok_y) stc(g.st[1], p, vy_c);
+                if (ix <= nx - 1 && iz <= nz - 1) stc(g.st[2], p, vz_c);
             }
             if (EN) {
                 const double m2 = (iz == 0 || iz == nz) ? 0.5 : 1.0;
@@ -333,7 +333,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
                     const T bq = syy_o + ((c12 - c23 * c13 / c33) * cx * dvx + (c22 - c23 * c23 / c33) * cy * dvy);
                     stc(sxx, p, a);
                     stc(syy, p, bq);
-                    if (sv) { g.st[3][p] = a; g.st[4][p] = bq; g.st[5][p] = szz_o; }
+                    if (sv) { stc(g.st[3], p, a); stc(g.st[4], p, bq); stc(g.st[5], p, szz_o); }
                 } else {
                     // update_stress.cpp:69-97
                     const T dvz = vz_c - vz_m;
@@ -343,7 +343,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
                     stc(sxx, p, a);
                     stc(syy, p, bq);
                     stc(szz, p, cc);
-                    if (sv) { g.st[3][p] = a; g.st[4][p] = bq; g.st[5][p] = cc; }
+                    if (sv) { stc(g.st[3], p, a); stc(g.st[4], p, bq); stc(g.st[5], p, cc); }
                 }
                 if (EN) {
                     double s11, s22, s33, s23, s13, s12;
@@ -362,7 +362,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
                 stc(syz, p, syz_n);
                 if (g.free_top && iz == 0) syz[p - S2] = -syz_n;  // update_stress.cpp:264-267
                 if (sv) {
-                    g.st[6][p] = syz_n;
+                    stc(g.st[6], p, syz_n);
                     if (g.free_top && iz == 0) g.st[6][p - S2] = -syz_n;
                 }
                 if (EN) acc += 2 * ((1.0 / (double)c44) * (double)syz_o * (double)syz_o);
@@ -377,7 +377,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
                     sxz[p - S2] = -sxz_n;  // update_stress.cpp:274-277
                     if (sv) g.st[7][p - S2] = -sxz_n;
                 }
-                if (sv) g.st[7][p] = sxz_n;
+                if (sv) stc(g.st[7], p, sxz_n);
                 if (EN) {
                     const double m = ((iy == 0 || iy == ny) ? 0.5 : 1.0) * ((ix == 0 || ix == nx) ? 0.5 : 1.0);
                     acc += 2 * (m * (1.0 / (double)c55) * (double)sxz_o * (double)sxz_o);
@@ -389,7 +389,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
                 const T c66 = ldc(C66, p);
                 const T sxy_n = sxy_o + T(0.5) * c66 * (cy * (vx[p + P] - vx_c) + cx * (vy_c - vy[p - 1]));
                 stc(sxy, p, sxy_n);
-                if (sv) g.st[8][p] = sxy_n;
+                if (sv) stc(g.st[8], p, sxy_n);
                 if (EN) {
                     const double m = ((iz == 0 || iz == nz) ? 0.5 : 1.0) * ((ix == 0 || ix == nx) ? 0.5 : 1.0);
                     acc += 2 * (m * (1.0 / (double)c66) * (double)sxy_o * (double)sxy_o);
