@@ -1414,7 +1414,16 @@ public:
     // one time step, main.cpp:165-261
     int enqueue_step(bool keep_open = false)
     {
-        if (overlap_ok()) return enqueue_step_overlap(keep_open);
+        if (overlap_ok()) {
+            const int rc = enqueue_step_overlap(keep_open);
+            if (rc != ECSG_OK && s2) {
+                // a failed launch: let the interface stream drain so that no
+                // later call races its remaining work
+                hipStreamSynchronize(s2);
+                ov_open = false;
+            }
+            return rc;
+        }
         const int ng = (int)G.size();
         for (int g = 0; g < ng; g++)
             if (int rc = launch_vel(g)) return rc;

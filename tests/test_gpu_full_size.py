@@ -10,7 +10,9 @@ on small models):
   inside the 1e-4 relative-L2 bar;
 * with vy mirrored on every plane, the composite energy (FP64) is constant
   once the source has stopped;
-* the interface overlap changes no value of the FP32 step.
+* the interface overlap changes no value of the FP32 step;
+* the two-grid ratio-3 configuration (4.3 M nodes, 1500 steps) against the
+  FP64 oracle itself.
 """
 import numpy as np
 import pytest
@@ -18,6 +20,7 @@ import pytest
 import ecsg_amd
 from ecsg_amd import model as M
 from helpers import rel
+from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -73,3 +76,34 @@ def test_full_size_overlap_bitwise(full_model):
     for k in range(3):
         assert np.array_equal(a[k], b[k]), k
     assert a[3] == b[3]
+
+
+def test_two_grid_config_fp32_against_fp64_oracle():
+    """The two-grid ratio-3 configuration at full size (241x241x61 h 5 m over
+    81x81x121 h 15 m, smoothed +-5 % orthorhombic perturbation, Ricker 20 Hz
+    at 30 m, 1500 steps): FP32 device traces against the FP64 oracle (CPU,
+    OpenMP) within the 1e-4 relative-L2 bar, FP64 device traces within the
+    1e-6 bar.  The FP64 traces are not bit for bit here: at step 104 the
+    first interface's Jacobi solve stops one sweep later on the device (81
+    against 80 sweeps) -- its norm and the AvgAbs tolerance are sums taken in
+    another order, and this step's norm lies within their rounding of the
+    tolerance; the extra sweep moves the ghosts by less than the tolerance."""
+    m = M.two_grid_workload()
+    o = Oracle(m, prec="f64", eci_mode=1, check_energy=False)
+    o.run(m.nsteps)
+    ref = o.traces()
+    o.close()
+    assert np.abs(ref).max() > 0
+    s = ecsg_amd.Simulation(m, precision="f32", check_energy=False)
+    s.step(m.nsteps)
+    s.sync()
+    err = rel(s.traces(), ref)
+    s.close()
+    assert err <= 1e-4, err                     # measured: 1.4e-6
+    s = ecsg_amd.Simulation(m, precision="f64", check_energy=False)
+    s.step(m.nsteps)
+    s.sync()
+    err64 = rel(s.traces(), ref)
+    s.close()
+    print("two-grid 1500 steps: FP32 %.3e, FP64 %.3e relative L2 against the FP64 oracle" % (err, err64))
+    assert err64 <= 1e-6, err64                 # measured: 8.7e-16
