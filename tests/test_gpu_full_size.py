@@ -11,8 +11,9 @@ on small models):
 * with vy mirrored on every plane, the composite energy (FP64) is constant
   once the source has stopped;
 * the interface overlap changes no value of the FP32 step;
-* the two-grid ratio-3 configuration (4.3 M nodes, 1500 steps) against the
-  FP64 oracle itself.
+* the two-grid ratio-3 configuration (4.3 M nodes, 1500 steps) and the
+  uniform 81^3 configuration (600 steps, energy every step) against the FP64
+  oracle itself.
 """
 import numpy as np
 import pytest
@@ -107,3 +108,25 @@ def test_two_grid_config_fp32_against_fp64_oracle():
     s.close()
     print("two-grid 1500 steps: FP32 %.3e, FP64 %.3e relative L2 against the FP64 oracle" % (err, err64))
     assert err64 <= 1e-6, err64                 # measured: 8.7e-16
+
+
+def test_uniform_config_fp64_against_oracle():
+    """The reference's CPU-runnable configuration: one uniform 81x81x81 grid,
+    h 10 m, homogeneous orthorhombic medium at isotropic values, explosive
+    Ricker 15 Hz at (40, 40, 20), free top, 81 receivers at z = 2, energy
+    every step, 600 steps, FP64: traces and final wavefields bit for bit
+    against the oracle, energy to 1e-12 (its reductions are summed in another
+    order)."""
+    m = M.uniform_model()
+    o = Oracle(m, prec="f64", check_energy=True)
+    o.run(m.nsteps)
+    s = ecsg_amd.Simulation(m, precision="f64", check_energy=True)
+    s.step(m.nsteps)
+    s.sync()
+    assert np.abs(o.traces()).max() > 0
+    assert np.array_equal(s.traces(), o.traces())
+    assert np.allclose(s.energy(), o.energy(), rtol=1e-12, atol=0)
+    for c in range(9):
+        assert np.array_equal(s.get(0, c), o.get(0, c)), c
+    s.close()
+    o.close()
