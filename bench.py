@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--ckpt-every", type=int, default=0,
                     help="checkpoint interval of the adjoint leg (0: the largest that fits in device memory)")
     ap.add_argument("--ckpt-format", default="native", choices=["native", "fp16", "bf16"])
+    ap.add_argument("--adjoint-mem-steps", type=int, default=40,
+                    help="steps of a second gradient run that keeps every state in memory (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400, help="steps of the batched-shots leg (0: skip)")
     ap.add_argument("--batch-solo-steps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -286,6 +288,16 @@ def main():
                    "cost_vs_forward_f64": round(ms_a / fp64["ms_per_step"], 3) if fp64 else None,
                    "adjoint_jacobi_sweeps_per_step": round(sweeps / args.adjoint_steps / max(1, m.num_grids - 1), 2)}
         log("[bench] fwd+adjoint: %.2f Gpt/s, %.3f ms/step (%.2fx the FP32 forward)" % (ga, ms_a, ms_a / ms_step))
+        if args.adjoint_mem_steps > 0:
+            # a run short enough for every state to stay in device memory
+            em, _, _, Km = adjoint_run(m, args.adjoint_mem_steps, 0, args.ckpt_format)
+            ms_m = em / args.adjoint_mem_steps * 1e3
+            adjoint["in_memory"] = {"steps": args.adjoint_mem_steps, "checkpoint_every": Km,
+                                    "recomputed_steps": args.adjoint_mem_steps - min(Km, args.adjoint_mem_steps),
+                                    "ms_per_step": round(ms_m, 4), "cost_vs_forward_f32": round(ms_m / ms_step, 3),
+                                    "cost_vs_forward_f64": round(ms_m / fp64["ms_per_step"], 3) if fp64 else None}
+            log("[bench] fwd+adjoint in memory (%d steps): %.3f ms/step (%.2fx the FP32 forward)"
+                % (args.adjoint_mem_steps, ms_m, ms_m / ms_step))
 
     batch = None
     if args.batch_steps > 0:
