@@ -2423,6 +2423,12 @@ public:
         K = std::min(K, N);
         last_K = K;
         const int nseg = (N + K - 1) / K;
+        // segment j covers steps [seg0(j), seg0(j) + seglen(j)); the first
+        // takes the remainder so that the last one, which is never
+        // recomputed, is a full K steps: N - K steps are recomputed
+        const int r0 = N - (nseg - 1) * K;
+        auto seg0 = [&](int j) { return j == 0 ? 0 : r0 + (j - 1) * K; };
+        auto seglen = [&](int j) { return j == 0 ? r0 : K; };
         if (int rc = ck_prepare(std::max(nseg - 1, 1), K)) return rc;
         if (int rc = reset()) return rc;
         // forward pass with checkpoints; the last segment fills the ring
@@ -2430,7 +2436,7 @@ public:
         // recomputed
         for (int j = 0; j < nseg; j++) {
             const bool last = j == nseg - 1;
-            const int len = std::min(K, N - j * K);
+            const int len = seglen(j);
             if (!last) {
                 if (int rc = ck_save(j)) return rc;
                 for (int t = 0; t < len; t++)
@@ -2464,9 +2470,9 @@ public:
         if (int rc = adj_zero()) return rc;
         // backward over the segments
         for (int j = nseg - 1; j >= 0; j--) {
-            const int len = std::min(K, N - j * K);
+            const int len = seglen(j);
             if (j < nseg - 1) {
-                if (int rc = ck_restore(j, j * K)) return rc;
+                if (int rc = ck_restore(j, seg0(j))) return rc;
                 if (int rc = ring_save(0)) return rc;
                 for (int t = 0; t < len; t++) {
                     // the step's kernels also write every value into ring slot t+1
@@ -2477,10 +2483,10 @@ public:
                 }
             }
             for (int t = len - 1; t >= 0; t--)
-                if (int rc = adj_step_impl(j * K + t, t + 1, t, true)) return rc;
+                if (int rc = adj_step_impl(seg0(j) + t, t + 1, t, true)) return rc;
         }
         HIPCHK(hipStreamSynchronize(s));
-        host_step = std::min(K, N);
+        host_step = nseg > 1 ? r0 : N;  // the state the last recomputation left
         for (int k = 0; k < (int)I.size(); k++) {
             long long st3[3];
             HIPCHK(hipMemcpy(st3, AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));

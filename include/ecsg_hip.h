@@ -200,8 +200,10 @@ int ecsg_reset(ecsg_ctx *ctx);
 /* Checkpoint interval (default 50; 0: the largest interval whose ring of
  * recomputed states and checkpoints fit in the free device memory) and
  * storage: native precision, or IEEE half / bfloat16 of each array divided
- * by its maximum magnitude.  The last segment of a gradient is never
- * recomputed (the forward pass leaves its states in the ring). */
+ * by its maximum magnitude.  With interval K over N steps the first
+ * segment takes the remainder and the last one K steps; the last is never
+ * recomputed (the forward pass leaves its states in the ring), so N - K
+ * steps are. */
 int ecsg_set_checkpointing(ecsg_ctx *ctx, int every, int format);
 
 /* The checkpoint interval the last ecsg_gradient call used (0 before any). */
