@@ -130,3 +130,84 @@ def test_uniform_config_fp64_against_oracle():
         assert np.array_equal(s.get(0, c), o.get(0, c)), c
     s.close()
     o.close()
+
+
+def _anomaly_observed(m, nsteps):
+    """Observed traces of the two-grid configuration: a Gaussian anomaly of
+    radius 60 m in the fine grid (-8 % on the three shear moduli, +3 %
+    density), FP64."""
+    import copy
+    from ecsg_amd.model import Grid
+    G = m.grids[0]
+    nz, ny, nx = G.buoy.shape
+    z, y, x = np.meshgrid(np.arange(nz) * G.h[2], np.arange(ny) * G.h[1], np.arange(nx) * G.h[0], indexing="ij")
+    c = (0.65 * G.n[0] * G.h[0], 0.5 * G.n[1] * G.h[1], 0.5 * G.n[2] * G.h[2])
+    bump = np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / 60.0 ** 2)
+    st = list(G.stiff)
+    for i in (6, 7, 8):
+        st[i] = st[i] * (1 - 0.08 * bump)
+    m2 = copy.copy(m)
+    m2.grids = [Grid(G.n, G.h, G.buoy / (1 + 0.03 * bump), st, G.origin_z)] + list(m.grids[1:])
+    s = ecsg_amd.Simulation(m2, precision="f64", check_energy=False, max_steps=nsteps)
+    s.step(nsteps, graph=False)
+    s.sync()
+    tr = s.traces()
+    s.close()
+    return tr
+
+
+def test_two_grid_config_adjoint_gradient():
+    """The adjoint configuration on the two-grid model at full size: L2 trace
+    misfit against the anomaly's traces from the smooth background, 300
+    steps, checkpoints every 50 steps.
+
+    FP64 gradient (nine stiffnesses and buoyancy on both grids) against
+    central differences along 8 seeded random directions (relative
+    perturbations of every node's inputs): with eps = 1e-3 their O(eps^2)
+    truncation is up to 20 % at this size, so the differences at eps = 1e-3
+    and 5e-4 are Richardson-combined, (4 D(eps/2) - D(eps)) / 3, and the
+    ratio of the two truncation errors is checked to be 4 (second order);
+    the combination is held to 1e-3 (measured: <= 7.3e-6).
+
+    FP32 gradient with FP16 checkpoints (per-array max-abs scaling) against
+    the FP64 one: 1.4e-2 relative L2 over all twenty arrays, 9.6e-5 on the
+    coarse grid.  The differences concentrate in the fine grid's planes
+    above and around the explosive source, where the per-step products that
+    the FP32 accumulators sum are large and cancel over the run; FP16
+    checkpoints themselves move the FP64 gradient by 1.6e-4."""
+    from test_gpu_adjoint import _directions, _grad_dot, _misfit, _with_params
+    N = 300
+    m = M.two_grid_workload(nsteps=N)
+    obs = _anomaly_observed(m, N)
+    s = ecsg_amd.Simulation(m, precision="f64", check_energy=False, max_steps=N)
+    s.set_checkpointing(50, "native")
+    J0 = s.gradient(N, obs)
+    assert J0 > 0 and s.checkpoint_interval == 50
+    rng = np.random.default_rng(31)
+    errs = []
+    for _ in range(8):
+        d = _directions(m, rng)
+        adj = _grad_dot(s, m, d)
+        D = {}
+        for eps in (1e-3, 5e-4):
+            Jp = _misfit(_with_params(m, [[1 + eps * a for a in dg] for dg in d]), N, obs)
+            Jm = _misfit(_with_params(m, [[1 - eps * a for a in dg] for dg in d]), N, obs)
+            D[eps] = (Jp - Jm) / (2 * eps)
+        ratio = (D[1e-3] - adj) / (D[5e-4] - adj)
+        assert 3.9 <= ratio <= 4.1, ratio
+        errs.append(abs((4 * D[5e-4] - D[1e-3]) / 3 - adj) / abs(adj))
+    g64 = [s.grad(g, sel) for g in range(2) for sel in range(9, 19)]
+    s.close()
+    s = ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=N)
+    s.set_checkpointing(50, "fp16")
+    J32 = s.gradient(N, obs)
+    g32 = [s.grad(g, sel) for g in range(2) for sel in range(9, 19)]
+    s.close()
+    e_all = rel(np.concatenate([x.ravel() for x in g32]), np.concatenate([x.ravel() for x in g64]))
+    e_coarse = rel(np.concatenate([x.ravel() for x in g32[10:]]), np.concatenate([x.ravel() for x in g64[10:]]))
+    print("two-grid adjoint: J %.6e, Richardson FD max rel %.2e over 8 directions, FP32/FP16-checkpoint gradient "
+          "%.2e (coarse grid %.2e), misfit %.2e" % (J0, max(errs), e_all, e_coarse, abs(J32 - J0) / J0))
+    assert max(errs) <= 1e-3, errs              # measured: 7.3e-6
+    assert e_all <= 3e-2, e_all                 # measured: 1.4e-2
+    assert e_coarse <= 1e-3, e_coarse           # measured: 9.6e-5
+    assert abs(J32 - J0) / J0 <= 1e-3           # measured: 6.6e-5
