@@ -172,9 +172,10 @@ def test_two_grid_config_adjoint_gradient():
     FP32 gradient with FP16 checkpoints (per-array max-abs scaling) against
     the FP64 one: 1.4e-2 relative L2 over all twenty arrays, 9.6e-5 on the
     coarse grid.  The differences concentrate in the fine grid's planes
-    above and around the explosive source, where the per-step products that
-    the FP32 accumulators sum are large and cancel over the run; FP16
-    checkpoints themselves move the FP64 gradient by 1.6e-4."""
+    between the surface and the explosive source, around its column; they
+    do not come from summing the gradient terms in FP32 (flushing the FP32
+    accumulators into FP64 totals every 16 steps left them unchanged), and
+    FP16 checkpoints themselves move the FP64 gradient by only 1.6e-4."""
     from test_gpu_adjoint import _directions, _grad_dot, _misfit, _with_params
     N = 300
     m = M.two_grid_workload(nsteps=N)
