@@ -79,3 +79,15 @@ def test_config_struct_layout_matches_header():
     import ctypes as C
     assert C.sizeof(ecsg_amd.EcsgConfig) == 4 + 96 + 4 + 192 + 8 + 9 * 4 + 4
     assert ecsg_amd.EcsgConfig.ii_vz_xend.offset == 4 + 96 + 4 + 192 + 8 + 8 * 4
+
+
+def test_bench_one_gpu_only():
+    """bench.py runs on one GPU; asked for more it says so and exits with
+    status 2 before touching any device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr
