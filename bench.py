@@ -110,14 +110,16 @@ def batch_run(steps: int, solo_steps: int, warmup: int):
     """Batched shots over uneven two-grid models (16 models x 4 shots):
     one launch per phase for all 64 members, graph-captured, autotuned
     tiling; beside it the same members stepped one at a time (each with its
-    own graph-captured step)."""
+    own graph-captured step on its own stream, so that several are in flight
+    at once; their vx-sxz systems are solved by one workgroup each,
+    sxz_solver "jacobi_lds", which lets them share the GPU)."""
     import ecsg_amd
     from ecsg_amd import model as M
     t0 = time.time()
     ms = M.batch_workload(nsteps=warmup + steps + solo_steps + 10)
     nodes = sum(m.total_nodes for m in ms)
-    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=warmup + steps + solo_steps + 10)
-            for m in ms]
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=warmup + steps + solo_steps + 10,
+                                sxz_solver="jacobi_lds") for m in ms]
     log("[bench] batch: %d members, %.1f M nodes, ready in %.1fs" % (len(ms), nodes / 1e6, time.time() - t0))
     b = ecsg_amd.Batch(sims)
     by, zc = b.autotune(reps=3)

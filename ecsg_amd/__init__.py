@@ -158,7 +158,7 @@ class Simulation:
         cfg.precision = {"f32": F32, "f64": F64}[precision]
         cfg.max_steps = int(m.nsteps if max_steps is None else max_steps)
         cfg.vy_mirror_all = int(vy_mirror_all)
-        cfg.sxz_solver = {"jacobi": 0, "chebyshev": 1}[sxz_solver]
+        cfg.sxz_solver = {"jacobi": 0, "chebyshev": 1, "jacobi_lds": 2}[sxz_solver]
         cfg.ii_vz_xend = int(ii_vz_xend)
         self.max_steps = cfg.max_steps
         self.precision = precision
@@ -344,7 +344,8 @@ class Simulation:
     def set_overlap(self, on=True):
         """Interface ghost updates on a second stream beside the stencil
         planes away from the interfaces (identical results).  True / False,
-        or None for the default: on when the finest grid has >= 2^23 nodes."""
+        or None for the default: on when the steps use the multi-workgroup
+        Jacobi solve (vx-sxz iterates above 2048 entries)."""
         self._chk(self.L.ecsg_set_overlap(self.h, -1 if on is None else int(bool(on))))
 
     @property

@@ -159,7 +159,21 @@ public:
     double tol_cap = 0, tol_fac = 0;
     int max_coop = 0;  // co-resident workgroups of the register Jacobi kernel
     int max_tiled = 0; // co-resident workgroups of the tiled Jacobi kernel
-    bool lds_ok = false;  // every interface solved by k_jacobi_lds
+    bool lds_ok = false;  // every interface fits the single-workgroup k_jacobi_lds
+    // With sxz_solver 0 the single-model step uses k_jacobi_lds
+    // (graph-capturable) for iterates of up to LDS_PREF_MAXW entries and the
+    // tiled cooperative solve above: one workgroup runs ~45 sweeps of a
+    // 41 x 41 iterate in about the time 4 tiles take (0.32 vs 0.35 ms per
+    // FP32 step, ratio-3 two-grid models), of a 51 x 51 one slower (0.39 vs
+    // 0.38 ms), of an 81 x 81 one in 0.76 vs 0.48 ms.  sxz_solver 2 raises the
+    // limit to all that fits in LDS (JL_MAXW): models stepped concurrently on
+    // their own streams then share the GPU, where each cooperative solve
+    // would hold ~200 CUs and run eagerly (64 two-grid models of 1-4 M nodes,
+    // 50 steps each: 8.6 against 3.1 Gpt/s).
+    static constexpr long LDS_PREF_MAXW = 2048;
+    long lds_pref_maxw = LDS_PREF_MAXW;
+    // the single-model step solves the vx-sxz systems with k_jacobi_lds
+    bool use_lds() const { return lds_ok && lds_maxw <= lds_pref_maxw; }
     long lds_maxw = 0;
     // fused velocity+stress pass on double-buffered fields
     int use_fused = 0;
@@ -179,7 +193,7 @@ public:
     // not touch the interfaces (enqueue_step_overlap)
     hipStream_t s2 = nullptr;
     hipEvent_t ov_ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    int overlap_on = -1;  // 1 on, 0 off, -1 by size (OVERLAP_MIN_NODES)
+    int overlap_on = -1;  // 1 on, 0 off, -1 with the tiled Jacobi solve
     bool ov_open = false;  // s has not yet waited for the last sxz/syz ghost update
     std::vector<int> h_rec;  // receiver locations (grid, ix, iy, iz)
 
@@ -221,6 +235,13 @@ public:
     int init(const ecsg_config &c) override
     {
         cfg = c;
+        if (c.sxz_solver < 0 || c.sxz_solver > 2)
+            return fail(ECSG_EINVAL, "sxz_solver must be 0 (Jacobi), 1 (Chebyshev) or 2 (Jacobi, one workgroup)");
+        if (c.sxz_solver == 2) {
+            // the same sweeps, one workgroup per interface wherever the iterate fits in LDS
+            cfg.sxz_solver = 0;
+            lds_pref_maxw = JL_MAXW;
+        }
         if (c.num_grids < 1 || c.num_grids > ECSG_MAX_GRIDS) return fail(ECSG_EINVAL, "num_grids out of range");
         if (c.radius < 1 || c.radius > RMAX) return fail(ECSG_EINVAL, "radius must be 1..6");
         if (c.radius > 1 && c.num_grids != 1)
@@ -328,10 +349,9 @@ public:
             int tot = 0;
             int tott = 0;
             for (int k = k0; k < std::min((int)I.size(), k0 + MAXI); k++) { tot += I[k].coop; tott += I[k].tiles; }
-            if (tot > max_coop || (c.sxz_solver == 0 && tott > max_tiled))
+            if (tot > max_coop || (cfg.sxz_solver == 0 && tott > max_tiled))
                 return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
         }
-        if (c.sxz_solver != 0 && c.sxz_solver != 1) return fail(ECSG_EINVAL, "sxz_solver must be 0 (Jacobi) or 1 (Chebyshev)");
         // interfaces whose iterate fits in LDS are solved by one workgroup
         // each (no cooperative launch: the step can be graph-captured)
         lds_maxw = 0;
@@ -871,7 +891,7 @@ public:
             ev_begin();
             for (int k = 0; k + 1 < ng; k++)
                 if (int rc = launch_sxz_rhs(k)) return rc;
-            if (lds_ok && cfg.sxz_solver == 0) {
+            if (use_lds() && cfg.sxz_solver == 0) {
                 if (int rc = launch_jacobi_lds(false)) return rc;
             } else {
                 for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
@@ -890,7 +910,7 @@ public:
         if (sets_dirty)
             if (int rc = sync_sets()) return rc;
         int left = n;
-        const bool can_graph = use_graph && !graph_failed && !prof && (I.empty() || (lds_ok && cfg.sxz_solver == 0));
+        const bool can_graph = use_graph && !graph_failed && !prof && (I.empty() || (use_lds() && cfg.sxz_solver == 0));
         if (can_graph) {
             if (G[0].cur == 1 && left > 0) {
                 if (int rc = enqueue_fused_step()) return rc;
@@ -1184,7 +1204,7 @@ public:
         }
         ev_begin();
         if (int rc = launch_sxz_rhs(k)) return rc;
-        if (lds_ok && cfg.sxz_solver == 0) {
+        if (use_lds() && cfg.sxz_solver == 0) {
             JacLdsSet<T> set{};
             set.it[0] = jac_lds(k, false);
             JacParams jp{};
@@ -1267,16 +1287,16 @@ public:
         return ov_join();
     }
 
-    // Below this many nodes in the finest grid the overlap's extra launches
-    // and cross-stream waits cost more than the interface latency they hide
-    // (64 two-grid members of 0.4-3.5 M nodes, stepped one at a time with
-    // graphs, ran at half speed with it).
-    static constexpr long OVERLAP_MIN_NODES = 1L << 23;
+    // By default the overlap goes with the eager step of the tiled Jacobi
+    // solve.  With the single-workgroup solve (graph-captured steps of small
+    // interfaces) its extra launches and cross-stream waits cost more than
+    // they hide: 0.58 against 0.25 ms per step at a 31 x 31 iterate, and 64
+    // two-grid members stepped one at a time ran at half speed.
     bool overlap_ok() const
     {
         if (G.size() < 2 || cfg.radius != 1 || cfg.imode != 1 || cfg.check_energy) return false;
         if (overlap_on >= 0) return overlap_on != 0;
-        return (long)(G[0].n[0] + 1) * (G[0].n[1] + 1) * (G[0].n[2] + 1) >= OVERLAP_MIN_NODES;
+        return !use_lds();
     }
     // the launches of a scope go to s2 (the interface stream) instead of s
     struct OnSecond {
@@ -1398,7 +1418,7 @@ public:
             ev_begin();
             for (int k = 0; k + 1 < ng; k++)
                 if (int rc = launch_sxz_rhs(k)) return rc;
-            if (lds_ok && cfg.sxz_solver == 0) {
+            if (use_lds() && cfg.sxz_solver == 0) {
                 if (int rc = launch_jacobi_lds(false)) return rc;
             } else {
                 for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
@@ -1626,7 +1646,7 @@ public:
         // The tiled interface Jacobi is a cooperative launch; its behaviour
         // inside a captured graph is not relied on, so composite models with
         // interfaces too large for the LDS solver run eagerly.
-        if (use_graph && !graph_failed && !prof && (I.empty() || cfg.imode == 0 || (lds_ok && cfg.sxz_solver == 0))) {
+        if (use_graph && !graph_failed && !prof && (I.empty() || cfg.imode == 0 || (use_lds() && cfg.sxz_solver == 0))) {
             if (!gexec) {
                 if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                     int rc = enqueue_step();
@@ -2152,7 +2172,7 @@ public:
                 hipLaunchKernelGGL((k_asxz_ghost<T>), dim3(nb((long)(a.nx2 - 1) * (a.ny2 - 1))), dim3(256), 0, s, a, AI[k].Ba);
                 hipLaunchKernelGGL((k_asxz_fine0<T>), dim3(nb((long)(nx1 - 1) * (ny1 - 1))), dim3(256), 0, s, a);
             }
-            if (lds_ok) {
+            if (use_lds()) {
                 if (int rc = launch_jacobi_lds(true)) return rc;
             } else {
                 for (int k0 = 0; k0 < ni; k0 += MAXI)

@@ -19,7 +19,8 @@
 //                       c23_<i> c13_<i> c12_<i> c44_<i> c55_<i> c66_<i>
 //                       (code convention c44 = 2*C44) instead of s11/s13
 //   vy_mirror_all=y     mirror vy across the y walls on every plane
-//   sxz_solver=0|1      vx-sxz interface system: Jacobi, or Chebyshev-accelerated
+//   sxz_solver=0|1|2    vx-sxz interface system: Jacobi, Chebyshev-accelerated, or
+//                       Jacobi by one workgroup per interface (up to 9728 entries)
 //   ii_vz_xend=y        imode=0: the working copy's form of the x-end vz ghost
 //                       edge (update_ghost_vz.cpp:98-107) instead of the
 //                       committed one the golden II traces were made with

@@ -69,9 +69,13 @@ typedef struct {
                                            (update_vel.cpp:211-218, 239-246); 1: mirror it there too,
                                            which makes the composite energy exactly conserved */
     int sxz_solver;                     /* vx-sxz interface system: 0 = Jacobi sweeps as the
-                                           reference (update_ghost_sxz.cpp:314-350); 1 = the same
-                                           sweeps with Chebyshev acceleration (same fixed point and
-                                           exit test, fewer sweeps) */
+                                           reference (update_ghost_sxz.cpp:314-350), one workgroup
+                                           per interface for iterates of up to 2048 entries, a
+                                           cooperative tiled solve above; 1 = the same sweeps with
+                                           Chebyshev acceleration (same fixed point and exit test,
+                                           fewer sweeps); 2 = the Jacobi sweeps of 0, one workgroup
+                                           per interface up to 9728 entries (graph-capturable; for
+                                           many models stepped at once on their own streams) */
     int ii_vz_xend;                     /* imode 0 only: the x-end edge of the fine vz ghosts
                                            (update_ghost_vz.cpp:98-107).  0: left unwritten, as the
                                            committed version of that line (whose stores fall outside
@@ -234,8 +238,10 @@ int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps);
  * interface ghost updates run on a second stream beside the stencil planes
  * that do not touch the interfaces, and the planes next to each interface
  * are updated after the ghosts they read.  Results are identical to the
- * serial step.  on: 1 = on, 0 = off, -1 (default) = on when the finest grid
- * has at least 2^23 nodes.  ecsg_overlap_active returns the schedule the
+ * serial step.  on: 1 = on, 0 = off, -1 (default) = on when the steps use
+ * the multi-workgroup Jacobi solve (vx-sxz iterates above 2048 entries;
+ * smaller ones are solved by one workgroup in graph-captured steps).
+ * ecsg_overlap_active returns the schedule the
  * next steps use: 0 = serial; 1 = overlap; 2 = overlap with the stress
  * interior of grid 0 also beside the sxz/syz ghost solves (the source and
  * every receiver at least two planes away from each interface). */

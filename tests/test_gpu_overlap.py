@@ -113,7 +113,6 @@ def test_overlap_modes():
 
 def _mode(m):
     s = ecsg_amd.Simulation(m, precision="f32", check_energy=False)
-    assert s.overlap_mode == 0  # by default only for finest grids of 2^23 nodes or more
     s.set_overlap(True)
     k = s.overlap_mode
     s.close()
@@ -133,16 +132,24 @@ def test_overlap_not_active_where_not_applicable():
     s.close()
 
 
-def test_overlap_by_size():
-    """Default: on for a finest grid of 2^23 nodes or more (state only, no steps)."""
-    m = composite((3,), coarse=(100, 100, 3), nz_fine=(93,), nsteps=2, seed=12)
-    assert (m.grids[0].n[0] + 1) * (m.grids[0].n[1] + 1) * (m.grids[0].n[2] + 1) >= 1 << 23
-    s = ecsg_amd.Simulation(m, precision="f32", check_energy=False)
+def test_overlap_default_follows_the_solver():
+    """Default: on with the tiled multi-workgroup Jacobi solve (vx-sxz
+    iterates above 2048 entries), off with the single-workgroup one of
+    small interfaces or of sxz_solver "jacobi_lds" (state only, no steps)."""
+    small = composite((3,), coarse=(6, 5, 4), nz_fine=(5,), nsteps=2, seed=4)      # 7 x 6 iterate
+    s = ecsg_amd.Simulation(small, precision="f32", check_energy=False)
+    assert not s.overlap_active
+    s.close()
+    big = composite((3,), coarse=(50, 50, 3), nz_fine=(3,), nsteps=2, seed=12)     # 51 x 51 iterate
+    s = ecsg_amd.Simulation(big, precision="f32", check_energy=False)
     assert s.overlap_active
     s.set_overlap(False)
     assert not s.overlap_active
     s.set_overlap(None)
     assert s.overlap_active
+    s.close()
+    s = ecsg_amd.Simulation(big, precision="f32", check_energy=False, sxz_solver="jacobi_lds")
+    assert not s.overlap_active
     s.close()
 
 

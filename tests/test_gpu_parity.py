@@ -147,6 +147,29 @@ def test_chebyshev_solver_matches_jacobi_oracle(ratios):
     assert it_cheb < s2.jacobi_stats(0)[0]
 
 
+@pytest.mark.parametrize("solver", ["jacobi", "jacobi_lds"])
+def test_jacobi_solver_forms_on_a_51x51_interface(solver):
+    """The Jacobi sweeps of a 51 x 51 vx-sxz iterate: by default the tiled
+    cooperative solve (iterates above 2048 entries), with "jacobi_lds" one
+    workgroup holding the iterate in LDS (graph-captured steps); both
+    against the oracle to 1e-12."""
+    m = composite((3,), coarse=(50, 50, 3), nz_fine=(3,), nsteps=40, seed=12, cfl=0.8)
+    o = Oracle(m, eci_mode=1)
+    o.run(m.nsteps)
+    s = ecsg_amd.Simulation(m, precision="f64", sxz_solver=solver)
+    s.step(m.nsteps, graph=True)
+    s.sync()
+    assert np.abs(o.traces()).max() > 0
+    assert rel(s.traces(), o.traces()) <= 1e-12
+    assert np.allclose(s.energy(), o.energy(), rtol=1e-12, atol=0)
+    for g in range(m.num_grids):
+        for c in range(9):
+            assert _max_rel(s.get(g, c), o.get(g, c)) <= 1e-12, "grid %d %s" % (g, FIELD_NAMES[c])
+    assert s.jacobi_stats(0)[0] > m.nsteps
+    s.close()
+    o.close()
+
+
 @pytest.mark.parametrize("ratios", [(2,), (3,), (2, 3)])
 def test_fp32_against_fp64_oracle(ratios):
     m = composite(ratios, nz_fine=(4,) * len(ratios), nsteps=120, seed=11)
