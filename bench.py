@@ -147,6 +147,49 @@ def batch_run(steps: int, solo_steps: int, warmup: int):
                         "coarse (nx/3, ny/3, 2nz); 4 Ricker shots each (10-25 Hz); FP32"}
 
 
+def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: int = 10, tilings=()):
+    """The streaming bound of the stress sweep's traversal: ecsg_stream_probe
+    (18 FP32 arrays read, 6 written back in place, no arithmetic beyond one
+    multiply-add) over the same nodes -- nx+1 by ny+1 nodes on `planes`
+    planes, the same row pitch and plane stride -- in the stencil tiling
+    (64 x by rows x zc planes, XCD order) and as one flat pass over the
+    arrays' planes (halo rows and padding included); mean of `reps` launches
+    each, HIP events on the launch stream.  `tilings`: further (by, zc)
+    pairs, reported as "tiles_<by>x<zc>_ms"."""
+    import torch
+    import ecsg_amd
+    L = ecsg_amd.load_library()
+    nz = planes - 1
+    F = int(L.ecsg_stream_probe_floats(nx, ny, nz))
+    buf = torch.zeros(F, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream()
+    P = (32 + nx + 2 + 31) // 32 * 32
+    S2 = P * (ny + 3)
+    out = {}
+    runs = [("tiles", by, zc), ("flat", by, 0)] + [("tiles_%dx%d" % t, t[0], t[1]) for t in tilings]
+    for name, byv, zcv in runs:
+        nbytes = 96 * planes * S2 if zcv == 0 else 96 * (nx + 1) * (ny + 1) * planes
+
+        def launch():
+            rc = L.ecsg_stream_probe(buf.data_ptr(), nx, ny, nz, byv, zcv, st.cuda_stream)
+            if rc != 0:
+                raise RuntimeError("ecsg_stream_probe failed (%d)" % rc)
+        for _ in range(2):
+            launch()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            launch()
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name + "_ms"] = round(ms, 5)
+        out[name + "_GBps"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
+    del buf
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(m, steps: int):
     """The oracle (CPU restatement, OpenMP) on the same model for `steps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -228,11 +271,21 @@ def main():
                     help="steps of a second gradient run that keeps every state in memory (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400, help="steps of the batched-shots leg (0: skip)")
     ap.add_argument("--batch-solo-steps", type=int, default=50)
+    ap.add_argument("--no-stream-bound", action="store_true",
+                    help="skip the streaming probe of the stress sweep's traversal")
+    ap.add_argument("--stream-bound-grid", type=int, nargs=3, metavar=("NX", "NY", "PLANES"),
+                    help="only run the streaming probe over this grid and print its JSON line")
+    ap.add_argument("--stream-tilings", default="",
+                    help="with --stream-bound-grid: further tilings to time, e.g. 4x1,4x2,1x8 (rows x planes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (scale factor for HBM byte counters)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.stream_bound_grid:
+        tilings = [tuple(int(v) for v in t.split("x")) for t in args.stream_tilings.split(",") if t]
+        print(json.dumps(stream_bound(*args.stream_bound_grid, tilings=tilings)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
@@ -263,6 +316,18 @@ def main():
     achieved = BYTES_STRESS_F32 * n0 / t_stress if t_stress > 0 else 0.0
     vel_bw = BYTES_VEL_F32 * n0v / t_vel if t_vel > 0 else 0.0
     log("[bench] timed steps, grid-0 launches avg ms: vel %.4f stress %.4f" % live)
+    sb = None
+    if not args.no_stream_bound:
+        # in a child process: torch allocates the probe's buffer there, and
+        # torch's HIP start-up needs a process in which HIP is not running yet
+        import subprocess
+        res = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--stream-bound-grid",
+                              str(g0.n[0]), str(g0.n[1]), str(planes0)], capture_output=True, text=True, timeout=600)
+        if res.returncode != 0:
+            raise RuntimeError("streaming probe failed:\n" + res.stdout + res.stderr)
+        sb = json.loads(res.stdout.strip().splitlines()[-1])
+        sb["kernel_vs_tiles"] = round(sb["tiles_ms"] / live[1], 4) if live[1] > 0 else None
+        log("[bench] streaming bound of the stress sweep's traversal: %s" % sb)
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
@@ -337,7 +402,8 @@ def main():
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": profiled_traffic(args.workload),
                      "bytes_per_launch": BYTES_STRESS_F32 * n0, "avg_launch_ms": round(live[1], 5),
-                     "vel_kernel_GBps": round(vel_bw / 1e9, 1), "interface_overlap": overlap},
+                     "vel_kernel_GBps": round(vel_bw / 1e9, 1), "interface_overlap": overlap,
+                     "stream_bound": sb},
         "fp64": fp64,
         "adjoint": adjoint,
         "batch": batch,

@@ -34,7 +34,8 @@ ABI_SYMBOLS = (
     "ecsg_get_adjoint_field", "ecsg_set_adjoint_field", "ecsg_adjoint_stats", "ecsg_autotune", "ecsg_set_fused",
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
-    "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed",
+    "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_stream_probe_floats", "ecsg_stream_probe",
+    "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -95,6 +96,9 @@ def load_library(path: Optional[str] = None):
     L.ecsg_sbp_residuals.argtypes = [vp, i, P(d)]
     L.ecsg_step_n.argtypes = [vp, i, i]
     L.ecsg_copy_probe.argtypes = [vp, vp, C.c_uint64, i, vp]
+    L.ecsg_stream_probe_floats.argtypes = [i, i, i]
+    L.ecsg_stream_probe_floats.restype = C.c_uint64
+    L.ecsg_stream_probe.argtypes = [vp, i, i, i, i, i, vp]
     L.ecsg_set_overlap.argtypes = [vp, i]
     L.ecsg_overlap_active.argtypes = [vp]
     L.ecsg_step_n_timed.argtypes = [vp, i, P(d)]

@@ -22,6 +22,24 @@
 
 using namespace ecsg;
 
+// Device layout of one grid's arrays: x fastest, ix = 0 at the start of a
+// 128-byte line, a halo of H entries on every side, rows padded to whole
+// lines; `astride` entries per array (a whole number of lines).
+struct Layout {
+    long P, S2, o, astride;
+};
+static Layout grid_layout(const int n[3], long H, long elem_bytes)
+{
+    const long line = 128 / elem_bytes;
+    const long xoff = line;  // line >= H for every supported radius
+    Layout L;
+    L.P = ((xoff + n[0] + 1 + H) + line - 1) / line * line;
+    L.S2 = L.P * (n[1] + 1 + 2 * H);
+    L.o = H * L.S2 + H * L.P + xoff;
+    L.astride = ((L.S2 * (n[2] + 1 + 2 * H)) + line - 1) / line * line;
+    return L;
+}
+
 namespace {
 
 // wavefield / stiffness positions (init_del_sim.cpp:163-173,
@@ -267,13 +285,12 @@ public:
                 if (d.n[k] < 2 || d.n[k] < c.radius) return fail(ECSG_EINVAL, "each grid needs at least max(2, radius) cells per dimension");
                 if (!(d.h[k] > 0)) return fail(ECSG_EINVAL, "grid steps must be positive");
             }
-            const long line = 128 / (long)sizeof(T);
-            const long xoff = line;  // ix = 0 starts a 128-byte line (line >= radius)
-            const long H = c.radius; // halo: indices [-H, n+H] in every dimension
-            d.P = ((xoff + d.n[0] + 1 + H) + line - 1) / line * line;
-            d.S2 = d.P * (d.n[1] + 1 + 2 * H);
-            d.o = H * d.S2 + H * d.P + xoff;
-            d.astride = ((d.S2 * (d.n[2] + 1 + 2 * H)) + line - 1) / line * line;
+            // halo: indices [-H, n+H] in every dimension, H = radius
+            const Layout lay = grid_layout(d.n, c.radius, (long)sizeof(T));
+            d.P = lay.P;
+            d.S2 = lay.S2;
+            d.o = lay.o;
+            d.astride = lay.astride;
             if (int rc = dalloc(&d.base, (size_t)d.astride * 19)) return rc;
             GridDev<T> &v = d.dev;
             for (int a = 0; a < 9; a++) v.f[a] = d.arr(a);
@@ -3030,6 +3047,50 @@ __global__ __launch_bounds__(256) void k_copy_probe(float *dst, const float *src
     }
 }
 
+// The stress sweep's memory traffic without its arithmetic: 18 FP32 arrays
+// read and the first 6 written back in place (a[k] + a[6+k] * a[12+k]), one
+// node per lane, with the sweeps' cache hints, over the nodes of a radius-1
+// grid in the device layout -- in the stencil tiling (64 x BYv rows x zc
+// planes, workgroups in XCD order, z marched per lane), or as one flat pass
+// over the arrays' planes 0..nz (halo rows and padding included).  Its time
+// is the streaming bound of the traversal on this GPU.
+struct StreamProbe {
+    float *a[18];
+    int nx, ny, nz, gx, gy, zc, nblk;
+    long P, S2, o;
+};
+
+template <int BYv>
+__global__ __launch_bounds__(64 * BYv) void k_stream_tiles(StreamProbe g)
+{
+    const int tile = xcd_tile(blockIdx.x, g.nblk);
+    const int bx = tile % g.gx, rest = tile / g.gx, by = rest % g.gy, bz = rest / g.gy;
+    const int ix = bx * 64 + threadIdx.x, iy = by * BYv + threadIdx.y;
+    if (ix > g.nx || iy > g.ny) return;
+    const int z0 = bz * g.zc, z1 = min(z0 + g.zc, g.nz + 1);
+    long p = g.o + (long)z0 * g.S2 + (long)iy * g.P + ix;
+    for (int iz = z0; iz < z1; ++iz, p += g.S2) {
+        float v[18];
+#pragma unroll
+        for (int k = 0; k < 18; k++) v[k] = ldc(g.a[k], p);
+#pragma unroll
+        for (int k = 0; k < 6; k++) stc(g.a[k], p, v[k] + v[6 + k] * v[12 + k]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stream_flat(StreamProbe g)
+{
+    const long n = (long)(g.nz + 1) * g.S2;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const long p = g.S2 + i;  // planes 0 .. nz
+    float v[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) v[k] = ldc(g.a[k], p);
+#pragma unroll
+    for (int k = 0; k < 6; k++) stc(g.a[k], p, v[k] + v[6 + k] * v[12 + k]);
+}
+
 extern "C" {
 
 const char *ecsg_version(void) { return "ecsg_hip 0.1 (gfx950)"; }
@@ -3093,6 +3154,41 @@ int ecsg_copy_probe(float *dst, const float *src, uint64_t n, int hints, void *s
     } else {
         if (hints == 3) k_copy_probe<3, false><<<blocks, 256, 0, st>>>(dst, src, n);
         else k_copy_probe<0, false><<<blocks, 256, 0, st>>>(dst, src, n);
+    }
+    return hipGetLastError() == hipSuccess ? ECSG_OK : ECSG_EHIP;
+}
+uint64_t ecsg_stream_probe_floats(int nx, int ny, int nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return 0;
+    const int n[3] = {nx, ny, nz};
+    return 18 * (uint64_t)grid_layout(n, 1, (long)sizeof(float)).astride;
+}
+int ecsg_stream_probe(float *base, int nx, int ny, int nz, int by, int zc, void *stream)
+{
+    if (!base || nx < 1 || ny < 1 || nz < 1 || zc < 0) return ECSG_EINVAL;
+    if (by != 1 && by != 2 && by != 4 && by != 8) return ECSG_EINVAL;
+    const int n[3] = {nx, ny, nz};
+    const Layout lay = grid_layout(n, 1, (long)sizeof(float));
+    StreamProbe g;
+    for (int k = 0; k < 18; k++) g.a[k] = base + k * lay.astride;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.P = lay.P; g.S2 = lay.S2; g.o = lay.o;
+    hipStream_t st = (hipStream_t)stream;
+    if (zc == 0) {
+        g.gx = g.gy = g.zc = g.nblk = 0;
+        const long cnt = (long)(nz + 1) * lay.S2;
+        k_stream_flat<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(g);
+    } else {
+        g.zc = zc;
+        g.gx = (nx + 1 + 63) / 64;
+        g.gy = (ny + 1 + by - 1) / by;
+        const long nb = (long)g.gx * g.gy * ((nz + 1 + zc - 1) / zc);
+        if (nb > (1L << 30)) return ECSG_EINVAL;
+        g.nblk = (int)nb;
+        if (by == 1) k_stream_tiles<1><<<g.nblk, dim3(64, 1), 0, st>>>(g);
+        else if (by == 2) k_stream_tiles<2><<<g.nblk, dim3(64, 2), 0, st>>>(g);
+        else if (by == 4) k_stream_tiles<4><<<g.nblk, dim3(64, 4), 0, st>>>(g);
+        else k_stream_tiles<8><<<g.nblk, dim3(64, 8), 0, st>>>(g);
     }
     return hipGetLastError() == hipSuccess ? ECSG_OK : ECSG_EHIP;
 }

@@ -74,6 +74,18 @@ def test_library_exports_every_declared_symbol():
     assert b"gfx950" in lib.ecsg_version()
 
 
+def test_stream_probe_buffer_size():
+    """ecsg_stream_probe_floats (host arithmetic only): 18 arrays of the
+    radius-1 FP32 device layout, ix = 0 on a 128-byte line, a halo of 1,
+    rows and arrays padded to 32 floats."""
+    L = ecsg_amd.load_library()
+    # 1 cell per dimension: rows of 64 floats (32 + 3 rounded up), 4 rows, 4 planes
+    assert L.ecsg_stream_probe_floats(1, 1, 1) == 18 * 64 * 4 * 4
+    # 1206 x 1206 x 80 cells: rows of 1248 floats, 1209 rows, 83 planes
+    assert L.ecsg_stream_probe_floats(1206, 1206, 80) == 18 * 1248 * 1209 * 83
+    assert L.ecsg_stream_probe_floats(0, 4, 4) == 0
+
+
 def test_config_struct_layout_matches_header():
     # ecsg_config: int, int[8][3], (pad), double[8][3], double, 9 ints, (pad to 8)
     import ctypes as C
