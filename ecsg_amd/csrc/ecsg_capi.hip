@@ -1205,6 +1205,15 @@ public:
             if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
             HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, 0, s));
         }
+        // fine ghosts from the coarse ghosts just written
+        long nfmax = 0;
+        for (int q = 0; q < n; q++) {
+            const long r = I[k0 + q].r;
+            nfmax = std::max(nfmax, (r * I[k0 + q].nx2 - 1) * (r * I[k0 + q].ny2 - 1));
+        }
+        if (nfmax > (1L << 31) - 256) return fail(ECSG_EINVAL, "interface too large for the fine ghost kernel");
+        hipLaunchKernelGGL((k_jacobi_fine<T>), dim3((unsigned)((nfmax + 255) / 256), n), dim3(256), 0, s, jb);
+        HIPCHK(hipGetLastError());
         return ECSG_OK;
     }
 

@@ -930,28 +930,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
         J.sxz2[J.o2 + (long)(-1) * J.S22 + i1 * J.P2 + i0] = (T)x;
         if (J.sxz2s) J.sxz2s[J.o2 + (long)(-1) * J.S22 + i1 * J.P2 + i0] = (T)x;
     }
-    grid.sync();
-    // fine ghosts on fine interior nodes
-    const int r = J.r, nx1 = r * nx2, ny1 = r * ny2;
-    const long nf = (long)(nx1 - 1) * (ny1 - 1);
-    for (long q = gtid; q < nf; q += gstride) {
-        const int i = (int)(q % (nx1 - 1)) + 1, jj = (int)(q / (nx1 - 1)) + 1;
-        const int I0 = i / r, J0 = jj / r, ao = i - r * I0, bo = jj - r * J0;
-        double S = 0;
-        for (int dJ = 0; dJ <= 1; dJ++)
-            for (int dI = 0; dI <= 1; dI++) {
-                const int Ic = I0 + dI, Jc = J0 + dJ;
-                if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
-                const double pw = hatw(ao - r * dI, r) * hatw(bo - r * dJ, r);
-                if (pw == 0) continue;
-                const long q2 = J.o2 + (long)Jc * J.P2 + Ic;
-                S += pw * ((double)J.sxz2[q2 - J.S22] + (double)J.sxz2[q2]);
-            }
-        const long qf = J.o1 + (long)J.nz1 * J.S21 + (long)jj * J.P1 + i;
-        const T v = (T)(-(double)J.sxz1[qf - J.S21] + S);
-        J.sxz1[qf] = v;
-        if (J.sxz1s) J.sxz1s[qf] = v;
-    }
+    // the fine ghosts follow in k_jacobi_fine
     if (lb == 0 && threadIdx.x == 0) {
         J.stats[0] += iters;
         J.stats[1] = iters;
@@ -1182,43 +1161,42 @@ __global__ __launch_bounds__(JTT) void k_jacobi_tiled(JacBatch<T> jb)
             }
         }
     }
-    if (jb.solve_only) {
-        if (lb == 0 && tid == 0) {
-            J.stats[0] += iters;
-            J.stats[1] = iters;
-            if (iters >= jb.cap) J.stats[2] = 1;
-        }
-        return;
-    }
-    grid.sync();
-    // fine ghosts on fine interior nodes
-    const int r = J.r, nx1 = r * nx2, ny1 = r * ny2;
-    const long nf = (long)(nx1 - 1) * (ny1 - 1);
-    const long gtid = (long)lb * blockDim.x + tid;
-    const long gstride = (long)J.nblk * blockDim.x;
-    for (long q = gtid; q < nf; q += gstride) {
-        const int i = (int)(q % (nx1 - 1)) + 1, jj = (int)(q / (nx1 - 1)) + 1;
-        const int Ia = i / r, Ja = jj / r, ao = i - r * Ia, bo = jj - r * Ja;
-        double S = 0;
-        for (int dJ = 0; dJ <= 1; dJ++)
-            for (int dI = 0; dI <= 1; dI++) {
-                const int Ic = Ia + dI, Jc = Ja + dJ;
-                if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
-                const double pw = hatw(ao - r * dI, r) * hatw(bo - r * dJ, r);
-                if (pw == 0) continue;
-                const long q2 = J.o2 + (long)Jc * J.P2 + Ic;
-                S += pw * ((double)J.sxz2[q2 - J.S22] + (double)J.sxz2[q2]);
-            }
-        const long qf = J.o1 + (long)J.nz1 * J.S21 + (long)jj * J.P1 + i;
-        const T v = (T)(-(double)J.sxz1[qf - J.S21] + S);
-        J.sxz1[qf] = v;
-        if (J.sxz1s) J.sxz1s[qf] = v;
-    }
+    // the fine ghosts follow in k_jacobi_fine
     if (lb == 0 && tid == 0) {
         J.stats[0] += iters;
         J.stats[1] = iters;
         if (iters >= jb.cap) J.stats[2] = 1;
     }
+}
+
+// Fine ghosts sxz1(i, j, nz1) of the vx-sxz interfaces from the coarse ghosts
+// the Jacobi kernels wrote (update_ghost_sxz.cpp:247-295, ratio-r form): one
+// thread per fine interior node (i, j) in [1, nx1-1] x [1, ny1-1] of
+// interface blockIdx.y, i fastest.  A launch of its own after the cooperative
+// solve, so that it runs at full occupancy.
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi_fine(JacBatch<T> jb)
+{
+    const JacItf<T> &J = jb.it[blockIdx.y];
+    const int r = J.r, nx2 = J.nx2, ny2 = J.ny2, nx1 = r * nx2, ny1 = r * ny2;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= (nx1 - 1) * (ny1 - 1)) return;
+    const int i = q % (nx1 - 1) + 1, jj = q / (nx1 - 1) + 1;
+    const int Ia = i / r, Ja = jj / r, ao = i - r * Ia, bo = jj - r * Ja;
+    double S = 0;
+    for (int dJ = 0; dJ <= 1; dJ++)
+        for (int dI = 0; dI <= 1; dI++) {
+            const int Ic = Ia + dI, Jc = Ja + dJ;
+            if (Ic < 1 || Ic > nx2 - 1 || Jc < 1 || Jc > ny2 - 1) continue;
+            const double pw = hatw(ao - r * dI, r) * hatw(bo - r * dJ, r);
+            if (pw == 0) continue;
+            const long q2 = J.o2 + (long)Jc * J.P2 + Ic;
+            S += pw * ((double)J.sxz2[q2 - J.S22] + (double)J.sxz2[q2]);
+        }
+    const long qf = J.o1 + (long)J.nz1 * J.S21 + (long)jj * J.P1 + i;
+    const T v = (T)(-(double)J.sxz1[qf - J.S21] + S);
+    J.sxz1[qf] = v;
+    if (J.sxz1s) J.sxz1s[qf] = v;
 }
 
 // vy-syz pair: pointwise coarse ghost and its fine children (derived ratio-r
