@@ -177,6 +177,7 @@ public:
     double tol_cap = 0, tol_fac = 0;
     int max_coop = 0;  // co-resident workgroups of the register Jacobi kernel
     int max_tiled = 0; // co-resident workgroups of the tiled Jacobi kernel
+    int max_tiled_few = 0;  // the same for its JTT_FEW-thread form
     bool lds_ok = false;  // every interface fits the single-workgroup k_jacobi_lds
     // With sxz_solver 0 the single-model step uses k_jacobi_lds
     // (graph-capturable) for iterates of up to LDS_PREF_MAXW entries and the
@@ -355,8 +356,10 @@ public:
                 return fail(ECSG_EINVAL, "interface " + std::to_string(k) + " too large for the resident Jacobi solve");
             f.coop = (int)std::max(1L, want);
             int pert = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pert, (const void *)k_jacobi_tiled<T>, JTT, 0));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pert, (const void *)k_jacobi_tiled<T, JTT>, JTT, 0));
             max_tiled = pert * ncu;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pert, (const void *)k_jacobi_tiled<T, JTT_FEW>, JTT_FEW, 0));
+            max_tiled_few = pert * ncu;
             f.tiles = ((f.nx2 - 1 + TJ - 1) / TJ) * ((f.ny2 - 1 + TJ - 1) / TJ);
             const size_t np = std::max(4 * (size_t)f.coop + 1, (size_t)2 * f.tiles * 2 * SJ);
             if (int rc = dalloc(&f.jpart, np)) return rc;
@@ -1168,6 +1171,20 @@ public:
         return ECSG_OK;
     }
 
+    // cooperative launch of the tiled Jacobi solve over blk tiles: JTT_FEW
+    // threads per tile for launches of at most JTT_FEW_TILES tiles, else JTT
+    int launch_tiled(JacBatch<T> &jb, int blk)
+    {
+        void *args[] = {(void *)&jb};
+        if (blk <= JTT_FEW_TILES && blk <= max_tiled_few) {
+            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T, JTT_FEW>, dim3(blk), dim3(JTT_FEW), args, 0, s));
+            return ECSG_OK;
+        }
+        if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T, JTT>, dim3(blk), dim3(JTT), args, 0, s));
+        return ECSG_OK;
+    }
+
     // Jacobi solves + ghost scatter of interfaces [k0, k0+n) in one launch
     int launch_jacobi(int k0, int n)
     {
@@ -1199,8 +1216,7 @@ public:
         void *args[] = {(void *)&jb};
         if (cfg.sxz_solver == 0) {
             // the reference's Jacobi sweeps, SJ per grid barrier in LDS tiles
-            if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
-            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JTT), args, 0, s));
+            if (int rc = launch_tiled(jb, blk)) return rc;
         } else {
             if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
             HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, 0, s));
@@ -2169,10 +2185,7 @@ public:
         jb.cap = JACOBI_CAP;
         jb.cheby = 0;
         jb.solve_only = 1;
-        if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
-        void *args[] = {(void *)&jb};
-        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T>, dim3(blk), dim3(JTT), args, 0, s));
-        return ECSG_OK;
+        return launch_tiled(jb, blk);
     }
 
     // One transposed step: forward step n read from ring slot slot_f (state

@@ -954,20 +954,25 @@ constexpr int TJ = 32;                   // tile edge (unknowns)
 constexpr int SJ = ECSG_SJ;               // sweeps per grid barrier
 constexpr int EJ = TJ + 2 * SJ;          // LDS edge of tile + halo
 constexpr int RJ = TJ + 2 * (SJ - 1);    // edge of the first sweep's region
-#ifndef ECSG_JTT
-#define ECSG_JTT 256
-#endif
-constexpr int JTT = ECSG_JTT;             // threads per tiled-Jacobi workgroup
-constexpr int KJ = (RJ * RJ + JTT - 1) / JTT;  // region points per thread
+// Threads per workgroup: JTT, or JTT_FEW for launches of at most
+// JTT_FEW_TILES tiles.  With few tiles the solve is the longest path of the
+// step and more waves per tile hide its LDS and FP64 latencies (ratio-3
+// two-grid model, 9 tiles: 0.354 against 0.382 ms per step); with many tiles
+// the extra waves slow the stencils beside the solve (194 tiles: 4.59 against
+// 4.53 ms per step).
+constexpr int JTT = 256;
+constexpr int JTT_FEW = 512;
+constexpr int JTT_FEW_TILES = 64;
 
-template <typename T>
-__global__ __launch_bounds__(JTT) void k_jacobi_tiled(JacBatch<T> jb)
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_jacobi_tiled(JacBatch<T> jb)
 {
+    constexpr int KJ = (RJ * RJ + NT - 1) / NT;  // region points per thread
     namespace cg = cooperative_groups;
     cg::grid_group grid = cg::this_grid();
     __shared__ double xb[2][EJ * EJ];
     __shared__ double stash[SJ][TJ * TJ];
-    __shared__ double red[JTT / 64][2 * SJ];
+    __shared__ double red[NT / 64][2 * SJ];
     __shared__ double s_tol[MAXI];
     __shared__ double s_n2[MAXI][SJ], s_x2[MAXI][SJ];
     int me = 0;
@@ -999,7 +1004,7 @@ __global__ __launch_bounds__(JTT) void k_jacobi_tiled(JacBatch<T> jb)
     int ti[KJ];          // tile index, -1 outside the tile
 #pragma unroll
     for (int k = 0; k < KJ; k++) {
-        const int p = tid + k * JTT;
+        const int p = tid + k * NT;
         li[k] = -1; sh[k] = 0; ti[k] = -1;
         if (p < RJ * RJ) {
             const int ri = p % RJ, rj = p / RJ;
@@ -1031,7 +1036,7 @@ __global__ __launch_bounds__(JTT) void k_jacobi_tiled(JacBatch<T> jb)
             if (k == me) active = !done_k[k];
         if (active) {
             // load tile + halo of the current iterate (zero outside the unknowns)
-            for (int p = tid; p < EJ * EJ; p += JTT) {
+            for (int p = tid; p < EJ * EJ; p += NT) {
                 const int ei = p % EJ, ej = p / EJ;
                 const int I = I0 - SJ + ei, Jg = J0 - SJ + ej;
                 double v = 0;
@@ -1084,11 +1089,11 @@ __global__ __launch_bounds__(JTT) void k_jacobi_tiled(JacBatch<T> jb)
             __syncthreads();
             if (tid < 2 * SJ) {
                 double v = 0;
-                for (int q = 0; q < JTT / 64; q++) v += red[q][tid];
+                for (int q = 0; q < NT / 64; q++) v += red[q][tid];
                 J.part[((long)(batch & 1) * J.nblk + lb) * 2 * SJ + tid] = v;
             }
             // the tile after SJ sweeps is the next batch's starting iterate
-            for (int p = tid; p < TJ * TJ; p += JTT) {
+            for (int p = tid; p < TJ * TJ; p += NT) {
                 const int I = I0 + p % TJ, Jg = J0 + p / TJ;
                 if (I <= n0 && Jg <= n1) Xn[(long)Jg * w + I] = stash[SJ - 1][p];
             }
@@ -1150,7 +1155,7 @@ __global__ __launch_bounds__(JTT) void k_jacobi_tiled(JacBatch<T> jb)
     for (int k = 0; k < MAXI; k++)
         if (k == me) { iters = it_k[k]; stop = stop_k[k]; }
     // the tile at the exit sweep: next warm start and coarse ghosts sxz2(I,J,-1)
-    for (int p = tid; p < TJ * TJ; p += JTT) {
+    for (int p = tid; p < TJ * TJ; p += NT) {
         const int I = I0 + p % TJ, Jg = J0 + p / TJ;
         if (I <= n0 && Jg <= n1) {
             const double x = stash[stop - 1][p];
