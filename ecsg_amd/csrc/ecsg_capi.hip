@@ -17,10 +17,39 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace ecsg;
+
+// Cooperative (grid-barrier) launches of every handle on one device are
+// issued in one order, each behind the previous one: two such kernels from
+// different streams could each hold part of the GPU while waiting at a grid
+// barrier for workgroups that the other keeps from being placed.  The
+// per-device events live until the process ends.
+static std::mutex coop_mu;
+static hipEvent_t coop_last[64];
+
+static hipError_t launch_cooperative(const void *fn, dim3 grid, dim3 block, void **args, hipStream_t s)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(coop_mu);
+    hipEvent_t &ev = coop_last[dev];
+    if (!ev) {
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) { ev = nullptr; return e; }
+    } else {
+        e = hipStreamWaitEvent(s, ev, 0);
+        if (e != hipSuccess) return e;
+    }
+    e = hipLaunchCooperativeKernel(fn, grid, block, args, 0, s);
+    if (e != hipSuccess) return e;
+    return hipEventRecord(ev, s);
+}
 
 // Device layout of one grid's arrays: x fastest, ix = 0 at the start of a
 // 128-byte line, a halo of H entries on every side, rows padded to whole
@@ -1177,11 +1206,11 @@ public:
     {
         void *args[] = {(void *)&jb};
         if (blk <= JTT_FEW_TILES && blk <= max_tiled_few) {
-            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T, JTT_FEW>, dim3(blk), dim3(JTT_FEW), args, 0, s));
+            HIPCHK(launch_cooperative((const void *)k_jacobi_tiled<T, JTT_FEW>, dim3(blk), dim3(JTT_FEW), args, s));
             return ECSG_OK;
         }
         if (blk > max_tiled) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
-        HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi_tiled<T, JTT>, dim3(blk), dim3(JTT), args, 0, s));
+        HIPCHK(launch_cooperative((const void *)k_jacobi_tiled<T, JTT>, dim3(blk), dim3(JTT), args, s));
         return ECSG_OK;
     }
 
@@ -1219,7 +1248,7 @@ public:
             if (int rc = launch_tiled(jb, blk)) return rc;
         } else {
             if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
-            HIPCHK(hipLaunchCooperativeKernel((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, 0, s));
+            HIPCHK(launch_cooperative((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, s));
         }
         // fine ghosts from the coarse ghosts just written
         long nfmax = 0;

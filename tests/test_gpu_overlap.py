@@ -190,3 +190,29 @@ def test_step_timed_equals_step(overlap):
         outs.append(([s.get(g, c) for g in range(m.num_grids) for c in range(9)], s.traces()))
         s.close()
     _same(outs[1], outs[0], "timed")
+
+
+def test_two_handles_interleaved():
+    """Two handles whose interfaces use the cooperative Jacobi solve (101 x
+    101 coarse interfaces), stepped in interleaved calls with no
+    synchronisation between them, each on its own streams (the library
+    issues cooperative launches of all handles in one order): every field
+    entry and trace equals the same handle stepped alone."""
+    nsteps = 16
+    ms = [composite((3,), coarse=(100, 100, 3), nz_fine=(3,), nsteps=nsteps, seed=sd, cfl=0.8) for sd in (12, 14)]
+    alone = []
+    for m in ms:
+        s = ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=nsteps + 2)
+        assert s.overlap_active
+        s.step(nsteps, graph=False)
+        s.sync()
+        alone.append(([s.get(g, c) for g in range(m.num_grids) for c in range(9)], s.traces()))
+        s.close()
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=nsteps + 2) for m in ms]
+    for _ in range(nsteps):
+        for s in sims:
+            s.step(1, graph=False)
+    for s, m, ref in zip(sims, ms, alone):
+        s.sync()
+        _same(([s.get(g, c) for g in range(m.num_grids) for c in range(9)], s.traces()), ref, "interleaved")
+        s.close()
