@@ -185,7 +185,8 @@ def batch_gradient_run(steps: int, every: int):
                     "member by member; gradients of a model = sum over its 4 shots"}
 
 
-def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: int = 10, tilings=()):
+def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: int = 10, tilings=(),
+                 vel_planes: int = 0):
     """The streaming bound of the stress sweep's traversal: ecsg_stream_probe
     (18 FP32 arrays read, 6 written back in place, no arithmetic beyond one
     multiply-add) over the same nodes -- nx+1 by ny+1 nodes on `planes`
@@ -193,23 +194,29 @@ def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: 
     (64 x by rows x zc planes, XCD order) and as one flat pass over the
     arrays' planes (halo rows and padding included); mean of `reps` launches
     each, HIP events on the launch stream.  `tilings`: further (by, zc)
-    pairs, reported as "tiles_<by>x<zc>_ms"."""
+    pairs, reported as "tiles_<by>x<zc>_ms".  vel_planes > 0: also the
+    velocity sweep's traffic (10 arrays read, 3 written back, 52 B per node)
+    over that many planes in the same tiling, "vel_tiles_ms"."""
     import torch
     import ecsg_amd
     L = ecsg_amd.load_library()
-    nz = planes - 1
+    nz = max(planes, vel_planes) - 1
     F = int(L.ecsg_stream_probe_floats(nx, ny, nz))
     buf = torch.zeros(F, dtype=torch.float32, device="cuda")
     st = torch.cuda.current_stream()
     P = (32 + nx + 2 + 31) // 32 * 32
     S2 = P * (ny + 3)
     out = {}
-    runs = [("tiles", by, zc), ("flat", by, 0)] + [("tiles_%dx%d" % t, t[0], t[1]) for t in tilings]
-    for name, byv, zcv in runs:
-        nbytes = 96 * planes * S2 if zcv == 0 else 96 * (nx + 1) * (ny + 1) * planes
+    runs = [("tiles", by, zc, 0, planes), ("flat", by, 0, 0, planes)]
+    runs += [("tiles_%dx%d" % t, t[0], t[1], 0, planes) for t in tilings]
+    if vel_planes > 0:
+        runs.append(("vel_tiles", by, zc, 1, vel_planes))
+    for name, byv, zcv, pat, npl in runs:
+        bpn = 96 if pat == 0 else 52
+        nbytes = bpn * npl * S2 if zcv == 0 else bpn * (nx + 1) * (ny + 1) * npl
 
         def launch():
-            rc = L.ecsg_stream_probe(buf.data_ptr(), nx, ny, nz, byv, zcv, st.cuda_stream)
+            rc = L.ecsg_stream_probe(buf.data_ptr(), nx, ny, npl - 1, byv, zcv, pat, st.cuda_stream)
             if rc != 0:
                 raise RuntimeError("ecsg_stream_probe failed (%d)" % rc)
         for _ in range(2):
@@ -316,6 +323,8 @@ def main():
                     help="skip the streaming probe of the stress sweep's traversal")
     ap.add_argument("--stream-bound-grid", type=int, nargs=3, metavar=("NX", "NY", "PLANES"),
                     help="only run the streaming probe over this grid and print its JSON line")
+    ap.add_argument("--stream-vel-planes", type=int, default=0,
+                    help="with --stream-bound-grid: also the velocity sweep's traffic over this many planes")
     ap.add_argument("--stream-tilings", default="",
                     help="with --stream-bound-grid: further tilings to time, e.g. 4x1,4x2,1x8 (rows x planes)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -326,7 +335,8 @@ def main():
         return
     if args.stream_bound_grid:
         tilings = [tuple(int(v) for v in t.split("x")) for t in args.stream_tilings.split(",") if t]
-        print(json.dumps(stream_bound(*args.stream_bound_grid, tilings=tilings)), flush=True)
+        print(json.dumps(stream_bound(*args.stream_bound_grid, tilings=tilings, vel_planes=args.stream_vel_planes)),
+              flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
@@ -363,11 +373,13 @@ def main():
         # torch's HIP start-up needs a process in which HIP is not running yet
         import subprocess
         res = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--stream-bound-grid",
-                              str(g0.n[0]), str(g0.n[1]), str(planes0)], capture_output=True, text=True, timeout=600)
+                              str(g0.n[0]), str(g0.n[1]), str(planes0), "--stream-vel-planes", str(planes0v)],
+                             capture_output=True, text=True, timeout=600)
         if res.returncode != 0:
             raise RuntimeError("streaming probe failed:\n" + res.stdout + res.stderr)
         sb = json.loads(res.stdout.strip().splitlines()[-1])
         sb["kernel_vs_tiles"] = round(sb["tiles_ms"] / live[1], 4) if live[1] > 0 else None
+        sb["vel_kernel_vs_tiles"] = round(sb["vel_tiles_ms"] / live[0], 4) if live[0] > 0 else None
         log("[bench] streaming bound of the stress sweep's traversal: %s" % sb)
 
     fp64 = None

@@ -98,7 +98,7 @@ def load_library(path: Optional[str] = None):
     L.ecsg_copy_probe.argtypes = [vp, vp, C.c_uint64, i, vp]
     L.ecsg_stream_probe_floats.argtypes = [i, i, i]
     L.ecsg_stream_probe_floats.restype = C.c_uint64
-    L.ecsg_stream_probe.argtypes = [vp, i, i, i, i, i, vp]
+    L.ecsg_stream_probe.argtypes = [vp, i, i, i, i, i, i, vp]
     L.ecsg_set_overlap.argtypes = [vp, i]
     L.ecsg_overlap_active.argtypes = [vp]
     L.ecsg_step_n_timed.argtypes = [vp, i, P(d)]

@@ -3098,20 +3098,38 @@ __global__ __launch_bounds__(256) void k_copy_probe(float *dst, const float *src
     }
 }
 
-// The stress sweep's memory traffic without its arithmetic: 18 FP32 arrays
-// read and the first 6 written back in place (a[k] + a[6+k] * a[12+k]), one
-// node per lane, with the sweeps' cache hints, over the nodes of a radius-1
-// grid in the device layout -- in the stencil tiling (64 x BYv rows x zc
-// planes, workgroups in XCD order, z marched per lane), or as one flat pass
-// over the arrays' planes 0..nz (halo rows and padding included).  Its time
-// is the streaming bound of the traversal on this GPU.
+// A stencil sweep's memory traffic without its arithmetic: NR FP32 arrays
+// read and the first NW written back in place (a[k] + a[k+NW] + a[k+2NW] +
+// ... over all NR arrays, so that every load is used), one node per lane, with the sweeps' cache hints, over the nodes
+// of a radius-1 grid in the device layout -- in the stencil tiling (64 x BYv
+// rows x zc planes, workgroups in XCD order, z marched per lane), or as one
+// flat pass over the arrays' planes 0..nz (halo rows and padding included).
+// NR = 18, NW = 6: the stress sweep (9 stiffnesses, 3 velocities and the 6
+// stresses it updates); NR = 10, NW = 3: the velocity sweep (6 stresses, the
+// buoyancy and the 3 velocities it updates).  Its time is the streaming bound of
+// the traversal on this GPU.
 struct StreamProbe {
     float *a[18];
     int nx, ny, nz, gx, gy, zc, nblk;
     long P, S2, o;
 };
 
-template <int BYv>
+template <int NR, int NW>
+__device__ __forceinline__ void stream_node(const StreamProbe &g, long p)
+{
+    float v[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) v[k] = ldc(g.a[k], p);
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        float u = v[k];
+#pragma unroll
+        for (int j = NW + k; j < NR; j += NW) u = u + v[j];
+        stc(g.a[k], p, u);
+    }
+}
+
+template <int BYv, int NR, int NW>
 __global__ __launch_bounds__(64 * BYv) void k_stream_tiles(StreamProbe g)
 {
     const int tile = xcd_tile(blockIdx.x, g.nblk);
@@ -3120,26 +3138,26 @@ __global__ __launch_bounds__(64 * BYv) void k_stream_tiles(StreamProbe g)
     if (ix > g.nx || iy > g.ny) return;
     const int z0 = bz * g.zc, z1 = min(z0 + g.zc, g.nz + 1);
     long p = g.o + (long)z0 * g.S2 + (long)iy * g.P + ix;
-    for (int iz = z0; iz < z1; ++iz, p += g.S2) {
-        float v[18];
-#pragma unroll
-        for (int k = 0; k < 18; k++) v[k] = ldc(g.a[k], p);
-#pragma unroll
-        for (int k = 0; k < 6; k++) stc(g.a[k], p, v[k] + v[6 + k] * v[12 + k]);
-    }
+    for (int iz = z0; iz < z1; ++iz, p += g.S2) stream_node<NR, NW>(g, p);
 }
 
+template <int NR, int NW>
 __global__ __launch_bounds__(256) void k_stream_flat(StreamProbe g)
 {
     const long n = (long)(g.nz + 1) * g.S2;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const long p = g.S2 + i;  // planes 0 .. nz
-    float v[18];
-#pragma unroll
-    for (int k = 0; k < 18; k++) v[k] = ldc(g.a[k], p);
-#pragma unroll
-    for (int k = 0; k < 6; k++) stc(g.a[k], p, v[k] + v[6 + k] * v[12 + k]);
+    stream_node<NR, NW>(g, g.S2 + i);  // planes 0 .. nz
+}
+
+template <int NR, int NW>
+static void launch_stream(const StreamProbe &g, int by, long cnt, hipStream_t st)
+{
+    if (g.zc == 0) k_stream_flat<NR, NW><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(g);
+    else if (by == 1) k_stream_tiles<1, NR, NW><<<g.nblk, dim3(64, 1), 0, st>>>(g);
+    else if (by == 2) k_stream_tiles<2, NR, NW><<<g.nblk, dim3(64, 2), 0, st>>>(g);
+    else if (by == 4) k_stream_tiles<4, NR, NW><<<g.nblk, dim3(64, 4), 0, st>>>(g);
+    else k_stream_tiles<8, NR, NW><<<g.nblk, dim3(64, 8), 0, st>>>(g);
 }
 
 extern "C" {
@@ -3214,9 +3232,9 @@ uint64_t ecsg_stream_probe_floats(int nx, int ny, int nz)
     const int n[3] = {nx, ny, nz};
     return 18 * (uint64_t)grid_layout(n, 1, (long)sizeof(float)).astride;
 }
-int ecsg_stream_probe(float *base, int nx, int ny, int nz, int by, int zc, void *stream)
+int ecsg_stream_probe(float *base, int nx, int ny, int nz, int by, int zc, int pattern, void *stream)
 {
-    if (!base || nx < 1 || ny < 1 || nz < 1 || zc < 0) return ECSG_EINVAL;
+    if (!base || nx < 1 || ny < 1 || nz < 1 || zc < 0 || (pattern != 0 && pattern != 1)) return ECSG_EINVAL;
     if (by != 1 && by != 2 && by != 4 && by != 8) return ECSG_EINVAL;
     const int n[3] = {nx, ny, nz};
     const Layout lay = grid_layout(n, 1, (long)sizeof(float));
@@ -3225,22 +3243,18 @@ int ecsg_stream_probe(float *base, int nx, int ny, int nz, int by, int zc, void 
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.P = lay.P; g.S2 = lay.S2; g.o = lay.o;
     hipStream_t st = (hipStream_t)stream;
-    if (zc == 0) {
-        g.gx = g.gy = g.zc = g.nblk = 0;
-        const long cnt = (long)(nz + 1) * lay.S2;
-        k_stream_flat<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(g);
-    } else {
+    g.gx = g.gy = g.zc = g.nblk = 0;
+    if (zc > 0) {
         g.zc = zc;
         g.gx = (nx + 1 + 63) / 64;
         g.gy = (ny + 1 + by - 1) / by;
         const long nb = (long)g.gx * g.gy * ((nz + 1 + zc - 1) / zc);
         if (nb > (1L << 30)) return ECSG_EINVAL;
         g.nblk = (int)nb;
-        if (by == 1) k_stream_tiles<1><<<g.nblk, dim3(64, 1), 0, st>>>(g);
-        else if (by == 2) k_stream_tiles<2><<<g.nblk, dim3(64, 2), 0, st>>>(g);
-        else if (by == 4) k_stream_tiles<4><<<g.nblk, dim3(64, 4), 0, st>>>(g);
-        else k_stream_tiles<8><<<g.nblk, dim3(64, 8), 0, st>>>(g);
     }
+    const long cnt = (long)(nz + 1) * lay.S2;
+    if (pattern == 0) launch_stream<18, 6>(g, by, cnt, st);
+    else launch_stream<10, 3>(g, by, cnt, st);
     return hipGetLastError() == hipSuccess ? ECSG_OK : ECSG_EHIP;
 }
 int ecsg_step_n(ecsg_ctx *ctx, int nsteps, int use_graph) { CTX_CHECK(ctx); return ret(ctx, ctx->e->step_n(nsteps, use_graph)); }

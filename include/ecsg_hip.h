@@ -291,21 +291,24 @@ int ecsg_batch_autotune(ecsg_batch *b, int reps, int *by, int *zc);
  * kernels that use the same hints.  Asynchronous; returns ECSG_EINVAL for
  * other hints or null pointers. */
 int ecsg_copy_probe(float *dst, const float *src, uint64_t n, int hints, void *stream);
-/* The stress sweep's memory traffic without its arithmetic, for the
- * streaming bound of its traversal: 18 FP32 arrays of a radius-1 grid of
- * nx x ny x nz cells in the device layout (x fastest, ix = 0 on a 128-byte
- * line, a halo of 1, rows padded to 128 bytes; array k at base + k *
- * ecsg_stream_probe_floats(...) / 18) are read and arrays 0..5 written back
- * in place, a[k] + a[6+k] * a[12+k] in FP32, with the sweeps' non-temporal
- * hints.  zc >= 1: over the (nx+1)(ny+1)(nz+1) nodes in the stencil tiling
+/* A stencil sweep's memory traffic without its arithmetic, for the
+ * streaming bound of its traversal.  pattern 0 (the stress sweep): 18 FP32
+ * arrays are read and arrays 0..5 written back in place; pattern 1 (the
+ * velocity sweep): 10 arrays read, 0..2 written back.  Array k (of a radius-1
+ * grid of nx x ny x nz cells in the device layout: x fastest, ix = 0 on a
+ * 128-byte line, a halo of 1, rows padded to 128 bytes; array k at base + k *
+ * ecsg_stream_probe_floats(...) / 18) receives a[k] + a[k+W] + a[k+2W] + ...
+ * over the arrays read (W = the number written back), summed left to right
+ * in FP32, with the sweeps' non-temporal hints.  zc >= 1: over the (nx+1)(ny+1)(nz+1) nodes in the stencil tiling
  * (64 x by rows x zc planes per workgroup, by = 1, 2, 4 or 8, workgroups in
  * XCD order); zc = 0: one flat pass over every entry of planes 0..nz (halo
  * rows and padding included).  Asynchronous on stream (NULL = the null
- * stream); ECSG_EINVAL for other by, negative zc, empty grids or a null base.
+ * stream); ECSG_EINVAL for other by or pattern, negative zc, empty grids or
+ * a null base.
  * ecsg_stream_probe_floats returns the buffer size in floats (0 for empty
  * grids). */
 uint64_t ecsg_stream_probe_floats(int nx, int ny, int nz);
-int ecsg_stream_probe(float *base, int nx, int ny, int nz, int by, int zc, void *stream);
+int ecsg_stream_probe(float *base, int nx, int ny, int nz, int by, int zc, int pattern, void *stream);
 
 #ifdef __cplusplus
 }

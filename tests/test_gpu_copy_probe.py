@@ -41,8 +41,9 @@ def _layout(nx, ny, nz):
     return P, S2, S2 + P + 32, (S2 * (nz + 3) + line - 1) // line * line
 
 
+@pytest.mark.parametrize("pattern", [0, 1])
 @pytest.mark.parametrize("by,zc", [(4, 2), (1, 1), (8, 8), (2, 0)])
-def test_stream_probe_updates(by, zc):
+def test_stream_probe_updates(by, zc, pattern):
     L = ecsg_amd.load_library()
     nx, ny, nz = 70, 9, 3                      # two tiles in x, a partial one in y
     F = L.ecsg_stream_probe_floats(nx, ny, nz)
@@ -51,7 +52,7 @@ def test_stream_probe_updates(by, zc):
     rng = np.random.default_rng(by * 10 + zc)
     a = rng.standard_normal(F).astype(np.float32)
     buf = torch.from_numpy(a).cuda()
-    assert L.ecsg_stream_probe(buf.data_ptr(), nx, ny, nz, by, zc, None) == 0
+    assert L.ecsg_stream_probe(buf.data_ptr(), nx, ny, nz, by, zc, pattern, None) == 0
     torch.cuda.synchronize()
     got = buf.cpu().numpy().reshape(18, astride)
     ref = a.reshape(18, astride).copy()
@@ -60,15 +61,20 @@ def test_stream_probe_updates(by, zc):
     else:
         iz, iy, ix = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
         idx = (o + iz * S2 + iy * P + ix).ravel()
-    for k in range(6):
-        ref[k, idx] = ref[k, idx] + ref[6 + k, idx] * ref[12 + k, idx]
+    nr, nw = (18, 6) if pattern == 0 else (10, 3)
+    for k in range(nw):
+        u = ref[k, idx].copy()
+        for j in range(nw + k, nr, nw):
+            u = u + ref[j, idx]
+        ref[k, idx] = u
     assert np.array_equal(got, ref)
 
 
 def test_stream_probe_rejects():
     L = ecsg_amd.load_library()
     x = torch.zeros(L.ecsg_stream_probe_floats(2, 2, 2), device="cuda")
-    assert L.ecsg_stream_probe(x.data_ptr(), 2, 2, 2, 3, 1, None) == -1
-    assert L.ecsg_stream_probe(x.data_ptr(), 2, 2, 2, 4, -1, None) == -1
-    assert L.ecsg_stream_probe(x.data_ptr(), 0, 2, 2, 4, 1, None) == -1
-    assert L.ecsg_stream_probe(None, 2, 2, 2, 4, 1, None) == -1
+    assert L.ecsg_stream_probe(x.data_ptr(), 2, 2, 2, 3, 1, 0, None) == -1
+    assert L.ecsg_stream_probe(x.data_ptr(), 2, 2, 2, 4, -1, 0, None) == -1
+    assert L.ecsg_stream_probe(x.data_ptr(), 0, 2, 2, 4, 1, 0, None) == -1
+    assert L.ecsg_stream_probe(x.data_ptr(), 2, 2, 2, 4, 1, 2, None) == -1
+    assert L.ecsg_stream_probe(None, 2, 2, 2, 4, 1, 0, None) == -1
