@@ -155,11 +155,20 @@ def _f6(v: float) -> float:
     return float("%f" % v)
 
 
+def _hdr(v: float) -> float:
+    """A value the flow hands to a Madagascar program with %f (eci/SConstruct:73-103,
+    138-148) and that program writes into its output header: parsed as float32,
+    written with %g (six significant digits).  main.exe reads it back in double
+    (init_del_par_field.cpp:34-40: `real step[3]`, `real` = double): the grid
+    steps are 4.16667 and 8.33333 m, not the scripts' 4.166667 and 8.333333."""
+    return float("%g" % float(np.float32(_f6(v))))
+
+
 def reference_flow(test: int = 1, steps: Optional[int] = None, eci: bool = True) -> Model:
     """Inputs of tests/test1-cmp-eci-ii/{eci,ii}/SConstruct with par{test}.py.
 
-    Two grids at ratio 2 (384x384x72 cells at h=4.166667 m over 192x192x59
-    at 8.333333 m), top layer rho 2.5 cs 1 cp 2, lower layer rho 2.65 cs 2
+    Two grids at ratio 2 (384x384x72 cells at h=4.16667 m over 192x192x59
+    at 8.33333 m, the steps the input headers hold), top layer rho 2.5 cs 1 cp 2, lower layer rho 2.65 cs 2
     cp 4 (units m/ms, g/cm^3 as in the flow), free surface on top, Ricker
     15 Hz vx force, 4 receivers at z=58.333 m.  1172 steps by default.
     """
@@ -179,8 +188,9 @@ def reference_flow(test: int = 1, steps: Optional[int] = None, eci: bool = True)
     nx1, ny1 = 2 * nx2, 2 * ny2
     nz1 = int(lz1 / dx1 / 2) * 2
     nz2 = int(lz2 / dx2)
-    h1 = _f6(dx1)
-    h2 = _f6(dx2)
+    # grid steps as the buoyancy files' headers hold them (sfspike d#=%f -> %g)
+    h1 = _hdr(dx1)
+    h2 = _hdr(dx2)
 
     def cube(nx, ny, nz, key, split=None):
         out = np.empty((nz + 1, ny + 1, nx + 1), np.float64)
@@ -211,7 +221,9 @@ def reference_flow(test: int = 1, steps: Optional[int] = None, eci: bool = True)
     src_x = int(lx * 0.5 / dx2) * dx2
     src_y = int(ly * 0.5 / dx2) * dx2
     src_trace = ricker_sfmath(700, dt_exact / 1000.0, fpeak, 1 / fpeak)
-    sg, six, siy, siz = map_location(grids, _f6(src_x), _f6(src_y), _f6(src_z))
+    # source position: the o2..o4 keys sfmath writes into the wavelet's header (init_src.cpp:28-31)
+    src_pos = (_hdr(src_x), _hdr(src_y), _hdr(src_z))
+    sg, six, siy, siz = map_location(grids, *src_pos)
 
     rx = [int(lx * f / dx2) * dx2 for f in (0.55, 0.6, 0.65, 0.7)]
     rz = int(lz1 * 0.2 / dx2) * dx2
@@ -229,8 +241,52 @@ def reference_flow(test: int = 1, steps: Optional[int] = None, eci: bool = True)
     return Model(grids=grids, dt=dt, nsteps=nsteps, iso=True, top_bc=1, imode=1 if eci else 0,
                  src_grid=sg, src_loc=(six, siy, siz), src_type=0, src_trace=src_trace,
                  rec_loc=np.array(rec, np.int32), name="reference_test%d" % test,
-                 src_xyz=(_f6(src_x), _f6(src_y), _f6(src_z)), rec_xyz=np.array(rec_xyz, np.float32),
+                 src_xyz=src_pos, rec_xyz=np.array(rec_xyz, np.float32),
                  movie_dt=movie_dt, movie_nt=movie_nt)
+
+
+# ---------------------------------------------------------------------------
+# parameter staging (what InitParField leaves in stiffs_ptr)
+
+# staggered position of each of the nine stiffnesses (init_del_par_field.cpp:82-88,
+# the normal-stress moduli all at s11's position)
+STIFF_POS = ((1, 0, 0),) * 6 + ((1, 1, 1), (0, 0, 1), (0, 1, 0))
+
+
+def reshape_field(prim: np.ndarray, pos) -> np.ndarray:
+    """ReshapeField (field.cpp:428-476): a primary-node array (nz+1, ny+1, nx+1)
+    averaged onto staggered position pos = (px, py, pz); the eight terms are
+    summed in the reference's order, then divided by 8."""
+    px, py, pz = pos
+    nz, ny, nx = (d - 1 for d in prim.shape)
+    ez, ey, ex = nz + 1 - pz, ny + 1 - py, nx + 1 - px
+
+    def f(dx, dy, dz):
+        return prim[dz:dz + ez, dy:dy + ey, dx:dx + ex]
+
+    s = f(0, 0, 0) + f(0, 0, pz)
+    s = s + f(0, py, 0)
+    s = s + f(0, py, pz)
+    s = s + f(px, 0, 0)
+    s = s + f(px, 0, pz)
+    s = s + f(px, py, 0)
+    s = s + f(px, py, pz)
+    return s / 8
+
+
+def staged_params(m: "Model", g: int) -> List[np.ndarray]:
+    """The nine Fields stiffs_ptr[g][0..8] as InitParField builds them
+    (init_del_par_field.cpp:94-149): for isotropic input three objects --
+    s11 and s13 at (1,0,0), shared by entries 0..2 and 3..5 -- and s44 = s11 - s13
+    at (1,1,1), (0,0,1), (0,1,0); otherwise each Cij staged on its own."""
+    G = m.grids[g]
+    if m.iso:
+        s11, s13 = G.stiff
+        s44 = s11 - s13
+        a = reshape_field(s11, STIFF_POS[0])
+        b = reshape_field(s13, STIFF_POS[4])
+        return [a, a, a, b, b, b] + [reshape_field(s44, STIFF_POS[k]) for k in (6, 7, 8)]
+    return [reshape_field(G.stiff[k], STIFF_POS[k]) for k in range(9)]
 
 
 # ---------------------------------------------------------------------------

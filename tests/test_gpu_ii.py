@@ -120,8 +120,8 @@ def test_ii_golden_reference_traces():
         err = rel(tr, gold["rec_ii_test%d" % test])
         err_eci = rel(tr, gold["rec_eci_test%d" % test])
         print("golden II test%d: relL2 vs II %.3e, vs ECI %.3e" % (test, err, err_eci))
-        assert err <= 1e-4
-        assert err_eci >= 100 * err
+        assert err <= 3e-7
+        assert err_eci >= 1000 * err
         s.close()
 
 

@@ -19,7 +19,7 @@ def test_reference_flow_geometry():
     m = reference_flow(1)
     (g1, g2) = m.grids
     assert g1.n == (384, 384, 72) and g2.n == (192, 192, 59)
-    assert g1.h == (4.166667,) * 3 and g2.h == (8.333333,) * 3
+    assert g1.h == (4.16667,) * 3 and g2.h == (8.33333,) * 3
     assert m.dt == 0.589378
     assert m.nsteps == 1172
     assert m.src_loc == (192, 192, 14) and m.src_grid == 0

@@ -6,10 +6,15 @@ ratio 2, 385x385x73 + 193x193x60 nodes, 1172 steps, free surface, vx force
 Ricker) in its SVN pristine store; tests/golden/reference_traces.npz holds
 them, decoded by tests/golden/decode_reference_traces.py.
 
-Over the full 1172 steps the oracle reproduces the ECI traces to 1.44e-5
-(test1) and 1.34e-5 (test2) relative L2, against 1.38e-2 / 7.2e-2 for the
-II-mode traces of the same flows; the full runs take minutes on 8 cores and
-are marked slow.  The default suite checks prefixes.
+Over the full 1172 steps the oracle reproduces the ECI traces of tests
+1/2/3 to 1.35e-7 / 1.36e-7 / 1.0e-7 relative L2 and the II traces to
+1.35e-7 / 1.36e-7 / 6.4e-8, against 1.38e-2 / 7.2e-2 / 0.83 between the
+modes; that is the size of the float32 rounding of the golden samples.  The
+grid steps are those the input files' headers hold: sfspike writes d1..d3
+with %g, so the reference reads h = 4.16667 and 8.33333 m (the flow script's
+4.166667 and 8.333333 leave a 1.4e-5 residual, a delay that grows with
+offset).  The full runs take minutes on 8 cores and are marked slow.  The
+default suite checks prefixes.
 
 The II flows (tests/test1-cmp-eci-ii/ii, imode=0) are matched with the
 x-end vz ghost edge left unwritten (ii_xend 0), as the committed version of
@@ -39,7 +44,7 @@ def test_golden_file_layout():
     assert np.allclose(geo[:, 0], [875.0, 958.3333, 1033.3334, 1116.6666], atol=1e-3)
 
 
-@pytest.mark.parametrize("test,steps,tol", [(1, 150, 2e-5), (3, 200, 1e-4)])
+@pytest.mark.parametrize("test,steps,tol", [(1, 150, 2e-7), (3, 200, 2e-7)])
 def test_oracle_matches_golden_prefix(test, steps, tol):
     m = reference_flow(test)
     o = Oracle(m, eci_mode=0, check_energy=False)
@@ -61,7 +66,7 @@ def test_oracle_matches_golden_full(test):
     g = GOLD["rec_eci_test%d" % test]
     err = rel(tr, g)
     err_ii = rel(tr, GOLD["rec_ii_test%d" % test])
-    assert err <= 3e-5 and err_ii >= 100 * err
+    assert err <= 3e-7 and err_ii >= 1000 * err
     e = o.energy()
     assert abs(e[-1] - e[701]) / e[701] <= 1e-6
 
@@ -77,7 +82,7 @@ def test_oracle_ii_matches_golden_prefix():
     tr = o.traces()
     err = rel(tr, GOLD["rec_ii_test3"][:, :200])
     err_eci = rel(tr, GOLD["rec_eci_test3"][:, :200])
-    assert err <= 1e-4, err
+    assert err <= 2e-7, err
     assert err_eci >= 1000 * err
     o.close()
 
@@ -91,5 +96,5 @@ def test_oracle_ii_matches_golden_full(test):
     tr = o.traces()
     err = rel(tr, GOLD["rec_ii_test%d" % test])
     err_eci = rel(tr, GOLD["rec_eci_test%d" % test])
-    assert err <= 1e-4 and err_eci >= 100 * err, (err, err_eci)
+    assert err <= 3e-7 and err_eci >= 1000 * err, (err, err_eci)
     o.close()
