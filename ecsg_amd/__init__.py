@@ -35,7 +35,8 @@ ABI_SYMBOLS = (
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
     "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_stream_probe_floats", "ecsg_stream_probe",
-    "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed",
+    "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed", "ecsg_set_barrier_timeout",
+    "ecsg_set_params_staged", "ecsg_cfl_dt",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -102,6 +103,9 @@ def load_library(path: Optional[str] = None):
     L.ecsg_set_overlap.argtypes = [vp, i]
     L.ecsg_overlap_active.argtypes = [vp]
     L.ecsg_step_n_timed.argtypes = [vp, i, P(d)]
+    L.ecsg_set_barrier_timeout.argtypes = [vp, i]
+    L.ecsg_set_params_staged.argtypes = [vp, i, P(d), P(P(d))]
+    L.ecsg_cfl_dt.argtypes = [P(EcsgConfig), P(P(d)), P((P(d) * 9)), i, d, P(d)]
     L.ecsg_field_dims.argtypes = [vp, i, i, P(i)]
     L.ecsg_get_field.argtypes = [vp, i, i, P(d)]
     L.ecsg_set_field.argtypes = [vp, i, i, P(d)]
@@ -137,6 +141,33 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def cfl_dt(m, frac: float = 0.9) -> float:
+    """ecsg_cfl_dt: frac x min over grids of h_min / (vp sqrt 3), vp^2 the
+    largest max(C11,C22,C33) x b, reduced on the device in FP64."""
+    L = load_library()
+    cfg = EcsgConfig()
+    cfg.num_grids = m.num_grids
+    keep = []
+    buoys = (C.POINTER(C.c_double) * m.num_grids)()
+    stiffs = ((C.POINTER(C.c_double) * 9) * m.num_grids)()
+    for g, G in enumerate(m.grids):
+        for k in range(3):
+            cfg.n[g][k] = int(G.n[k])
+            cfg.h[g][k] = float(G.h[k])
+        b = np.ascontiguousarray(G.buoy, np.float64)
+        keep.append(b)
+        buoys[g] = _dp(b)
+        for k, a in enumerate(list(G.stiff)[:9]):
+            a = np.ascontiguousarray(a, np.float64)
+            keep.append(a)
+            stiffs[g][k] = _dp(a)
+    out = C.c_double()
+    rc = L.ecsg_cfl_dt(C.byref(cfg), buoys, stiffs, int(m.iso), float(frac), C.byref(out))
+    if rc != 0:
+        raise EcsgError("ecsg_cfl_dt failed (%d)" % rc)
+    return out.value
+
+
 class Simulation:
     """Device state of one composite model (all grids), advanced on one stream.
 
@@ -145,7 +176,8 @@ class Simulation:
     """
 
     def __init__(self, m, precision: str = "f64", check_energy: bool = True, max_steps: Optional[int] = None,
-                 vy_mirror_all: bool = False, sxz_solver: str = "jacobi", ii_vz_xend: bool = False):
+                 vy_mirror_all: bool = False, sxz_solver: str = "jacobi", ii_vz_xend: bool = False,
+                 staged: bool = False):
         self.L = load_library()
         self.model = m
         cfg = EcsgConfig()
@@ -166,6 +198,7 @@ class Simulation:
         cfg.ii_vz_xend = int(ii_vz_xend)
         self.max_steps = cfg.max_steps
         self.precision = precision
+        self._staged = set()
         self.h = C.c_void_p()
         rc = self.L.ecsg_create(C.byref(cfg), C.byref(self.h))
         if rc != 0:
@@ -174,7 +207,10 @@ class Simulation:
             self.h = None
             raise EcsgError("ecsg_create failed (%d): %s" % (rc, msg))
         for g in range(m.num_grids):
-            self._set_params(g)
+            if staged:
+                self.set_params_staged(g)
+            else:
+                self._set_params(g)
         if m.src_trace is not None and len(m.src_trace):
             tr = np.ascontiguousarray(m.src_trace, np.float32)
             self._chk(self.L.ecsg_set_source(self.h, m.src_grid, *map(int, m.src_loc), m.src_type,
@@ -193,6 +229,26 @@ class Simulation:
         arr = (C.POINTER(C.c_double) * 9)(*[_dp(a) for a in st])
         b = np.ascontiguousarray(G.buoy, np.float64)
         self._chk(self.L.ecsg_set_params(self.h, g, _dp(b), arr, int(self.model.iso)))
+
+    def set_params_staged(self, g, stiff=None, buoy=None):
+        """ecsg_set_params_staged: the nine staged Fields (default: the model's
+        parameters staged as InitParField does, isotropic entries aliased)."""
+        from .model import staged_params
+        st = staged_params(self.model, g) if stiff is None else stiff
+        keep = {}
+        ptrs = []
+        for a in st:
+            if id(a) not in keep:
+                keep[id(a)] = np.ascontiguousarray(a, np.float64)
+            ptrs.append(_dp(keep[id(a)]))
+        arr = (C.POINTER(C.c_double) * 9)(*ptrs)
+        b = np.ascontiguousarray(self.model.grids[g].buoy if buoy is None else buoy, np.float64)
+        self._chk(self.L.ecsg_set_params_staged(self.h, g, _dp(b), arr))
+        self._staged.add(g)
+
+    def set_barrier_timeout(self, ms: int):
+        """Longest wait at one grid barrier of the cooperative Jacobi solve."""
+        self._chk(self.L.ecsg_set_barrier_timeout(self.h, int(ms)))
 
     def close(self):
         if getattr(self, "h", None):
@@ -318,9 +374,13 @@ class Simulation:
 
     def grad(self, g: int, sel: int) -> np.ndarray:
         """dJ/d(input) at primary nodes, (nz+1, ny+1, nx+1): sel 9 = buoyancy,
-        10 + m = stiff[m] of set_params."""
+        10 + m = stiff[m] of set_params; for a grid given staged Fields, the
+        gradient w.r.t. staged Field m in its own shape."""
         G = self.model.grids[g]
-        out = np.empty((G.n[2] + 1, G.n[1] + 1, G.n[0] + 1), np.float64)
+        if sel >= 10 and g in self._staged:
+            out = np.empty(self.dims(g, sel), np.float64)
+        else:
+            out = np.empty((G.n[2] + 1, G.n[1] + 1, G.n[0] + 1), np.float64)
         self._chk(self.L.ecsg_get_gradient(self.h, g, sel, _dp(out)))
         return out
 

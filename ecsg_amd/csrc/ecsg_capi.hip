@@ -23,11 +23,18 @@
 
 using namespace ecsg;
 
-// Cooperative (grid-barrier) launches of every handle on one device are
-// issued in one order, each behind the previous one: two such kernels from
-// different streams could each hold part of the GPU while waiting at a grid
-// barrier for workgroups that the other keeps from being placed.  The
-// per-device events live until the process ends.
+// Grid-barrier launches (the tiled and register Jacobi kernels).  Their
+// barrier is the library's own (grid_barrier: device-scope atomics with a
+// bounded wait), so they are ordinary launches: the runtime's cooperative
+// launch would route them through the global-wave-sync queue, which one
+// process holds at a time (a second process then gets "illegal state").  The
+// callers check that every workgroup of the launch can be resident at once
+// (occupancy x CUs), which is what the barrier needs.  Launches of every
+// handle on one device are issued in one order, each behind the previous
+// one: two such kernels from different streams could otherwise each hold
+// part of the GPU while waiting at a barrier for workgroups that the other
+// keeps from being placed.  The per-device events live until the process
+// ends.
 static std::mutex coop_mu;
 static hipEvent_t coop_last[64];
 
@@ -46,7 +53,7 @@ static hipError_t launch_cooperative(const void *fn, dim3 grid, dim3 block, void
         e = hipStreamWaitEvent(s, ev, 0);
         if (e != hipSuccess) return e;
     }
-    e = hipLaunchCooperativeKernel(fn, grid, block, args, 0, s);
+    e = hipLaunchKernel(fn, grid, block, args, 0, s);
     if (e != hipSuccess) return e;
     return hipEventRecord(ev, s);
 }
@@ -98,6 +105,7 @@ public:
     }
     virtual int init(const ecsg_config &c) = 0;
     virtual int set_params(int g, const double *buoy, const double *const stiff[9], int iso) = 0;
+    virtual int set_params_staged(int g, const double *buoy, const double *const stiff[9]) = 0;
     virtual int set_source(int g, int ix, int iy, int iz, int type, const float *tr, int len) = 0;
     virtual int set_receivers(int n, const int *loc) = 0;
     virtual int update_vel(int g) = 0;
@@ -135,6 +143,7 @@ public:
     virtual int set_fused(int on) = 0;
     virtual int set_overlap(int on) = 0;
     virtual int overlap_active() const = 0;
+    virtual int set_barrier_timeout(int ms) = 0;
 };
 
 #define HIPCHK(x)                                                                                  \
@@ -151,6 +160,7 @@ struct DevGrid {
     T *base = nullptr;
     GridDev<T> dev{};
     bool params = false;
+    bool staged = false;  // parameters given at their staggered positions (ecsg_set_params_staged)
     int iso = 0;
     double *ke = nullptr, *se = nullptr;  // energy partials, nblk each
     int by = 4;                           // workgroup height of the stencil tiles
@@ -207,6 +217,13 @@ public:
     int max_coop = 0;  // co-resident workgroups of the register Jacobi kernel
     int max_tiled = 0; // co-resident workgroups of the tiled Jacobi kernel
     int max_tiled_few = 0;  // the same for its JTT_FEW-thread form
+    int ncu = 0;          // compute units of the device
+    // grid barrier state of the cooperative Jacobi kernels (arrivals, exits,
+    // timeout flag) and the longest wait at one barrier, in ticks of the
+    // device's constant-rate wall clock
+    unsigned *d_bar = nullptr;
+    unsigned long long bar_limit = 0;
+    int bar_ms = 10000;
     bool lds_ok = false;  // every interface fits the single-workgroup k_jacobi_lds
     // With sxz_solver 0 the single-model step uses k_jacobi_lds
     // (graph-capturable) for iterates of up to LDS_PREF_MAXW entries and the
@@ -260,6 +277,7 @@ public:
             hipFree(f.Bs); hipFree(f.A); hipFree(f.absp); hipFree(f.X0); hipFree(f.X1);
             hipFree(f.jpart); hipFree(f.stats);
         }
+        hipFree(d_bar);
         hipFree(d_rec); hipFree(d_traces); hipFree(d_step); hipFree(d_energy); hipFree(d_fix); hipFree(d_diag); hipFree(e_save); hipFree(d_sbp);
         hipFree(d_src_old); hipFree(d_src_loc); hipFree(d_src_tr);
         adj_free();
@@ -373,7 +391,7 @@ public:
             const long nxw = (long)(f.nx2 + 1) * (f.ny2 + 1);
             if (int rc = dalloc(&f.X0, nxw)) return rc;
             if (int rc = dalloc(&f.X1, nxw)) return rc;
-            int dev = 0, ncu = 0, per = 0;
+            int dev = 0, per = 0;
             HIPCHK(hipGetDevice(&dev));
             HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
             HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_jacobi<T>, JT, 0));
@@ -415,6 +433,10 @@ public:
                                        (int)(2 * JL_MAXW * sizeof(double))));
             HIPCHK(hipFuncSetAttribute((const void *)k_jacobi_lds_batch<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)(2 * JL_MAXW * sizeof(double))));
+        }
+        if (!I.empty()) {
+            if (int rc = dalloc(&d_bar, 4)) return rc;
+            if (int rc = set_barrier_timeout(bar_ms)) return rc;
         }
         if (int rc = dalloc(&d_step, 1)) return rc;
         if (int rc = dalloc(&d_energy, cfg.max_steps)) return rc;
@@ -629,15 +651,33 @@ public:
             if (c == 0 || c == 1 || c == 2 || c >= 6)
                 for (double v : out)
                     if (!(v > 0)) return fail(ECSG_EINVAL, "c11, c22, c33, c44, c55, c66 must be positive");
-            if (int rc = upload_domain(g, 9 + c, ST_POS[c], out)) return rc;
-            if (c == 2) {
-                const long pl = (long)d.n[0] * (d.n[1] + 1);
-                d.c33_top.assign(out.begin(), out.begin() + pl);
-                d.c33_bot.assign(out.end() - pl, out.end());
-                for (auto &x : d.c33_top) x = (double)(T)x;
-                for (auto &x : d.c33_bot) x = (double)(T)x;
-            }
+            if (int rc = upload_stiff(g, c, out)) return rc;
         }
+        d.staged = false;
+        return finish_params(g, buoy, iso);
+    }
+
+    // stiffness c (staged values at ST_POS[c], domain extents) into the device
+    int upload_stiff(int g, int c, const std::vector<double> &out)
+    {
+        DevGrid<T> &d = G[g];
+        if (int rc = upload_domain(g, 9 + c, ST_POS[c], out)) return rc;
+        if (c == 2) {
+            const long pl = (long)d.n[0] * (d.n[1] + 1);
+            d.c33_top.assign(out.begin(), out.begin() + pl);
+            d.c33_bot.assign(out.end() - pl, out.end());
+            for (auto &x : d.c33_top) x = (double)(T)x;
+            for (auto &x : d.c33_bot) x = (double)(T)x;
+        }
+        return ECSG_OK;
+    }
+
+    // buoyancy (primary nodes) and everything that follows from a grid's parameters
+    int finish_params(int g, const double *buoy, int iso)
+    {
+        DevGrid<T> &d = G[g];
+        static const int p0[3] = {0, 0, 0};
+        const long np = (long)(d.n[0] + 1) * (d.n[1] + 1) * (d.n[2] + 1);
         std::vector<double> b(buoy, buoy + np);
         if (int rc = upload_domain(g, 18, p0, b)) return rc;
         const long pl = (long)(d.n[0] + 1) * (d.n[1] + 1);
@@ -654,6 +694,42 @@ public:
                 if (int rc = build_iface(k)) return rc;
         invalidate_graph();
         return ECSG_OK;
+    }
+
+    // The parameters main.cpp holds after InitParField
+    // (init_del_par_field.cpp:77-150): stiff[m] = stiffs_ptr[g][m], the Field
+    // of stiffness m at its staggered position ST_POS[m] (halo 0, x fastest,
+    // n or n+1 entries per dimension as Field::Allocate sizes them,
+    // field.cpp:85-98), and buoys[g] at the primary nodes.  The values are
+    // taken as they are (no averaging).  The isotropic compliance of the
+    // energy (init_del_par_field.cpp:259-278) is used when stiff[0..2] are one
+    // Field and stiff[3..5] another, as InitParField's pointers are.
+    int set_params_staged(int g, const double *buoy, const double *const stiff[9]) override
+    {
+        coef_ok = false;
+        if (g < 0 || g >= (int)G.size()) return fail(ECSG_EINVAL, "grid index out of range");
+        if (!buoy || !stiff) return fail(ECSG_EINVAL, "null parameter array");
+        DevGrid<T> &d = G[g];
+        const long np = (long)(d.n[0] + 1) * (d.n[1] + 1) * (d.n[2] + 1);
+        for (long q = 0; q < np; q++)
+            if (!(buoy[q] > 0) || !std::isfinite(buoy[q])) return fail(ECSG_EINVAL, "buoyancy must be positive and finite");
+        for (int c = 0; c < 9; c++)
+            if (!stiff[c]) return fail(ECSG_EINVAL, "null stiffness array");
+        std::vector<double> out;
+        for (int c = 0; c < 9; c++) {
+            const int *pos = ST_POS[c];
+            const long len = (long)(pos[0] ? d.n[0] : d.n[0] + 1) * (pos[1] ? d.n[1] : d.n[1] + 1) *
+                             (pos[2] ? d.n[2] : d.n[2] + 1);
+            out.assign(stiff[c], stiff[c] + len);
+            for (double v : out) {
+                if (!std::isfinite(v)) return fail(ECSG_EINVAL, "stiffness must be finite");
+                if ((c <= 2 || c >= 6) && !(v > 0)) return fail(ECSG_EINVAL, "c11, c22, c33, c44, c55, c66 must be positive");
+            }
+            if (int rc = upload_stiff(g, c, out)) return rc;
+        }
+        const int iso = stiff[0] == stiff[1] && stiff[1] == stiff[2] && stiff[3] == stiff[4] && stiff[4] == stiff[5];
+        d.staged = true;
+        return finish_params(g, buoy, iso);
     }
 
     static double hat(int a, int r) { return (double)(r - std::abs(a)) / (double)r; }
@@ -818,6 +894,48 @@ public:
         return ECSG_OK;
     }
     int overlap_active() const override { return fused_ok() ? 0 : overlap_mode(); }
+
+    // Longest wait at one grid barrier of the cooperative Jacobi kernels;
+    // past it the kernel ends early and sync() reports ECSG_ETIMEOUT.
+    int set_barrier_timeout(int ms) override
+    {
+        if (ms < 0) return fail(ECSG_EINVAL, "barrier timeout must be >= 0 ms");
+        bar_ms = ms;
+        int dev = 0, khz = 0;
+        HIPCHK(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;  // the 100 MHz constant clock of CDNA devices
+        bar_limit = (unsigned long long)ms * (unsigned long long)khz;
+        return ECSG_OK;
+    }
+    // A timed-out grid barrier: clear the state for the next launch and
+    // report it (the solve it belonged to ended early, so the state is wrong)
+    int check_barrier()
+    {
+        if (!d_bar) return ECSG_OK;
+        unsigned h[3] = {0, 0, 0};
+        HIPCHK(hipMemcpy(h, d_bar, sizeof(h), hipMemcpyDeviceToHost));
+        if (!h[2]) return ECSG_OK;
+        HIPCHK(hipMemset(d_bar, 0, 4 * sizeof(unsigned)));
+        return fail(ECSG_ETIMEOUT, "a grid barrier of the cooperative Jacobi solve timed out; the wavefield is invalid");
+    }
+    // Each cooperative launch of the step has at most one workgroup per
+    // compute unit.  Only then may it run beside the stencils on the other
+    // stream: its workgroups are placed as the stencil workgroups leave, one
+    // per CU, while a launch with several per CU (e.g. 757 tiles of 16 x 16
+    // unknowns on 256 CUs) waits at its first barrier for room the stencils
+    // hold; such launches get the GPU to themselves (serial step).
+    bool coop_one_per_cu() const
+    {
+        if (use_lds()) return true;  // no cooperative launch in the step
+        for (int k0 = 0; k0 < (int)I.size(); k0 += MAXI) {
+            long tot = 0;
+            for (int k = k0; k < std::min((int)I.size(), k0 + MAXI); k++)
+                tot += cfg.sxz_solver == 0 ? I[k].tiles : I[k].coop;
+            if (tot > ncu) return false;
+        }
+        return true;
+    }
 
     int set_fused(int on) override
     {
@@ -1204,6 +1322,8 @@ public:
     // threads per tile for launches of at most JTT_FEW_TILES tiles, else JTT
     int launch_tiled(JacBatch<T> &jb, int blk)
     {
+        jb.bar = d_bar;
+        jb.bar_limit = bar_limit;
         void *args[] = {(void *)&jb};
         if (blk <= JTT_FEW_TILES && blk <= max_tiled_few) {
             HIPCHK(launch_cooperative((const void *)k_jacobi_tiled<T, JTT_FEW>, dim3(blk), dim3(JTT_FEW), args, s));
@@ -1248,6 +1368,8 @@ public:
             if (int rc = launch_tiled(jb, blk)) return rc;
         } else {
             if (blk > max_coop) return fail(ECSG_EINVAL, "interfaces too large for one resident Jacobi launch");
+            jb.bar = d_bar;
+            jb.bar_limit = bar_limit;
             HIPCHK(launch_cooperative((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, s));
         }
         // fine ghosts from the coarse ghosts just written
@@ -1366,6 +1488,7 @@ public:
     bool overlap_ok() const
     {
         if (G.size() < 2 || cfg.radius != 1 || cfg.imode != 1 || cfg.check_energy) return false;
+        if (!coop_one_per_cu()) return false;
         if (overlap_on >= 0) return overlap_on != 0;
         return !use_lds();
     }
@@ -1714,9 +1837,10 @@ public:
         if (fused_ok()) return step_fused(n, use_graph);
         if (overlap_ok())  // before any capture
             if (int rc = ensure_s2()) return rc;
-        // The tiled interface Jacobi is a cooperative launch; its behaviour
-        // inside a captured graph is not relied on, so composite models with
-        // interfaces too large for the LDS solver run eagerly.
+        // The tiled interface Jacobi is a grid-barrier launch ordered behind
+        // every other handle's through a process-wide event, which a captured
+        // graph cannot hold, so composite models with interfaces too large
+        // for the LDS solver run eagerly.
         if (use_graph && !graph_failed && !prof && (I.empty() || cfg.imode == 0 || (use_lds() && cfg.sxz_solver == 0))) {
             if (!gexec) {
                 if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
@@ -1756,6 +1880,7 @@ public:
     {
         if (int rc = ov_join()) return rc;
         HIPCHK(hipStreamSynchronize(s));
+        if (int rc = check_barrier()) return rc;
         for (int k = 0; k < (int)I.size(); k++) {
             long long st3[3];
             HIPCHK(hipMemcpy(st3, I[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
@@ -2540,6 +2665,7 @@ public:
             }
         }
         HIPCHK(hipStreamSynchronize(s));
+        if (int rc = check_barrier()) return rc;
         // residuals and misfit
         std::vector<T> tr((size_t)std::max(nrec, 1) * cfg.max_steps);
         if (nrec) HIPCHK(hipMemcpy(tr.data(), d_traces, sizeof(T) * nrec * cfg.max_steps, hipMemcpyDeviceToHost));
@@ -2574,6 +2700,7 @@ public:
                 if (int rc = adj_step_impl(seg0(j) + t, t + 1, t, true)) return rc;
         }
         HIPCHK(hipStreamSynchronize(s));
+        if (int rc = check_barrier()) return rc;
         host_step = nseg > 1 ? r0 : N;  // the state the last recomputation left
         for (int k = 0; k < (int)I.size(); k++) {
             long long st3[3];
@@ -2607,6 +2734,15 @@ public:
                 const int *pos = ST_POS[m];
                 const int e0 = pos[0] ? n0 - 1 : n0, e1 = pos[1] ? n1 - 1 : n1, e2 = pos[2] ? n2 - 1 : n2;
                 std::vector<double> &o = gm[m];
+                if (d.staged) {
+                    // w.r.t. the staged Field itself, in its own layout
+                    o.assign((size_t)(e0 + 1) * (e1 + 1) * (e2 + 1), 0.0);
+                    long q = 0;
+                    for (int k = 0; k <= e2; k++)
+                        for (int j = 0; j <= e1; j++)
+                            for (int i = 0; i <= e0; i++, q++) o[q] = (double)hg[(size_t)m * as + DI(i, j, k)];
+                    continue;
+                }
                 for (int k = 0; k <= e2; k++)
                     for (int j = 0; j <= e1; j++)
                         for (int i = 0; i <= e0; i++) {
@@ -2616,7 +2752,7 @@ public:
                                 o[PI(i + ((c >> 2) & 1) * pos[0], j + ((c >> 1) & 1) * pos[1], k + (c & 1) * pos[2])] += v;
                         }
             }
-            if (d.iso) {
+            if (d.iso && !d.staged) {
                 std::vector<double> s11((size_t)np, 0.0), s13((size_t)np, 0.0);
                 for (long q = 0; q < np; q++) {
                     s11[q] = gm[0][q] + gm[1][q] + gm[2][q] + gm[6][q] + gm[7][q] + gm[8][q];
@@ -2673,6 +2809,7 @@ public:
         }
         if (int rc = adj_step_impl(0, -1, -1, false)) return rc;
         HIPCHK(hipStreamSynchronize(s));
+        if (int rc = check_barrier()) return rc;
         for (int k = 0; k < (int)I.size(); k++) {
             long long st3[3];
             HIPCHK(hipMemcpy(st3, AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
@@ -3189,6 +3326,11 @@ int ecsg_set_params(ecsg_ctx *ctx, int g, const double *buoy, const double *cons
     CTX_CHECK(ctx);
     return ret(ctx, ctx->e->set_params(g, buoy, stiff, iso));
 }
+int ecsg_set_params_staged(ecsg_ctx *ctx, int g, const double *buoy, const double *const stiff[9])
+{
+    CTX_CHECK(ctx);
+    return ret(ctx, ctx->e->set_params_staged(g, buoy, stiff));
+}
 int ecsg_set_source(ecsg_ctx *ctx, int g, int ix, int iy, int iz, int type, const float *trace, int len)
 {
     CTX_CHECK(ctx);
@@ -3225,6 +3367,51 @@ int ecsg_copy_probe(float *dst, const float *src, uint64_t n, int hints, void *s
         else k_copy_probe<0, false><<<blocks, 256, 0, st>>>(dst, src, n);
     }
     return hipGetLastError() == hipSuccess ? ECSG_OK : ECSG_EHIP;
+}
+int ecsg_cfl_dt(const ecsg_config *cfg, const double *const buoy[], const double *const stiff[][9], int iso,
+                double frac, double *dt_out)
+{
+    if (!cfg || !buoy || !stiff || !dt_out || !(frac > 0) || cfg->num_grids < 1 || cfg->num_grids > ECSG_MAX_GRIDS)
+        return ECSG_EINVAL;
+    double best = INFINITY;
+    for (int g = 0; g < cfg->num_grids; g++) {
+        const int *n = cfg->n[g];
+        const double *h = cfg->h[g];
+        if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(h[0] > 0) || !(h[1] > 0) || !(h[2] > 0)) return ECSG_EINVAL;
+        const double *src[4] = {buoy[g], stiff[g][0], iso ? nullptr : stiff[g][1], iso ? nullptr : stiff[g][2]};
+        const int na = iso ? 2 : 4;
+        for (int a = 0; a < na; a++)
+            if (!src[a]) return ECSG_EINVAL;
+        const long np = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+        for (int a = 0; a < na; a++)
+            for (long q = 0; q < np; q++)
+                if (!(src[a][q] > 0) || !std::isfinite(src[a][q])) return ECSG_EINVAL;
+        const int nblk = (int)std::min<long>((np + 255) / 256, 2048);
+        double *d = nullptr;
+        if (hipMalloc((void **)&d, sizeof(double) * ((size_t)na * np + nblk + 1)) != hipSuccess) return ECSG_ENOMEM;
+        double *dv[4] = {nullptr, nullptr, nullptr, nullptr};
+        int rc = ECSG_OK;
+        for (int a = 0; a < na && rc == ECSG_OK; a++) {
+            dv[a] = d + (size_t)a * np;
+            if (hipMemcpy(dv[a], src[a], sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess) rc = ECSG_EHIP;
+        }
+        double *part = d + (size_t)na * np, *res = part + nblk;
+        double m = 0.0;
+        if (rc == ECSG_OK) {
+            hipLaunchKernelGGL(k_cfl_part, dim3(nblk), dim3(256), 0, 0, dv[0], dv[1], dv[2], dv[3], np, part);
+            hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(64), 0, 0, part, nblk, res);
+            if (hipGetLastError() != hipSuccess || hipMemcpy(&m, res, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                rc = ECSG_EHIP;
+        }
+        hipFree(d);
+        if (rc != ECSG_OK) return rc;
+        // h_min / (vp sqrt 3), vp = sqrt(max c b)
+        const double vp = std::sqrt(m);
+        const double hmin = std::min(std::min(h[0], h[1]), h[2]);
+        best = std::min(best, hmin / (vp * std::sqrt(3.0)));
+    }
+    *dt_out = frac * best;
+    return ECSG_OK;
 }
 uint64_t ecsg_stream_probe_floats(int nx, int ny, int nz)
 {
@@ -3266,6 +3453,7 @@ int ecsg_step_n_timed(ecsg_ctx *ctx, int nsteps, double ms_out[2])
     return ret(ctx, ctx->e->step_timed(nsteps, ms_out));
 }
 int ecsg_overlap_active(const ecsg_ctx *ctx) { return ctx && ctx->e ? ctx->e->overlap_active() : 0; }
+int ecsg_set_barrier_timeout(ecsg_ctx *ctx, int ms) { CTX_CHECK(ctx); return ret(ctx, ctx->e->set_barrier_timeout(ms)); }
 int ecsg_steps_done(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->steps_done() : -1; }
 int ecsg_synchronize(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sync()); }
 int ecsg_field_dims(const ecsg_ctx *ctx, int g, int sel, int len[3])

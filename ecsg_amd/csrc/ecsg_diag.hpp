@@ -102,4 +102,39 @@ __global__ __launch_bounds__(64) void k_sbp_final(const double *__restrict__ par
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------------------
+// CFL bound of the flows (eci/SConstruct:107-114): the largest
+// max(c11, c22, c33) * b over the primary nodes of a grid, in FP64.  A max of
+// correctly rounded products: the same value in any order.  c1 == nullptr:
+// the isotropic input, c0 = s11 = rho vp^2.
+__global__ __launch_bounds__(256) void k_cfl_part(const double *__restrict__ b, const double *__restrict__ c0,
+                                                  const double *__restrict__ c1, const double *__restrict__ c2,
+                                                  long n, double *__restrict__ part)
+{
+    __shared__ double lds[4];
+    double m = 0.0;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        double c = c0[q];
+        if (c1) {
+            c = fmax(c, c1[q]);
+            c = fmax(c, c2[q]);
+        }
+        m = fmax(m, c * b[q]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+}
+
+__global__ __launch_bounds__(64) void k_cfl_final(const double *__restrict__ part, int n, double *__restrict__ out)
+{
+    double m = 0.0;
+    for (int q = threadIdx.x; q < n; q += 64) m = fmax(m, part[q]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x == 0) out[0] = m;
+}
+
 }  // namespace ecsg

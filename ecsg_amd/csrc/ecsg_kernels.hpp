@@ -18,7 +18,6 @@
 // reference arithmetic operation for operation.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <hip/hip_cooperative_groups.h>
 
 namespace ecsg {
 
@@ -759,7 +758,63 @@ struct JacBatch {
     int cap;
     int cheby;                // 1: Chebyshev acceleration of the Jacobi sweeps
     int solve_only;           // 1: leave the solution in X0 only (no ghost scatter)
+    unsigned *bar;            // grid barrier state: arrivals, exits, timeout flag
+    unsigned long long bar_limit;  // longest wait at one barrier, wall-clock ticks
 };
+
+// Grid barrier of the cooperative Jacobi kernels, with a bounded wait.
+// bar[0] counts arrivals over the launch (barrier number g is passed once it
+// reaches g * nblk), bar[1] counts workgroups that have finished, bar[2] is a
+// timeout flag.  A workgroup that waits more than `limit` ticks of the
+// constant-rate wall clock raises the flag; every waiting or later-arriving
+// workgroup sees it and gets false back, and the kernel ends early with the
+// flag left for the host to report, instead of waiting for ever on a
+// workgroup that cannot be placed.  limit = 0 raises the flag at the first
+// barrier (the host's test of the error path).  Release / acquire at device scope around
+// the arrival, as a grid barrier needs (the L2s of the XCDs are not coherent
+// with each other).
+__device__ __forceinline__ bool grid_barrier(unsigned *bar, unsigned nblk, unsigned &gen, unsigned long long limit)
+{
+    __shared__ int s_ok;
+    __syncthreads();
+    gen++;
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned target = gen * nblk;
+        __threadfence();
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (limit == 0) __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // forced
+        const unsigned long long t0 = wall_clock64();
+        while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) { ok = 0; break; }
+            if (wall_clock64() - t0 > limit) {
+                __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// End of a kernel that used grid_barrier: the last workgroup to finish
+// (every other one has passed its last barrier by then) zeroes the counters
+// for the next launch.  The timeout flag stays until the host clears it.
+__device__ __forceinline__ void grid_barrier_done(unsigned *bar, unsigned nblk)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned old = __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == nblk - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
 
 // Jacobi2D + Norm2D (update_ghost_sxz.cpp:314-388) for all interfaces of a
 // composite model in one cooperative persistent kernel: the iterations of the
@@ -775,8 +830,7 @@ struct JacBatch {
 template <typename T>
 __global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
 {
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
+    unsigned bgen = 0;  // grid barriers passed
     __shared__ double lds[JT / 64];
     __shared__ double lds2[JT / 64];
     __shared__ double s_tol[MAXI], s_n2[MAXI], s_x2[MAXI];
@@ -880,7 +934,10 @@ __global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
             }
             double *tsw = Xo; Xo = Xn; Xn = tsw;
         }
-        grid.sync();
+        if (!grid_barrier(jb.bar, gridDim.x, bgen, jb.bar_limit)) {
+            grid_barrier_done(jb.bar, gridDim.x);  // timed out: the host reports it
+            return;
+        }
         // every workgroup evaluates every active interface's exit test from
         // the same partial sums in the same order
         if (threadIdx.x < 64)
@@ -936,6 +993,7 @@ __global__ __launch_bounds__(JT) void k_jacobi(JacBatch<T> jb)
         J.stats[1] = iters;
         if (iters >= jb.cap) J.stats[2] = 1;
     }
+    grid_barrier_done(jb.bar, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -968,8 +1026,7 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_jacobi_tiled(JacBatch<T> jb)
 {
     constexpr int KJ = (RJ * RJ + NT - 1) / NT;  // region points per thread
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
+    unsigned bgen = 0;  // grid barriers passed
     __shared__ double xb[2][EJ * EJ];
     __shared__ double stash[SJ][TJ * TJ];
     __shared__ double red[NT / 64][2 * SJ];
@@ -1098,7 +1155,10 @@ __global__ __launch_bounds__(NT) void k_jacobi_tiled(JacBatch<T> jb)
                 if (I <= n0 && Jg <= n1) Xn[(long)Jg * w + I] = stash[SJ - 1][p];
             }
         }
-        grid.sync();
+        if (!grid_barrier(jb.bar, gridDim.x, bgen, jb.bar_limit)) {
+            grid_barrier_done(jb.bar, gridDim.x);  // timed out: the host reports it
+            return;
+        }
         // every workgroup replays every active interface's per-sweep exit test;
         // one pass over the workgroups' partials gathers all 2*SJ sums at once
         // (lane-strided sums in increasing workgroup order, then the butterfly)
@@ -1172,6 +1232,7 @@ __global__ __launch_bounds__(NT) void k_jacobi_tiled(JacBatch<T> jb)
         J.stats[1] = iters;
         if (iters >= jb.cap) J.stats[2] = 1;
     }
+    grid_barrier_done(jb.bar, gridDim.x);
 }
 
 // Fine ghosts sxz1(i, j, nz1) of the vx-sxz interfaces from the coarse ghosts

@@ -25,6 +25,9 @@
 //                       edge (update_ghost_vz.cpp:98-107) instead of the
 //                       committed one the golden II traces were made with
 //   graph=y|n           replay the time step from a HIP graph (default y)
+//   cfl=<frac>          without dt: dt = frac x the energy-method CFL bound
+//                       h_min / (vp sqrt 3) of the inputs (eci/SConstruct:
+//                       107-114), reduced on the device (ecsg_cfl_dt)
 //   sbp=y               print the summation-by-parts residuals of grid 1 after
 //                       each stress update (TestSBP, test_sbp.cpp, as the
 //                       committed main.cpp calls it) and write them to
@@ -32,6 +35,7 @@
 //   dry_run=y           read and check every input, print the mapped source and
 //                       receiver positions, and stop before any device work
 // Errors are reported on stderr with exit status 1 where the reference asserts.
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -181,8 +185,11 @@ int run(int argc, char **argv)
     const int radius = (int)P.geti("radius", 0);
     if (radius <= 0) throw Fail("radius must be given and positive");
     if (radius > 6) throw Fail("radius " + std::to_string(radius) + " is not supported (FD coefficients exist for 1..6)");
-    const double dt = P.getd("dt", -1.0);
-    if (!(dt > 0)) throw Fail("dt must be given and positive");
+    double dt = P.getd("dt", -1.0);
+    // without dt: a fraction of the energy-method CFL bound of the inputs
+    // (eci/SConstruct:107-114), computed on the device once they are read
+    const double cfl = P.getd("cfl", -1.0);
+    if (!(dt > 0) && !(cfl > 0)) throw Fail("dt must be given and positive (or cfl=, a fraction of the CFL bound)");
     const int imode = (int)P.geti("imode", 1);
     const bool verb = P.getb("verb", false);
     const int top_bc = (int)P.geti("top_bc", 0);
@@ -199,11 +206,6 @@ int run(int argc, char **argv)
     const int movie_nt = (int)P.geti("movie_nt", -1);
     if (movie_nt <= 0) throw Fail("movie_nt must be given and positive");
     bool movie_flag[3] = {P.getb("movie_vx_flag", false), P.getb("movie_vy_flag", false), P.getb("movie_vz_flag", false)};
-
-    // iterations niter = 0 .. ceil(movie_dt*movie_nt/dt) (main.cpp:165)
-    const double last = std::ceil(movie_dt * movie_nt / dt);
-    if (last > 2e9) throw Fail("too many time steps");
-    const int nsteps = (int)last + 1;
 
     // ----- grids and parameters (init_del_par_field.cpp:21-72) -----
     std::vector<GridGeom> G(num_grids);
@@ -274,9 +276,7 @@ int run(int argc, char **argv)
             std::cout << "(grid#" << rec_loc[4 * r] << ", ix=" << rec_loc[4 * r + 1] << ",  iy=" << rec_loc[4 * r + 2]
                       << ", iz=" << rec_loc[4 * r + 3] << ")" << std::endl;
     }
-    std::cout << "== " << nsteps << " time steps of " << dt << " ms" << std::endl;
-    if (P.getb("dry_run", false)) return 0;  // inputs read and checked; no device work
-
+    const bool dry = P.getb("dry_run", false);
     ecsg_config cfg{};
     cfg.num_grids = num_grids;
     for (int i = 0; i < num_grids; i++)
@@ -284,6 +284,27 @@ int run(int argc, char **argv)
             cfg.n[i][d] = G[i].n[d];
             cfg.h[i][d] = G[i].h[d];
         }
+    if (!(dt > 0)) {
+        if (dry) throw Fail("dry_run needs dt (the CFL bound is computed on the device)");
+        std::vector<const double *> bp(num_grids);
+        std::vector<std::array<const double *, 9>> sp(num_grids);
+        for (int i = 0; i < num_grids; i++) {
+            bp[i] = gp[i].buoy.data();
+            for (int k = 0; k < 9; k++) sp[i][k] = gp[i].st[k].empty() ? nullptr : gp[i].st[k].data();
+        }
+        // std::array<T, 9> has the layout of T[9]
+        if (ecsg_cfl_dt(&cfg, bp.data(), reinterpret_cast<const double *const (*)[9]>(sp.data()), aniso ? 0 : 1, cfl,
+                        &dt) != ECSG_OK)
+            throw Fail("ecsg_cfl_dt failed");
+        std::cout << "== dt = " << cfl << " x the CFL bound = " << dt << " ms" << std::endl;
+    }
+    // iterations niter = 0 .. ceil(movie_dt*movie_nt/dt) (main.cpp:165)
+    const double last = std::ceil(movie_dt * movie_nt / dt);
+    if (last > 2e9) throw Fail("too many time steps");
+    const int nsteps = (int)last + 1;
+    std::cout << "== " << nsteps << " time steps of " << dt << " ms" << std::endl;
+    if (dry) return 0;  // inputs read and checked; no device work
+
     cfg.dt = dt;
     cfg.radius = radius;
     cfg.top_bc = top_bc;

@@ -36,7 +36,8 @@ enum {
     ECSG_EHIP = -2,     /* HIP runtime error */
     ECSG_ENOMEM = -3,   /* device allocation failed */
     ECSG_ENOCONV = -4,  /* Jacobi iteration cap reached */
-    ECSG_ESTATE = -5    /* call out of order (e.g. params not set) */
+    ECSG_ESTATE = -5,   /* call out of order (e.g. params not set) */
+    ECSG_ETIMEOUT = -6  /* a grid barrier of the cooperative Jacobi solve waited past its limit */
 };
 
 enum { ECSG_F32 = 0, ECSG_F64 = 1 };
@@ -102,6 +103,32 @@ const char *ecsg_version(void);
  * c44 = c55 = c66 = s11 - s13 (init_del_par_field.cpp:119).  Also builds the
  * time-invariant interface matrices of the interfaces touching g. */
 int ecsg_set_params(ecsg_ctx *ctx, int g, const double *buoy, const double *const stiff[9], int iso);
+
+/* Time step from the energy-method CFL bound the reference's flows use
+ * (tests/test1-cmp-eci-ii/eci/SConstruct:107-114, where main.cpp takes dt as
+ * given, main.cpp:51): dt = frac * min over grids g of
+ * min(h_g) / (vp_g sqrt 3), vp_g^2 = the largest max(c11, c22, c33) * b over
+ * the primary nodes of grid g, reduced on the device in FP64.  Inputs as for
+ * ecsg_set_params (primary nodes, per grid buoy[g] and stiff[g][0..8]; iso:
+ * only stiff[g][0] = s11 is read); cfg supplies num_grids, n and h (its dt is
+ * not read), so the call comes before ecsg_create.  Needs no handle. */
+int ecsg_cfl_dt(const ecsg_config *cfg, const double *const buoy[], const double *const stiff[][9], int iso,
+                double frac, double *dt_out);
+
+/* Grid g's parameters as main.cpp holds them after InitParField, i.e. what
+ * UpdateVel / UpdateStress / UpdateGhost* receive (sim_utils.hpp:128-143):
+ * buoy = buoys[g] (primary nodes) and stiff[m] = stiffs_ptr[g][m]->ptr1 for
+ * m = 0..8 (order c11 c22 c33 c23 c13 c12 c44 c55 c66), each Field at its
+ * staggered position -- c11..c12 at (1,0,0), c44 at (1,1,1), c55 at (0,0,1),
+ * c66 at (0,1,0) (init_del_par_field.cpp:82-88, 129-149) -- with halo 0, x
+ * fastest, and n or n+1 entries per dimension for a staggered or primary
+ * direction (field.cpp:85-98).  Values are taken as given (already averaged).
+ * Pointers may alias, as InitParField's do (stiff[0..2] one Field, stiff[3..5]
+ * another): then the energy uses the reference's isotropic compliance
+ * (init_del_par_field.cpp:259-278).  Gradients of such a grid
+ * (ecsg_get_gradient) are w.r.t. the staged Fields, in their layout; a caller
+ * whose pointers alias sums the gradients of the aliased entries. */
+int ecsg_set_params_staged(ecsg_ctx *ctx, int g, const double *buoy, const double *const stiff[9]);
 
 /* Source (InitSrc, init_src.cpp:7-45, resolved to grid indices by the caller).
  * type 0: vx += 1000*trace[n] in float (main.cpp:194-195);
@@ -219,7 +246,8 @@ int ecsg_checkpoint_interval(const ecsg_ctx *ctx);
 int ecsg_gradient(ecsg_ctx *ctx, int nsteps, const double *observed, double *misfit);
 
 /* dJ/d(input) of the last ecsg_gradient: sel = ECSG_BUOY for buoyancy or
- * ECSG_C11 + m for stiff[m]; out has (nx+1)(ny+1)(nz+1) values. */
+ * ECSG_C11 + m for stiff[m]; out has (nx+1)(ny+1)(nz+1) values, or for a
+ * grid set by ecsg_set_params_staged the length of staged Field m. */
 int ecsg_get_gradient(ecsg_ctx *ctx, int g, int sel, double *out);
 
 /* Test hooks: one transposed time step applied to the adjoint wavefields
@@ -244,9 +272,21 @@ int ecsg_adjoint_stats(ecsg_ctx *ctx, long long *sweeps);
  * ecsg_overlap_active returns the schedule the
  * next steps use: 0 = serial; 1 = overlap; 2 = overlap with the stress
  * interior of grid 0 also beside the sxz/syz ghost solves (the source and
- * every receiver at least two planes away from each interface). */
+ * every receiver at least two planes away from each interface).  The
+ * overlap is used only when every cooperative (grid-barrier) Jacobi launch of
+ * the step has at most one workgroup per compute unit; steps with larger
+ * launches are serial whatever `on` says. */
 int ecsg_set_overlap(ecsg_ctx *ctx, int on);
 int ecsg_overlap_active(const ecsg_ctx *ctx);
+
+/* Longest wait, in milliseconds, of one workgroup at one grid barrier of the
+ * cooperative Jacobi solves (default 10000; no counterpart in the reference,
+ * whose Jacobi2D is a host loop, update_ghost_sxz.cpp:314-350).  A wait past
+ * it ends the solve early and the next ecsg_sync (or gradient) returns
+ * ECSG_ETIMEOUT instead of the step never finishing; the wavefield is then
+ * invalid.  0 makes the first barrier of every solve time out (a test of
+ * the error path). */
+int ecsg_set_barrier_timeout(ecsg_ctx *ctx, int ms);
 
 /* ecsg_step_n's velocity and stress updates in one fused pass (default off;
  * used when the energy check is off): the pass reads the state before the
