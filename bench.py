@@ -57,20 +57,37 @@ def build_model(name: str, steps: int):
     raise SystemExit("unknown workload %s" % name)
 
 
-def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_steps: int = 50, solver: str = "jacobi"):
+def gpu_run(m, precision: str, warmup: int, preroll: int, steps: int, profile: bool, prof_steps: int = 50,
+            solver: str = "jacobi"):
+    """warmup + preroll untimed steps from rest (the pre-roll carries the
+    wavefield across the interfaces, so that the timed steps include the
+    interface solves' steady-state sweep counts), then `steps` timed steps."""
     import ecsg_amd
     t0 = time.time()
-    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=warmup + steps + prof_steps + 2,
-                              sxz_solver=solver)
+    sim = ecsg_amd.Simulation(m, precision=precision, check_energy=False,
+                              max_steps=warmup + preroll + steps + prof_steps + 2, sxz_solver=solver)
     log("[bench] %s state ready in %.1fs" % (precision, time.time() - t0))
     sim.step(warmup)
     sim.sync()
     log("[bench] %s warmup %d steps done" % (precision, warmup))
+    pre = None
+    if preroll > 0:
+        s0 = [sim.jacobi_stats(k)[0] for k in range(m.num_grids - 1)]
+        t = time.perf_counter()
+        sim.step(preroll)
+        sim.sync()
+        ep = time.perf_counter() - t
+        s1 = [sim.jacobi_stats(k)[0] for k in range(m.num_grids - 1)]
+        pre = {"steps": preroll, "ms_per_step": round(ep / preroll * 1e3, 4),
+               "jacobi_sweeps_per_step": [round((b - a) / preroll, 2) for a, b in zip(s0, s1)]}
+        log("[bench] %s pre-roll %d steps: %s" % (precision, preroll, pre))
     # the timed steps, with HIP events around the grid-0 stencil launches
+    s0 = [sim.jacobi_stats(k)[0] for k in range(m.num_grids - 1)]
     t = time.perf_counter()
     live = sim.step_timed(steps)
     el = time.perf_counter() - t
-    stats = [sim.jacobi_stats(k) for k in range(m.num_grids - 1)]
+    s1 = [sim.jacobi_stats(k)[0] for k in range(m.num_grids - 1)]
+    stats = [round((b - a) / steps, 2) for a, b in zip(s0, s1)]
     overlap = sim.overlap_mode
     if profile and prof_steps > 0:
         # every kernel kind: further steps with each launch bracketed by events
@@ -80,7 +97,7 @@ def gpu_run(m, precision: str, warmup: int, steps: int, profile: bool, prof_step
             % (precision, ms[0], ms[1], ms[4], cnt[4], ms[5]))
     tr = sim.traces()
     sim.close()
-    return el, live, stats, tr, overlap
+    return el, live, stats, tr, overlap, pre
 
 
 def adjoint_run(m, steps: int, every: int, fmt: str, precision: str = "f32"):
@@ -235,21 +252,47 @@ def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: 
     return out
 
 
-def cpu_baseline(m, steps: int):
-    """The oracle (CPU restatement, OpenMP) on the same model for `steps` steps."""
+def cpu_baseline(m, steps: int, steps_1t: int, c0: bool):
+    """The oracle (CPU restatement of the reference's loops, OpenMP) on the
+    host cores, threads bound close (OMP_PROC_BIND=close, OMP_PLACES=cores):
+    the same model for `steps` steps with every thread the process has
+    (OMP_NUM_THREADS), then `steps_1t` steps on one thread; with c0, the
+    81^3-node uniform configuration in full (600 steps, FP64, energy every
+    step) on every thread.  Wall-clock times (perf_counter)."""
+    os.environ["OMP_PROC_BIND"] = "close"      # read when the OpenMP runtime loads with the oracle
+    os.environ["OMP_PLACES"] = "cores"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_lib import Oracle, lib
     L = lib("f32")
     threads = int(L.orc_num_threads())
     t0 = time.time()
-    o = Oracle(m, prec="f32", check_energy=False, max_steps=steps + 1)
+    o = Oracle(m, prec="f32", check_energy=False, max_steps=steps + steps_1t + 2)
     log("[bench] cpu oracle ready in %.1fs (%d threads)" % (time.time() - t0, threads))
     o.run(1)  # first touch
     t = time.perf_counter()
     o.run(steps)
     el = time.perf_counter() - t
+    out = {"threads": threads, "steps": steps, "seconds": el}
+    if steps_1t > 0:
+        L.orc_set_num_threads(1)
+        t = time.perf_counter()
+        o.run(steps_1t)
+        out["one_thread"] = {"steps": steps_1t, "seconds": time.perf_counter() - t}
+        L.orc_set_num_threads(threads)
     o.close()
-    return el, threads
+    if c0:
+        from ecsg_amd import model as M
+        mc = M.uniform_model(nsteps=600)
+        L64 = lib("f64")
+        L64.orc_set_num_threads(threads)
+        oc = Oracle(mc, prec="f64", check_energy=True, max_steps=600)
+        t = time.perf_counter()
+        oc.run(600)
+        ec = time.perf_counter() - t
+        e = oc.energy()
+        oc.close()
+        out["c0"] = {"steps": 600, "nodes": mc.total_nodes, "seconds": ec, "energy_last": float(e[-1])}
+    return out
 
 
 def profiled_traffic(workload: str):
@@ -298,9 +341,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--preroll", type=int, default=1000,
+                    help="untimed steps after the warm-up, before the timed steps (the wavefield crosses the "
+                         "interfaces; default: half of the 2000-step run)")
     ap.add_argument("--workload", default="three_grid")
     ap.add_argument("--fp64-steps", type=int, default=500)
     ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--cpu-steps-1t", type=int, default=8, help="steps of the CPU baseline's one-thread leg")
+    ap.add_argument("--no-cpu-c0", action="store_true", help="skip the CPU baseline's full 81^3 uniform run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=0,
@@ -343,15 +391,17 @@ def main():
               "(see the sharding section of the design notes), no N>1 figure is produced", file=sys.stderr)
         sys.exit(2)
 
-    total_steps = args.warmup + max(args.steps, args.fp64_steps, args.adjoint_steps) + 60
+    affinity = len(os.sched_getaffinity(0))   # before the CPU baseline's OpenMP runtime binds this thread
+    total_steps = args.warmup + args.preroll + max(args.steps, args.fp64_steps, args.adjoint_steps) + 60
     t0 = time.time()
     m = build_model(args.workload, total_steps)
     nodes = m.total_nodes
     log("[bench] model %s: %d grids, %.1f M nodes, dt=%.4g, built in %.1fs"
         % (m.name, m.num_grids, nodes / 1e6, m.dt, time.time() - t0))
 
-    el, live, stats, _, overlap = gpu_run(m, "f32", args.warmup, args.steps, profile=args.prof_steps > 0,
-                                          prof_steps=args.prof_steps, solver=args.solver)
+    el, live, stats, tr32, overlap, pre32 = gpu_run(m, "f32", args.warmup, args.preroll, args.steps,
+                                                    profile=args.prof_steps > 0, prof_steps=args.prof_steps,
+                                                    solver=args.solver)
     gpts = nodes * args.steps / el / 1e9
     ms_step = el / args.steps * 1e3
     log("[bench] f32: %.3f s for %d steps -> %.2f Gpt/s, %.3f ms/step" % (el, args.steps, gpts, ms_step))
@@ -384,14 +434,22 @@ def main():
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
-        el64, live64, _, _, _ = gpu_run(m, "f64", args.warmup, args.fp64_steps, profile=args.prof_steps > 0,
-                                        prof_steps=args.prof_steps, solver=args.solver)
+        el64, live64, st64, tr64, _, pre64 = gpu_run(m, "f64", args.warmup, args.preroll, args.fp64_steps,
+                                                     profile=args.prof_steps > 0, prof_steps=args.prof_steps,
+                                                     solver=args.solver)
         g64 = nodes * args.fp64_steps / el64 / 1e9
+        # the FP32 run's receiver traces against the FP64 run's, over the steps both made
+        nc = min(tr32.shape[1], tr64.shape[1])
+        d = float(np.linalg.norm(tr32[:, :nc] - tr64[:, :nc]))
+        ref = float(np.linalg.norm(tr64[:, :nc]))
+        fp32_err = d / ref if ref > 0 else d
         fp64 = {"value": round(g64, 3), "unit": "Gpt/s", "steps": args.fp64_steps,
                 "ms_per_step": round(el64 / args.fp64_steps * 1e3, 4),
                 "hbm_frac_step": round(2 * BYTES_STEP_F32 * g64 * 1e9 / HBM_PEAK, 4),
-                "stress0_ms": round(live64[1], 5), "vel0_ms": round(live64[0], 5)}
-        log("[bench] f64: %.2f Gpt/s" % g64)
+                "stress0_ms": round(live64[1], 5), "vel0_ms": round(live64[0], 5),
+                "jacobi_sweeps_per_timed_step": st64,
+                "fp32_trace_rel_l2": float("%.3e" % fp32_err), "traces_compared_steps": nc}
+        log("[bench] f64: %.2f Gpt/s; FP32 traces vs FP64 over %d steps: %.3e relative L2" % (g64, nc, fp32_err))
 
     adjoint = None
     if args.adjoint_steps > 0:
@@ -429,11 +487,28 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        ce, threads = cpu_baseline(m, args.cpu_steps)
+        cb = cpu_baseline(m, args.cpu_steps, args.cpu_steps_1t, not args.no_cpu_c0)
+        ce, threads = cb["seconds"], cb["threads"]
+        try:
+            import psutil
+            phys = psutil.cpu_count(logical=False)
+        except Exception:
+            phys = None
         cpu = {"value": round(nodes * args.cpu_steps / ce / 1e9, 5), "unit": "Gpt/s", "cores": threads,
                "kind": "port",
-               "sample": "the same %s model, FP32 restatement (oracle/ecsg_oracle.c, OpenMP), %d steps after one "
-                         "warm-up step, %.1f s wall" % (m.name, args.cpu_steps, ce)}
+               "sample": "the same %s model, FP32 restatement of the reference's loops (oracle/ecsg_oracle.c, "
+                         "OpenMP, OMP_PROC_BIND=close), %d steps after one warm-up step, %.1f s wall on %d threads"
+                         % (m.name, args.cpu_steps, ce, threads),
+               "host_physical_cores": phys, "affinity_cpus": affinity}
+        if "one_thread" in cb:
+            o1 = cb["one_thread"]
+            cpu["one_thread"] = {"value": round(nodes * o1["steps"] / o1["seconds"] / 1e9, 5), "unit": "Gpt/s",
+                                 "steps": o1["steps"], "seconds": round(o1["seconds"], 2)}
+        if "c0" in cb:
+            c = cb["c0"]
+            cpu["c0_full"] = {"value": round(c["nodes"] * c["steps"] / c["seconds"] / 1e9, 5), "unit": "Gpt/s",
+                              "steps": c["steps"], "seconds": round(c["seconds"], 2), "threads": threads,
+                              "what": "81^3-node uniform configuration in full, FP64, energy every step"}
         log("[bench] cpu: %s" % cpu)
 
     out = {
@@ -463,7 +538,9 @@ def main():
         "fp64": fp64,
         "adjoint": adjoint,
         "batch": batch,
-        "jacobi_iters_per_step": [round(s[0] / (args.warmup + args.steps), 2) for s in stats],
+        "preroll": pre32,
+        "jacobi_sweeps_per_timed_step": stats,
+        "fp32_trace_rel_l2_vs_fp64": fp64["fp32_trace_rel_l2"] if fp64 else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

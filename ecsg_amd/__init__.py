@@ -9,8 +9,10 @@ is missing, importing works but creating a Simulation raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -65,6 +67,18 @@ class EcsgError(RuntimeError):
 
 
 _LIB = None
+# open handles, closed at interpreter exit before the HIP runtime is torn down
+_OPEN = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_handles():
+    for kind in (Batch, Simulation):
+        for obj in [o for o in list(_OPEN) if isinstance(o, kind)]:
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 
 def load_library(path: Optional[str] = None):
@@ -206,6 +220,7 @@ class Simulation:
             self.L.ecsg_destroy(self.h)
             self.h = None
             raise EcsgError("ecsg_create failed (%d): %s" % (rc, msg))
+        _OPEN.add(self)
         for g in range(m.num_grids):
             if staged:
                 self.set_params_staged(g)
@@ -447,6 +462,7 @@ class Batch:
             self.L.ecsg_batch_destroy(self.h)
             self.h = None
             raise EcsgError("ecsg_batch_create failed (%d): %s" % (rc, msg))
+        _OPEN.add(self)
 
     def _chk(self, rc):
         if rc != 0:

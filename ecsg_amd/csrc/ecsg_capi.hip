@@ -34,9 +34,27 @@ using namespace ecsg;
 // one: two such kernels from different streams could otherwise each hold
 // part of the GPU while waiting at a barrier for workgroups that the other
 // keeps from being placed.  The per-device events live until the process
-// ends.
+// ends or the last handle is destroyed (coop_release), so that no HIP
+// object of the library outlives the handles into the runtime's teardown.
 static std::mutex coop_mu;
 static hipEvent_t coop_last[64];
+static int live_handles = 0;
+
+static void coop_acquire()
+{
+    std::lock_guard<std::mutex> lock(coop_mu);
+    live_handles++;
+}
+static void coop_release()
+{
+    std::lock_guard<std::mutex> lock(coop_mu);
+    if (--live_handles > 0) return;
+    for (auto &ev : coop_last)
+        if (ev) {
+            hipEventDestroy(ev);
+            ev = nullptr;
+        }
+}
 
 static hipError_t launch_cooperative(const void *fn, dim3 grid, dim3 block, void **args, hipStream_t s)
 {
@@ -95,7 +113,8 @@ struct Status {
 
 class EngineBase {
 public:
-    virtual ~EngineBase() {}
+    EngineBase() { coop_acquire(); }
+    virtual ~EngineBase() { coop_release(); }
     Status st;
     int fail(int code, const std::string &m)
     {

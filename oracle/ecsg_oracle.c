@@ -1548,4 +1548,14 @@ int orc_num_threads(void)
 #endif
 }
 
+/* threads of the following parallel regions (the CPU baseline's one-thread leg) */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int orc_precision_bytes(void) { return (int)sizeof(real); }

@@ -55,6 +55,7 @@ def _load(prec: str):
     lib.orc_fd_coeff.argtypes = [i, P(C.c_double)]
     lib.orc_set_vy_mirror_all.argtypes = [vp, i]
     lib.orc_set_ii_xend.argtypes = [vp, i]
+    lib.orc_set_num_threads.argtypes = [i]
     return lib
 
 
