@@ -38,7 +38,7 @@ ABI_SYMBOLS = (
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
     "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_stream_probe_floats", "ecsg_stream_probe",
     "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed", "ecsg_set_barrier_timeout",
-    "ecsg_set_params_staged", "ecsg_cfl_dt",
+    "ecsg_set_params_staged", "ecsg_cfl_dt", "ecsg_cfl_dt_noalloc",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -120,6 +120,7 @@ def load_library(path: Optional[str] = None):
     L.ecsg_set_barrier_timeout.argtypes = [vp, i]
     L.ecsg_set_params_staged.argtypes = [vp, i, P(d), P(P(d))]
     L.ecsg_cfl_dt.argtypes = [P(EcsgConfig), P(P(d)), P((P(d) * 9)), i, d, P(d)]
+    L.ecsg_cfl_dt_noalloc.argtypes = [P(EcsgConfig), P(vp), P(vp * 9), i, d, vp, vp]
     L.ecsg_field_dims.argtypes = [vp, i, i, P(i)]
     L.ecsg_get_field.argtypes = [vp, i, i, P(d)]
     L.ecsg_set_field.argtypes = [vp, i, i, P(d)]

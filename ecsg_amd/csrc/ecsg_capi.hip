@@ -3432,6 +3432,56 @@ int ecsg_cfl_dt(const ecsg_config *cfg, const double *const buoy[], const double
     *dt_out = frac * best;
     return ECSG_OK;
 }
+// Scratch of ecsg_cfl_dt_noalloc: per grid, CFL_NB block maxima and the
+// grid's maximum (a stated, fixed size: 8 grids x 1025 doubles = 65.6 KB,
+// held by every GPU the library runs on until the process ends).  Every
+// block writes its entry before the final pass reads it, so nothing needs
+// clearing between calls; calls that overlap on one GPU race.  In host code
+// the name is not a device address: kernels get cfl_scratch_ptr[device].
+constexpr int CFL_NB = 1024;
+static_assert(ECSG_MAX_GRIDS <= 8, "CflDtArgs holds 8 grid steps");
+static __device__ __align__(256) double scratch[ECSG_MAX_GRIDS * (CFL_NB + 1)];
+static std::mutex scratch_mu;
+static double *cfl_scratch_ptr[64];
+
+int ecsg_cfl_dt_noalloc(const ecsg_config *cfg, const double *const buoy[], const double *const stiff[][9], int iso,
+                        double frac, double *dt_dev, void *stream)
+{
+    if (!cfg || !buoy || !stiff || !dt_dev || !(frac > 0)) return ECSG_EINVAL;
+    if (cfg->num_grids < 1 || cfg->num_grids > ECSG_MAX_GRIDS) return ECSG_EINVAL;  // beyond the scratch
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return ECSG_EHIP;
+    double *sp = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        if (!cfl_scratch_ptr[dev]) {
+            void *p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return ECSG_EHIP;
+            cfl_scratch_ptr[dev] = (double *)p;
+        }
+        sp = cfl_scratch_ptr[dev];
+    }
+    hipStream_t st = (hipStream_t)stream;
+    CflDtArgs a{};
+    a.ng = cfg->num_grids;
+    a.stride = CFL_NB + 1;
+    a.frac = frac;
+    for (int g = 0; g < cfg->num_grids; g++) {
+        const int *n = cfg->n[g];
+        const double *h = cfg->h[g];
+        if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(h[0] > 0) || !(h[1] > 0) || !(h[2] > 0)) return ECSG_EINVAL;
+        if (!buoy[g] || !stiff[g][0] || (!iso && (!stiff[g][1] || !stiff[g][2]))) return ECSG_EINVAL;
+        a.hmin[g] = std::min(std::min(h[0], h[1]), h[2]);
+        const long np = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+        const int nblk = (int)std::min<long>((np + 255) / 256, CFL_NB);
+        double *part = sp + (long)g * (CFL_NB + 1);
+        hipLaunchKernelGGL(k_cfl_part, dim3(nblk), dim3(256), 0, st, buoy[g], stiff[g][0], iso ? nullptr : stiff[g][1],
+                           iso ? nullptr : stiff[g][2], np, part);
+        hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(64), 0, st, part, nblk, part + CFL_NB);
+    }
+    hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(64), 0, st, sp + CFL_NB, a, dt_dev);
+    return hipGetLastError() == hipSuccess ? ECSG_OK : ECSG_EHIP;
+}
 uint64_t ecsg_stream_probe_floats(int nx, int ny, int nz)
 {
     if (nx < 1 || ny < 1 || nz < 1) return 0;

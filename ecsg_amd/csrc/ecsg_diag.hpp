@@ -137,4 +137,23 @@ __global__ __launch_bounds__(64) void k_cfl_final(const double *__restrict__ par
     if (threadIdx.x == 0) out[0] = m;
 }
 
+// dt = frac * min over grids of h_min / (sqrt(vp2_g) sqrt 3), from the
+// per-grid maxima vp2[g * stride], in the host formula's operation order.
+struct CflDtArgs {
+    double hmin[8];
+    int ng;
+    long stride;
+    double frac;
+};
+__global__ void k_cfl_dt(const double *__restrict__ vp2, CflDtArgs a, double *__restrict__ dt)
+{
+    if (threadIdx.x != 0) return;
+    double best = INFINITY;
+    for (int g = 0; g < a.ng; g++) {
+        const double vp = sqrt(vp2[(long)g * a.stride]);
+        best = fmin(best, a.hmin[g] / (vp * sqrt(3.0)));
+    }
+    dt[0] = a.frac * best;
+}
+
 }  // namespace ecsg

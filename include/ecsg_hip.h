@@ -115,6 +115,19 @@ int ecsg_set_params(ecsg_ctx *ctx, int g, const double *buoy, const double *cons
 int ecsg_cfl_dt(const ecsg_config *cfg, const double *const buoy[], const double *const stiff[][9], int iso,
                 double frac, double *dt_out);
 
+/* ecsg_cfl_dt for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc); no counterpart in
+ * the reference.  Inputs as for ecsg_cfl_dt but resident in device memory
+ * (buoy[g], stiff[g][0..8]: device pointers to primary-node arrays); dt is
+ * written to the device double dt_dev by work enqueued on `stream` (a
+ * hipStream_t; null = the null stream).  No device or host allocation: the
+ * block maxima live in a fixed scratch array of the library, 65.6 KB per GPU
+ * (at most ECSG_MAX_GRIDS grids, else ECSG_EINVAL), shared by all calls on
+ * that GPU -- calls that overlap on one GPU race.  The result equals
+ * ecsg_cfl_dt's to within one ulp (its square roots are the device's). */
+int ecsg_cfl_dt_noalloc(const ecsg_config *cfg, const double *const buoy[], const double *const stiff[][9], int iso,
+                        double frac, double *dt_dev, void *stream);
+
 /* Grid g's parameters as main.cpp holds them after InitParField, i.e. what
  * UpdateVel / UpdateStress / UpdateGhost* receive (sim_utils.hpp:128-143):
  * buoy = buoys[g] (primary nodes) and stiff[m] = stiffs_ptr[g][m]->ptr1 for
