@@ -165,41 +165,45 @@ def batch_run(steps: int, solo_steps: int, warmup: int):
 
 
 def batch_gradient_run(steps: int, every: int):
-    """The batched-shots workload's gradients: each of the 64 members (16
-    models x 4 shots) computes forward + adjoint over `steps` steps on its own
-    handle (checkpoints every `every` steps, FP32, native checkpoints; zero
-    observed data, misfit 1/2 sum traces^2), one member at a time, each closed
-    after its gradient so that one adjoint state is allocated at once.  A
-    first call of `every` steps per member does its set-up (allocation of the
-    checkpoint ring, interface term lists), as the first gradient of an
-    iterative inversion would; the second call, over `steps` steps, is timed.
-    The misfits are summed per model (the shots of a model share its media, so
-    its gradient is the sum of its shots' gradients)."""
+    """The batched-shots workload's gradients with one launch per phase
+    (ecsg_batch_gradient): 64 members (16 models x 4 shots), forward with
+    checkpoints + recomputation + transposed steps + 9 Cij and buoyancy
+    gradients, FP32, zero observed data (misfit 1/2 sum traces^2), each
+    model's gradient summed over its shots on the device.  Checkpoint
+    interval `every` (0: the largest that fits every member's ring in device
+    memory).  A first call of the same length does the set-up (rings,
+    checkpoints, interface term lists), as the first gradient of an iterative
+    inversion would; the second is timed."""
     import ecsg_amd
     from ecsg_amd import model as M
     ms = M.batch_workload(nsteps=steps + 2)
     nodes = sum(m.total_nodes for m in ms)
-    el = setup = 0.0
-    Js = []
-    for m in ms:
-        t0 = time.perf_counter()
-        s = ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=steps)
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=steps, sxz_solver="jacobi_lds")
+            for m in ms]
+    for s in sims:
         s.set_checkpointing(every, "native")
-        s.gradient(min(every, steps), np.zeros((len(m.rec_loc), min(every, steps))))
-        s.sync()
-        t1 = time.perf_counter()
-        Js.append(s.gradient(steps, np.zeros((len(m.rec_loc), steps))))
-        s.sync()
-        t2 = time.perf_counter()
+    b = ecsg_amd.Batch(sims)
+    b.autotune(reps=2)
+    obs = [np.zeros((len(m.rec_loc), steps)) for m in ms]
+    groups = [k // 4 for k in range(len(ms))]
+    t0 = time.perf_counter()
+    b.gradient(steps, obs, group=groups)
+    setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    Js = b.gradient(steps, obs, group=groups)
+    el = time.perf_counter() - t1
+    K = b.checkpoint_interval
+    b.close()
+    for s in sims:
         s.close()
-        setup += t1 - t0
-        el += t2 - t1
-    per_model = [sum(Js[k:k + 4]) for k in range(0, len(Js), 4)]
+    per_model = [float(sum(Js[k:k + 4])) for k in range(0, len(Js), 4)]
     return {"value": round(nodes * steps / el / 1e9, 3), "unit": "Gpt/s", "members": len(ms), "steps": steps,
-            "checkpoint_every": every, "recomputed_steps": max(0, steps - every), "seconds": round(el, 3),
-            "setup_seconds": round(setup, 3), "misfit_per_model": [float("%.6e" % j) for j in per_model],
-            "what": "forward with checkpoints + recomputation + adjoint + 9 Cij and buoyancy gradients per member, "
-                    "member by member; gradients of a model = sum over its 4 shots"}
+            "checkpoint_every": K, "recomputed_steps": max(0, steps - K), "seconds": round(el, 3),
+            "first_call_seconds": round(setup, 3), "ms_per_step": round(el / steps * 1e3, 3),
+            "misfit_per_model": [float("%.6e" % j) for j in per_model],
+            "what": "ecsg_batch_gradient: forward with checkpoints + recomputation + transposed step + 9 Cij and "
+                    "buoyancy gradients of all 64 members, one launch per phase; gradients summed per model on the "
+                    "device"}
 
 
 def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: int = 10, tilings=(),
@@ -366,7 +370,8 @@ def main():
     ap.add_argument("--batch-solo-steps", type=int, default=50)
     ap.add_argument("--batch-grad-steps", type=int, default=200,
                     help="steps of each batch member's gradient (0: skip)")
-    ap.add_argument("--batch-grad-every", type=int, default=100)
+    ap.add_argument("--batch-grad-every", type=int, default=0,
+                    help="checkpoint interval of the batched gradient (0: the largest that fits in device memory)")
     ap.add_argument("--no-stream-bound", action="store_true",
                     help="skip the streaming probe of the stress sweep's traversal")
     ap.add_argument("--stream-bound-grid", type=int, nargs=3, metavar=("NX", "NY", "PLANES"),
@@ -483,6 +488,7 @@ def main():
         log("[bench] batch: %s" % batch)
         if args.batch_grad_steps > 0:
             batch["gradient"] = batch_gradient_run(args.batch_grad_steps, args.batch_grad_every)
+            batch["gradient"]["cost_vs_batched_forward"] = round(batch["value"] / batch["gradient"]["value"], 3)
             log("[bench] batch gradients: %s" % batch["gradient"])
 
     cpu = None

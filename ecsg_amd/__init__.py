@@ -38,7 +38,8 @@ ABI_SYMBOLS = (
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
     "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_stream_probe_floats", "ecsg_stream_probe",
     "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed", "ecsg_set_barrier_timeout",
-    "ecsg_set_params_staged", "ecsg_cfl_dt", "ecsg_cfl_dt_noalloc",
+    "ecsg_set_params_staged", "ecsg_cfl_dt", "ecsg_cfl_dt_noalloc", "ecsg_batch_gradient",
+    "ecsg_batch_checkpoint_interval",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -147,6 +148,8 @@ def load_library(path: Optional[str] = None):
     L.ecsg_batch_synchronize.argtypes = [vp]
     L.ecsg_batch_set_tiling.argtypes = [vp, i, i]
     L.ecsg_batch_autotune.argtypes = [vp, i, P(i), P(i)]
+    L.ecsg_batch_gradient.argtypes = [vp, i, P(P(d)), P(d), P(i)]
+    L.ecsg_batch_checkpoint_interval.argtypes = [vp]
     if path is None:
         _LIB = L
     return L
@@ -482,6 +485,28 @@ class Batch:
         by, zc = C.c_int(), C.c_int()
         self._chk(self.L.ecsg_batch_autotune(self.h, int(reps), C.byref(by), C.byref(zc)))
         return by.value, zc.value
+
+    def gradient(self, nsteps: int, observed, group=None) -> np.ndarray:
+        """ecsg_batch_gradient: every member's misfit and gradient (read with
+        each member's grad()) with one launch per phase; observed[m] is member
+        m's nrec x nsteps traces.  group[m]: member m's model; the first member
+        of each group then holds the model's summed gradient.  Returns the
+        misfits."""
+        obs = []
+        for sm, o in zip(self.sims, observed):
+            obs.append(np.ascontiguousarray(o, np.float64).reshape(sm.nrec, nsteps) if sm.nrec else np.zeros(1))
+        ptrs = (C.POINTER(C.c_double) * len(obs))(*[_dp(o) for o in obs])
+        mis = np.zeros(len(self.sims))
+        grp = None
+        if group is not None:
+            grp = np.ascontiguousarray(group, np.int32)
+        self._chk(self.L.ecsg_batch_gradient(self.h, int(nsteps), ptrs, _dp(mis),
+                                             grp.ctypes.data_as(C.POINTER(C.c_int)) if grp is not None else None))
+        return mis
+
+    @property
+    def checkpoint_interval(self) -> int:
+        return int(self.L.ecsg_batch_checkpoint_interval(self.h))
 
     def close(self):
         if getattr(self, "h", None):

@@ -118,9 +118,8 @@ template <typename T>
 __device__ __forceinline__ T shfl_upw(T v) { return __shfl_up(v, 1, 64); }
 
 template <typename T>
-__global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
+__device__ __forceinline__ void adj_stress_body(const AdjGridDev<T> &a, const int tile)
 {
-    const int tile = xcd_tile(blockIdx.x, a.gx * a.gy * a.gz);
     const int bx = tile % a.gx;
     const int rest = tile / a.gx;
     const int by = rest % a.gy;
@@ -182,6 +181,12 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
         eyp = eyp_n; hxym = hxym_n; hyzm = hyzm_n;
     }
 }
+template <typename T>
+__global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
+{
+    adj_stress_body(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz));
+}
+
 
 // Gradient w.r.t. the nine staged stiffnesses at the stress points:
 // lam_s (after the transposed stress mirrors) times the derivative of the
@@ -192,12 +197,18 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
 constexpr int GBY = 4;
 constexpr int GZC = 8;
 
+// workgroups of a gradient sweep over one grid
 template <typename T>
-__global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
+__host__ __device__ __forceinline__ int adj_grad_tiles(const AdjGridDev<T> &a)
+{
+    return ((a.nx + 64) / 64) * ((a.ny + GBY) / GBY) * ((a.nz + GZC) / GZC);
+}
+
+template <typename T>
+__device__ __forceinline__ void adj_grad_body(const AdjGridDev<T> &a, const int tile)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int gx = (nx + 64) / 64, gy = (ny + GBY) / GBY;
-    const int tile = xcd_tile(blockIdx.x, gx * gy * ((nz + GZC) / GZC));
     const int bx = tile % gx, by = (tile / gx) % gy, bz = tile / (gx * gy);
     const int ix = bx * 64 + threadIdx.x, iy = by * GBY + threadIdx.y;
     if (ix > nx || iy > ny) return;
@@ -245,6 +256,12 @@ __global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
         fvy_c = fvy_n;
     }
 }
+template <typename T>
+__global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
+{
+    adj_grad_body(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
+}
+
 
 // ---------------------------------------------------------------------------
 // Transpose of UpdateVel (velocity stencil) on the stresses (the gradient
@@ -254,9 +271,8 @@ __global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
 //   vz += mean_xz(b) (cx dx sxz + cz dz szz + cy dy syz)       on the vz update set
 // with the adjoint weights w = (masked buoyancy) * lam_v.
 template <typename T>
-__global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
+__device__ __forceinline__ void adj_vel_body(const AdjGridDev<T> &a, const int tile)
 {
-    const int tile = xcd_tile(blockIdx.x, a.gx * a.gy * a.gz);
     const int bx = tile % a.gx;
     const int rest = tile / a.gx;
     const int by = rest % a.gy;
@@ -312,17 +328,22 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
         wx = wx_n; wy = wy_n; wz = wz_n; wxp = wxp_n; wzp = wzp_n; wym = wym_n;
     }
 }
+template <typename T>
+__global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
+{
+    adj_vel_body(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz));
+}
+
 
 // Gradient w.r.t. buoyancy at the velocity points (vx node, vy xy mean, vz
 // xz mean): lam_v (after the transposed velocity mirrors) times the stress
 // differences of this step's forward stresses s^n.  Own sweep, like
 // k_adj_grad; lam_v is not written by k_adj_vel.
 template <typename T>
-__global__ __launch_bounds__(64 * GBY) void k_adj_gradb(AdjGridDev<T> a)
+__device__ __forceinline__ void adj_gradb_body(const AdjGridDev<T> &a, const int tile)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int gx = (nx + 64) / 64, gy = (ny + GBY) / GBY;
-    const int tile = xcd_tile(blockIdx.x, gx * gy * ((nz + GZC) / GZC));
     const int bx = tile % gx, by = (tile / gx) % gy, bz = tile / (gx * gy);
     const int ix = bx * 64 + threadIdx.x, iy = by * GBY + threadIdx.y;
     if (ix > nx || iy > ny) return;
@@ -347,34 +368,50 @@ __global__ __launch_bounds__(64 * GBY) void k_adj_gradb(AdjGridDev<T> a)
         szz_c = szz_n;
     }
 }
+template <typename T>
+__global__ __launch_bounds__(64 * GBY) void k_adj_gradb(AdjGridDev<T> a)
+{
+    adj_gradb_body(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
+}
+
 
 // ---------------------------------------------------------------------------
 // Transposed ghost mirrors h = -i: lam[i] -= sum over the mirrors of i of
 // lam[h] (gather per source entry), then lam[h] = 0 (k_zero_list).
 template <typename T>
-__global__ __launch_bounds__(256) void k_mirror_T(T *lam, const long *__restrict__ src, const int *__restrict__ start,
-                                                  const long *__restrict__ halo, int n)
+__device__ __forceinline__ void mirror_T_body(T *lam, const long *__restrict__ src, const int *__restrict__ start,
+                                                  const long *__restrict__ halo, int n, const int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     T acc = lam[src[t]];
     for (int q = start[t]; q < start[t + 1]; q++) acc -= lam[halo[q]];
     lam[src[t]] = acc;
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_mirror_T(T *lam, const long *__restrict__ src, const int *__restrict__ start,
+                                                  const long *__restrict__ halo, int n)
+{
+    mirror_T_body(lam, src, start, halo, n, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
 
+
+template <typename T>
+__device__ __forceinline__ void zero_list_body(T *lam, const long *__restrict__ idx, int n, const int t)
+{
+    if (t < n) lam[idx[t]] = T(0);
+}
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero_list(T *lam, const long *__restrict__ idx, int n)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) lam[idx[t]] = T(0);
+    zero_list_body(lam, idx, n, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
+
 
 // Tridiagonal solve with the prefactored elimination of the vz-szz system
 // (symmetric, so it also solves the transposed system); solution in B.
-__global__ __launch_bounds__(64) void k_tri_solve(double *__restrict__ B, const double *__restrict__ M,
-                                                  const double *__restrict__ Dp, const double *__restrict__ U, int nx2, int n)
+__device__ __forceinline__ void tri_solve_body(double *__restrict__ B, const double *__restrict__ M,
+                                                  const double *__restrict__ Dp, const double *__restrict__ U, int nx2, int n, const int I)
 {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nx2) return;
     constexpr int CH = 16;
     double prev = 0;
@@ -413,6 +450,12 @@ __global__ __launch_bounds__(64) void k_tri_solve(double *__restrict__ B, const 
             }
     }
 }
+__global__ __launch_bounds__(64) void k_tri_solve(double *__restrict__ B, const double *__restrict__ M,
+                                                  const double *__restrict__ Dp, const double *__restrict__ U, int nx2, int n)
+{
+    tri_solve_body(B, M, Dp, U, nx2, n, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 // Denominator of the pointwise vy-syz ghost of each coarse (I, J), as the
 // forward kernel k_gsyz forms it.
@@ -450,9 +493,8 @@ struct RecAdj {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_adj_rec(RecAdj<T> a, int step)
+__device__ __forceinline__ void adj_rec_body(const RecAdj<T> &a, int step, const int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nloc) return;
     T acc = T(0);
     for (int q = a.start[t]; q < a.start[t + 1]; q++) acc += a.resid[(long)a.rec[q] * a.nsteps + step];
@@ -460,6 +502,12 @@ __global__ __launch_bounds__(256) void k_adj_rec(RecAdj<T> a, int step)
     T *v = a.lamvx[k >> 40];
     v[k & ((1L << 40) - 1)] += acc;
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_rec(RecAdj<T> a, int step)
+{
+    adj_rec_body(a, step, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 __global__ void k_set_int(int *p, int v)
 {
@@ -596,10 +644,9 @@ __device__ __forceinline__ double sxz_nu(const double *X, int i, int j, int r, i
 
 // fine-ghost transpose gathered on interior coarse nodes; rhs of the solve
 template <typename T>
-__global__ __launch_bounds__(256) void k_asxz_ghost(IfAdj<T> a, double *__restrict__ rhs)
+__device__ __forceinline__ void asxz_ghost_body(const IfAdj<T> &a, double *__restrict__ rhs, const int t)
 {
     const int n0 = a.nx2 - 1, n1 = a.ny2 - 1, r = a.r;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n0 * n1) return;
     const int I = t % n0 + 1, J = t / n0 + 1;
     double R = 0;
@@ -612,12 +659,17 @@ __global__ __launch_bounds__(256) void k_asxz_ghost(IfAdj<T> a, double *__restri
     rhs[t] = (double)s2[qc - a.SC] + R;
     s2[qc - a.SC] = T(0);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_ghost(IfAdj<T> a, double *__restrict__ rhs)
+{
+    asxz_ghost_body(a, rhs, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_asxz_fine0(IfAdj<T> a)
+__device__ __forceinline__ void asxz_fine0_body(const IfAdj<T> &a, const int t)
 {
     const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nx1 - 1) * (ny1 - 1)) return;
     const int i = t % (nx1 - 1) + 1, j = t / (nx1 - 1) + 1;
     const long q = AF_(i, j, a.nz1);
@@ -625,12 +677,17 @@ __global__ __launch_bounds__(256) void k_asxz_fine0(IfAdj<T> a)
     s1[q - a.SF] -= s1[q];
     s1[q] = T(0);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_fine0(IfAdj<T> a)
+{
+    asxz_fine0_body(a, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_asxz_fine(IfAdj<T> a, const double *__restrict__ X)
+__device__ __forceinline__ void asxz_fine_body(const IfAdj<T> &a, const double *__restrict__ X, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, nx1 = r * nx2, ny1 = r * ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nx1 + 1) * (ny1 + 1)) return;
     const int i = t % (nx1 + 1), j = t / (nx1 + 1);
     const double nu0 = sxz_nu(X, i, j, r, nx2, ny2);
@@ -661,12 +718,17 @@ __global__ __launch_bounds__(256) void k_asxz_fine(IfAdj<T> a, const double *__r
     if (i <= nx1 - 1 && (nu0 != 0.0 || nux != 0.0)) acc_to(&a.lf[3][q], -a.k1x * (b * nu0 - bx * nux));
     if (j <= ny1 - 1 && (nu0 != 0.0 || nuy != 0.0)) acc_to(&a.lf[8][q], -a.k1y * (b * nu0 - by * nuy));
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_fine(IfAdj<T> a, const double *__restrict__ X)
+{
+    asxz_fine_body(a, X, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_asxz_coarse(IfAdj<T> a, const double *__restrict__ X)
+__device__ __forceinline__ void asxz_coarse_body(const IfAdj<T> &a, const double *__restrict__ X, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nx2 + 1) * (ny2 + 1)) return;
     const int I = t % (nx2 + 1), J = t / (nx2 + 1);
     const double r2 = (double)(r * r), r4 = r2 * r2;
@@ -694,6 +756,12 @@ __global__ __launch_bounds__(256) void k_asxz_coarse(IfAdj<T> a, const double *_
         }
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asxz_coarse(IfAdj<T> a, const double *__restrict__ X)
+{
+    asxz_coarse_body(a, X, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 // ----- vz-szz --------------------------------------------------------------
 __device__ __forceinline__ double vz_mu(const double *M, int I, int J, int nx2, int ny2)
@@ -715,10 +783,9 @@ __device__ __forceinline__ double vz_nu(const double *M, int i, int j, int r, in
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_avz_ghost(IfAdj<T> a, double *__restrict__ rhs)
+__device__ __forceinline__ void avz_ghost_body(const IfAdj<T> &a, double *__restrict__ rhs, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, ny1 = r * ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx2 * (ny2 + 1)) return;
     const int I = t % nx2, J = t / nx2;
     double R = 0;
@@ -738,12 +805,17 @@ __global__ __launch_bounds__(256) void k_avz_ghost(IfAdj<T> a, double *__restric
         acc_to(&v2[qc - a.SC], R);
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_ghost(IfAdj<T> a, double *__restrict__ rhs)
+{
+    avz_ghost_body(a, rhs, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_avz_fine0(IfAdj<T> a)
+__device__ __forceinline__ void avz_fine0_body(const IfAdj<T> &a, const int t)
 {
     const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx1 * (ny1 - 1)) return;
     const int i = t % nx1, j = t / nx1 + 1;
     const long q = AF_(i, j, a.nz1);
@@ -751,12 +823,17 @@ __global__ __launch_bounds__(256) void k_avz_fine0(IfAdj<T> a)
     v1[q - a.SF] -= v1[q];
     v1[q] = T(0);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_fine0(IfAdj<T> a)
+{
+    avz_fine0_body(a, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_avz_fine(IfAdj<T> a, const double *__restrict__ M)
+__device__ __forceinline__ void avz_fine_body(const IfAdj<T> &a, const double *__restrict__ M, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, nx1 = r * nx2, ny1 = r * ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nx1 + 1) * (ny1 + 1)) return;
     const int i = t % (nx1 + 1), j = t / (nx1 + 1);
     const double nu0 = vz_nu(M, i, j, r, nx2, ny2);
@@ -793,12 +870,17 @@ __global__ __launch_bounds__(256) void k_avz_fine(IfAdj<T> a, const double *__re
         acc_to(&a.gf[2][q], -nu0 * a.k1z * (pv + px - 2 * (double)vz1[q - a.SF]));
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_fine(IfAdj<T> a, const double *__restrict__ M)
+{
+    avz_fine_body(a, M, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_avz_coarse(IfAdj<T> a, const double *__restrict__ M)
+__device__ __forceinline__ void avz_coarse_body(const IfAdj<T> &a, const double *__restrict__ M, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, ny1 = r * ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nx2 + 1) * (ny2 + 1)) return;
     const int I = t % (nx2 + 1), J = t / (nx2 + 1);
     const double r3 = (double)(r * r * r);
@@ -839,13 +921,18 @@ __global__ __launch_bounds__(256) void k_avz_coarse(IfAdj<T> a, const double *__
         }
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_avz_coarse(IfAdj<T> a, const double *__restrict__ M)
+{
+    avz_coarse_body(a, M, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 // ----- vy-syz --------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_asyz_ghost(IfAdj<T> a, const double *__restrict__ den, double *__restrict__ mu)
+__device__ __forceinline__ void asyz_ghost_body(const IfAdj<T> &a, const double *__restrict__ den, double *__restrict__ mu, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx2 * ny2) return;
     const int I = t % nx2, J = t / nx2;
     double R = 0;
@@ -857,12 +944,17 @@ __global__ __launch_bounds__(256) void k_asyz_ghost(IfAdj<T> a, const double *__
     mu[t] = ((double)s2[qc - a.SC] + R) / den[t];
     s2[qc - a.SC] = T(0);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_ghost(IfAdj<T> a, const double *__restrict__ den, double *__restrict__ mu)
+{
+    asyz_ghost_body(a, den, mu, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_asyz_fine0(IfAdj<T> a)
+__device__ __forceinline__ void asyz_fine0_body(const IfAdj<T> &a, const int t)
 {
     const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx1 * ny1) return;
     const int i = t % nx1, j = t / nx1;
     const long q = AF_(i, j, a.nz1);
@@ -870,6 +962,12 @@ __global__ __launch_bounds__(256) void k_asyz_fine0(IfAdj<T> a)
     s1[q - a.SF] -= s1[q];
     s1[q] = T(0);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_fine0(IfAdj<T> a)
+{
+    asyz_fine0_body(a, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 __device__ __forceinline__ double syz_mu(const double *mu, int I, int J, int nx2, int ny2)
 {
@@ -878,10 +976,9 @@ __device__ __forceinline__ double syz_mu(const double *mu, int I, int J, int nx2
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_asyz_fine(IfAdj<T> a, const double *__restrict__ mu)
+__device__ __forceinline__ void asyz_fine_body(const IfAdj<T> &a, const double *__restrict__ mu, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2, nx1 = r * nx2, ny1 = r * ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nx1 + 1) * (ny1 + 1)) return;
     const int i = t % (nx1 + 1), j = t / (nx1 + 1);
     auto nu = [&](int ii, int jj) -> double {
@@ -913,12 +1010,17 @@ __global__ __launch_bounds__(256) void k_asyz_fine(IfAdj<T> a, const double *__r
         acc_to(&a.gf[10][q], -nu0 * k);
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_fine(IfAdj<T> a, const double *__restrict__ mu)
+{
+    asyz_fine_body(a, mu, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_asyz_coarse(IfAdj<T> a, const double *__restrict__ mu)
+__device__ __forceinline__ void asyz_coarse_body(const IfAdj<T> &a, const double *__restrict__ mu, const int t)
 {
     const int r = a.r, nx2 = a.nx2, ny2 = a.ny2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nx2 + 1) * (ny2 + 1)) return;
     const int I = t % (nx2 + 1), J = t / (nx2 + 1);
     const double r2 = (double)(r * r);
@@ -949,6 +1051,12 @@ __global__ __launch_bounds__(256) void k_asyz_coarse(IfAdj<T> a, const double *_
         acc_to(&a.gc[10][qc], mu0 * r2 * k);
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_asyz_coarse(IfAdj<T> a, const double *__restrict__ mu)
+{
+    asyz_coarse_body(a, mu, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 
 #undef AF_
 #undef AC_

@@ -5,6 +5,7 @@
 #include "ecsg_kernels.hpp"
 #include "ecsg_adjoint.hpp"
 #include "ecsg_batch.hpp"
+#include "ecsg_batch_adjoint.hpp"
 #include "ecsg_fused.hpp"
 #include "ecsg_ii.hpp"
 #include "ecsg_radius.hpp"
@@ -2938,6 +2939,8 @@ public:
     virtual int sync() = 0;
     virtual int set_tiling(int by, int zc) = 0;
     virtual int autotune(int reps, int *by, int *zc) = 0;
+    virtual int gradient(int nsteps, const double *const *obs, double *misfit, const int *group) = 0;
+    virtual int last_interval() const = 0;
 };
 
 template <typename T>
@@ -2964,6 +2967,7 @@ public:
         drop_graph();
         hipFree(d_g); hipFree(d_gs); hipFree(d_if); hipFree(d_jl); hipFree(d_pa);
         for (auto p : d_is) hipFree(p);
+        free_grad_tables();
         if (done) hipEventDestroy(done);
         if (s) hipStreamDestroy(s);
     }
@@ -3071,44 +3075,46 @@ public:
     }
 
     template <int BYv>
-    void stencils(bool vel)
+    void stencils(bool vel, const GridDev<T> *g)
     {
         if (vel)
-            hipLaunchKernelGGL((k_vel_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, (const GridDev<T> *)d_g, (const int *)d_gs, ng, gtot);
+            hipLaunchKernelGGL((k_vel_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, g, (const int *)d_gs, ng, gtot);
         else
-            hipLaunchKernelGGL((k_stress_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, (const GridDev<T> *)d_g, (const int *)d_gs, ng,
-                               gtot);
+            hipLaunchKernelGGL((k_stress_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, g, (const int *)d_gs, ng, gtot);
     }
-    void launch_stencils(bool vel)
+    void launch_stencils(bool vel, const GridDev<T> *g = nullptr)
     {
+        if (!g) g = d_g;
         switch (by) {
-        case 1: stencils<1>(vel); break;
-        case 2: stencils<2>(vel); break;
-        case 8: stencils<8>(vel); break;
-        default: stencils<4>(vel); break;
+        case 1: stencils<1>(vel, g); break;
+        case 2: stencils<2>(vel, g); break;
+        case 8: stencils<8>(vel, g); break;
+        default: stencils<4>(vel, g); break;
         }
     }
 
     // one batched time step (main.cpp:165-261 for every member)
-    int enqueue()
+    int enqueue() { return enqueue_tables(d_g, d_if, d_jl, d_pa); }
+
+    // the same over other descriptor tables (those of a ring-store slot)
+    int enqueue_tables(const GridDev<T> *g, const IfWork<T> *w, const JacLds<T> *jl, const PointArgs<T> *pa)
     {
-        launch_stencils(true);
-        hipLaunchKernelGGL((k_recsrc_batch<T>), dim3((int)M.size()), dim3(256), 0, s, (const PointArgs<T> *)d_pa);
+        launch_stencils(true, g);
+        hipLaunchKernelGGL((k_recsrc_batch<T>), dim3((int)M.size()), dim3(256), 0, s, pa);
         if (ni) {
-            hipLaunchKernelGGL((k_gvz_rhs_batch<T>), dim3(itot[0]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[0], ni);
-            hipLaunchKernelGGL((k_gvz_solve_batch<T>), dim3(itot[1]), dim3(64), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[1], ni);
-            hipLaunchKernelGGL((k_gvz_fine_batch<T>), dim3(itot[2]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[2], ni);
+            hipLaunchKernelGGL((k_gvz_rhs_batch<T>), dim3(itot[0]), dim3(256), 0, s, w, (const int *)d_is[0], ni);
+            hipLaunchKernelGGL((k_gvz_solve_batch<T>), dim3(itot[1]), dim3(64), 0, s, w, (const int *)d_is[1], ni);
+            hipLaunchKernelGGL((k_gvz_fine_batch<T>), dim3(itot[2]), dim3(256), 0, s, w, (const int *)d_is[2], ni);
         }
-        launch_stencils(false);
+        launch_stencils(false, g);
         if (ni) {
-            hipLaunchKernelGGL((k_gsxz_rhs_batch<T>), dim3(itot[3]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[3], ni);
+            hipLaunchKernelGGL((k_gsxz_rhs_batch<T>), dim3(itot[3]), dim3(256), 0, s, w, (const int *)d_is[3], ni);
             JacParams jp{};
             jp.tol_cap = M[0]->tol_cap; jp.tol_fac = M[0]->tol_fac;
             jp.floor = 1024 * 2.220446049250313e-16;
             jp.cap = JACOBI_CAP;
-            hipLaunchKernelGGL((k_jacobi_lds_batch<T>), dim3(ni), dim3(JL), (size_t)(2 * maxw * sizeof(double)), s,
-                               (const JacLds<T> *)d_jl, jp);
-            hipLaunchKernelGGL((k_gsyz_batch<T>), dim3(itot[4]), dim3(256), 0, s, (const IfWork<T> *)d_if, (const int *)d_is[4], ni);
+            hipLaunchKernelGGL((k_jacobi_lds_batch<T>), dim3(ni), dim3(JL), (size_t)(2 * maxw * sizeof(double)), s, jl, jp);
+            hipLaunchKernelGGL((k_gsyz_batch<T>), dim3(itot[4]), dim3(256), 0, s, w, (const int *)d_is[4], ni);
         }
         HIPCHK(hipGetLastError());
         return ECSG_OK;
@@ -3218,6 +3224,446 @@ public:
         if (int rc = set_tiling(bb, bz)) return rc;
         if (by_out) *by_out = bb;
         if (zc_out) *zc_out = bz;
+        return ECSG_OK;
+    }
+    // ======================================================================
+    // Gradients of every member (each member's ecsg_gradient, bit for bit)
+    // with one launch per phase across members.  Every member keeps its own
+    // checkpoints, ring of saved states, residuals and accumulators; the
+    // forward steps are batched steps over descriptor tables whose second
+    // destinations point into each member's ring slot, the transposed steps
+    // batched launches of the adjoint kernel bodies.
+    // ======================================================================
+    int K_used = 0;
+    int last_interval() const override { return K_used; }
+    GridDev<T> *d_gF = nullptr;        // [slot][grid]: forward stencils, ring slot `slot` (0: none)
+    IfWork<T> *d_ifF = nullptr;        // [slot][interface]
+    JacLds<T> *d_jlF = nullptr;        // [slot][interface]
+    PointArgs<T> *d_paF = nullptr;     // [slot][member]
+    AdjGridDev<T> *d_ag = nullptr;     // [t][grid]: transposed step over ring slots t+1, t
+    AdjIfWork<T> *d_aw = nullptr;      // [t][interface, by level]
+    MirrorWork<T> *d_mw = nullptr;     // [set][grid], sets: free surface, vy y walls, other velocity mirrors
+    JacLds<T> *d_jla = nullptr;        // transposed vx-sxz systems
+    RecAdj<T> *d_ra = nullptr;
+    int *d_gt = nullptr, *d_gtg = nullptr;   // tile starts: adjoint stencils, gradient sweeps
+    int gt_tot = 0, gtg_tot = 0;
+    int *d_ms[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int ms_tot[6] = {0, 0, 0, 0, 0, 0};
+    int *d_ras = nullptr;
+    int ra_tot = 0;
+    int *d_aps = nullptr;              // block starts, [level][phase][interfaces of the level]
+    std::vector<int> lev_off, lev_n, aps_off, aps_tot;
+    int naw = 0, nlev = 0;
+
+    void free_grad_tables()
+    {
+        for (void *p : {(void *)d_gF, (void *)d_ifF, (void *)d_jlF, (void *)d_paF, (void *)d_ag, (void *)d_aw, (void *)d_mw,
+                        (void *)d_jla, (void *)d_ra, (void *)d_gt, (void *)d_gtg, (void *)d_ras, (void *)d_aps})
+            hipFree(p);
+        for (auto &p : d_ms) { hipFree(p); p = nullptr; }
+        d_gF = nullptr; d_ifF = nullptr; d_jlF = nullptr; d_paF = nullptr; d_ag = nullptr; d_aw = nullptr; d_mw = nullptr;
+        d_jla = nullptr; d_ra = nullptr; d_gt = nullptr; d_gtg = nullptr; d_ras = nullptr; d_aps = nullptr;
+    }
+
+    // members' calls issued on the batch stream for the duration of a scope
+    struct MembersOnBatch {
+        Batch *b;
+        std::vector<hipStream_t> saved;
+        explicit MembersOnBatch(Batch *x) : b(x)
+        {
+            for (Engine<T> *e : b->M) { saved.push_back(e->s); e->s = b->s; }
+        }
+        ~MembersOnBatch()
+        {
+            for (size_t m = 0; m < b->M.size(); m++) b->M[m]->s = saved[m];
+        }
+    };
+
+    // the largest interval whose rings and checkpoints of all members fit in
+    // free device memory (with a reserve)
+    int batch_interval(int N, int *K_out)
+    {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        double state = 0, ckb = 0, held = 0;
+        for (Engine<T> *e : M) {
+            double st1 = 0, ck1 = 0;
+            for (const DevGrid<T> &d : e->G) {
+                st1 += 9.0 * (double)d.astride * sizeof(T);
+                ck1 += 9.0 * (double)d.astride * (e->ck_fmt == 0 ? sizeof(T) : sizeof(unsigned short));
+            }
+            for (const DevIface &f : e->I) ck1 += 8.0 * (f.nx2 + 1) * (f.ny2 + 1);
+            state += st1;
+            ckb += ck1;
+            held += (double)e->ring_alloc * st1 + (double)e->ck_alloc * ck1;
+        }
+        const double avail = (double)fr + held - std::max(4.0e9, 0.05 * (double)tot);
+        int K = 1;
+        for (int k = N; k >= 1; k--) {
+            const int nseg = (N + k - 1) / k;
+            if ((k + 1) * state + std::max(nseg - 1, 1) * ckb <= avail) { K = k; break; }
+        }
+        *K_out = K;
+        return ECSG_OK;
+    }
+
+    template <typename P>
+    int upload_vec(P **dst, const std::vector<P> &v) { return upload(dst, v); }
+
+    // descriptor tables of one gradient call (K: ring slots 1..K)
+    int build_grad_tables(int K)
+    {
+        free_grad_tables();
+        std::vector<GridDev<T>> gF;
+        std::vector<IfWork<T>> ifF;
+        std::vector<JacLds<T>> jlF;
+        std::vector<PointArgs<T>> paF;
+        for (int sl = 0; sl <= K; sl++) {
+            for (Engine<T> *e : M) e->set_store(sl == 0 ? -1 : sl);
+            for (Engine<T> *e : M) {
+                for (auto &d : e->G) {
+                    GridDev<T> v = d.dev;
+                    v.zc = zc;
+                    v.gx = (d.n[0] + 1 + BX - 1) / BX;
+                    v.gy = (d.n[1] + 1 + by - 1) / by;
+                    v.nblk = v.gx * v.gy * ((d.n[2] + 1 + zc - 1) / zc);
+                    v.zb = 0;
+                    v.ze = d.n[2] + 1;
+                    gF.push_back(v);
+                }
+                for (int k = 0; k < (int)e->I.size(); k++) {
+                    DevIface &f = e->I[k];
+                    IfWork<T> w{};
+                    w.a = e->iface_dev(k);
+                    w.B = f.B; w.M = f.M; w.Rp = f.Rp; w.U = f.U; w.Bs = f.Bs; w.absp = f.absp;
+                    ifF.push_back(w);
+                    jlF.push_back(e->jac_lds(k, false));
+                }
+                paF.push_back(e->point_args());
+            }
+            for (Engine<T> *e : M) e->set_store(-1);
+        }
+        if (int rc = upload(&d_gF, gF)) return rc;
+        if (int rc = upload(&d_ifF, ifF)) return rc;
+        if (int rc = upload(&d_jlF, jlF)) return rc;
+        if (int rc = upload(&d_paF, paF)) return rc;
+        // transposed steps: grid descriptors and their tile starts
+        std::vector<AdjGridDev<T>> ag;
+        std::vector<int> gt, gtg;
+        gt_tot = gtg_tot = 0;
+        for (int t = 0; t < K; t++)
+            for (Engine<T> *e : M)
+                for (int g = 0; g < (int)e->G.size(); g++) {
+                    const AdjGridDev<T> a = e->adj_dev(g, t + 1, t);
+                    ag.push_back(a);
+                    if (t == 0) {
+                        gt.push_back(gt_tot); gt_tot += a.gx * a.gy * a.gz;
+                        gtg.push_back(gtg_tot); gtg_tot += adj_grad_tiles(a);
+                    }
+                }
+        if (int rc = upload(&d_ag, ag)) return rc;
+        if (int rc = upload(&d_gt, gt)) return rc;
+        if (int rc = upload(&d_gtg, gtg)) return rc;
+        // interfaces by level (interface index k), members in order
+        nlev = 0;
+        for (Engine<T> *e : M) nlev = std::max(nlev, (int)e->I.size());
+        lev_off.assign(nlev, 0);
+        lev_n.assign(nlev, 0);
+        std::vector<std::pair<Engine<T> *, int>> order;
+        for (int L = 0; L < nlev; L++) {
+            lev_off[L] = (int)order.size();
+            for (Engine<T> *e : M)
+                if (L < (int)e->I.size()) order.push_back({e, L});
+            lev_n[L] = (int)order.size() - lev_off[L];
+        }
+        naw = (int)order.size();
+        std::vector<AdjIfWork<T>> aw;
+        for (int t = 0; t < K; t++)
+            for (auto &ek : order) {
+                Engine<T> *e = ek.first;
+                const int k = ek.second;
+                AdjIfWork<T> w{};
+                w.a = e->if_adj(k, t + 1, t + 1);
+                w.ap = e->if_adj(k, t + 1, t);
+                w.Ba = e->AI[k].Ba; w.X0a = e->AI[k].X0a; w.mu_vz = e->AI[k].mu_vz; w.mu_syz = e->AI[k].mu_syz;
+                w.den = e->AI[k].den;
+                w.M = e->I[k].M; w.Dp = e->I[k].Dp; w.U = e->I[k].U;
+                aw.push_back(w);
+            }
+        if (int rc = upload(&d_aw, aw)) return rc;
+        std::vector<int> aps;
+        aps_off.assign((size_t)nlev * AB_NPHASE, 0);
+        aps_tot.assign((size_t)nlev * AB_NPHASE, 0);
+        for (int L = 0; L < nlev; L++)
+            for (int ph = 0; ph < AB_NPHASE; ph++) {
+                aps_off[L * AB_NPHASE + ph] = (int)aps.size();
+                int acc = 0;
+                for (int q = 0; q < lev_n[L]; q++) {
+                    const auto &ek = order[lev_off[L] + q];
+                    const DevIface &f = ek.first->I[ek.second];
+                    const long nt = adj_if_threads(ph, f.r, f.nx2, f.ny2);
+                    aps.push_back(acc);
+                    acc += (int)((nt + (ph == AB_VZ_TRI ? 63 : 255)) / (ph == AB_VZ_TRI ? 64 : 256));
+                }
+                aps_tot[L * AB_NPHASE + ph] = acc;
+            }
+        if (int rc = upload(&d_aps, aps)) return rc;
+        // transposed ghost mirrors and Jacobi descriptors
+        std::vector<MirrorWork<T>> mw;
+        std::vector<int> ms[6];
+        for (int q = 0; q < 6; q++) ms_tot[q] = 0;
+        for (int set = 0; set < 3; set++)
+            for (Engine<T> *e : M)
+                for (int g = 0; g < (int)e->G.size(); g++) {
+                    const auto &md = set == 0 ? e->AG[g].sm : set == 1 ? e->AG[g].vA : e->AG[g].vB;
+                    MirrorWork<T> w{};
+                    w.lam = e->AG[g].lam; w.src = md.src; w.start = md.start; w.halo = md.halo; w.n = md.n; w.nh = md.nh;
+                    mw.push_back(w);
+                    ms[2 * set].push_back(ms_tot[2 * set]); ms_tot[2 * set] += (md.n + 255) / 256;
+                    ms[2 * set + 1].push_back(ms_tot[2 * set + 1]); ms_tot[2 * set + 1] += (md.nh + 255) / 256;
+                }
+        if (int rc = upload(&d_mw, mw)) return rc;
+        for (int q = 0; q < 6; q++)
+            if (int rc = upload(&d_ms[q], ms[q])) return rc;
+        std::vector<JacLds<T>> jla;
+        for (Engine<T> *e : M)
+            for (int k = 0; k < (int)e->I.size(); k++) jla.push_back(e->jac_lds(k, true));
+        if (int rc = upload(&d_jla, jla)) return rc;
+        return ECSG_OK;
+    }
+
+    // misfit injection descriptors (after the residuals are on the device)
+    int build_rec_table()
+    {
+        std::vector<RecAdj<T>> ra;
+        std::vector<int> st;
+        ra_tot = 0;
+        for (Engine<T> *e : M) {
+            RecAdj<T> r{};
+            for (int g = 0; g < (int)e->G.size(); g++) r.lamvx[g] = e->AG[g].lam;
+            r.loc = e->d_rloc; r.start = e->d_rstart; r.rec = e->d_rrec; r.resid = e->d_resid;
+            r.nloc = e->nrloc; r.nsteps = e->resid_steps;
+            ra.push_back(r);
+            st.push_back(ra_tot);
+            ra_tot += (e->nrloc + 255) / 256;
+        }
+        hipFree(d_ra); d_ra = nullptr;
+        hipFree(d_ras); d_ras = nullptr;
+        if (int rc = upload(&d_ra, ra)) return rc;
+        return upload(&d_ras, st);
+    }
+
+    template <int PH>
+    void if_phase(int t, int L)
+    {
+        const int tot = aps_tot[L * AB_NPHASE + PH];
+        if (!tot) return;
+        hipLaunchKernelGGL((k_adj_if_batch<T, PH>), dim3(tot), dim3(256), 0, s,
+                           (const AdjIfWork<T> *)(d_aw + (long)t * naw + lev_off[L]),
+                           (const int *)(d_aps + aps_off[L * AB_NPHASE + PH]), lev_n[L]);
+    }
+    void mirror_phase(int set)
+    {
+        const int ngr = ng;  // grids of all members
+        const MirrorWork<T> *w = d_mw + (long)set * ngr;
+        if (ms_tot[2 * set])
+            hipLaunchKernelGGL((k_mirror_batch<T, false>), dim3(ms_tot[2 * set]), dim3(256), 0, s, w, (const int *)d_ms[2 * set], ngr);
+        if (ms_tot[2 * set + 1])
+            hipLaunchKernelGGL((k_mirror_batch<T, true>), dim3(ms_tot[2 * set + 1]), dim3(256), 0, s, w, (const int *)d_ms[2 * set + 1],
+                               ngr);
+    }
+    template <int PH>
+    void grid_phase(int t)
+    {
+        const bool grad = PH == AG_GRAD || PH == AG_GRADB;
+        const int tot = grad ? gtg_tot : gt_tot;
+        hipLaunchKernelGGL((k_adj_grid_batch<T, PH>), dim3(tot), dim3(64, grad ? GBY : AWY), 0, s,
+                           (const AdjGridDev<T> *)(d_ag + (long)t * ng), (const int *)(grad ? d_gtg : d_gt), ng, tot);
+    }
+
+    // one transposed step of every member (adj_step_impl's phases and order):
+    // forward step n, ring slots t+1 (after) and t (before)
+    int adj_step(int n, int t, bool inject)
+    {
+        for (int L = nlev - 1; L >= 0; L--) {
+            if_phase<AB_SYZ_GHOST>(t, L);
+            if_phase<AB_SYZ_FINE0>(t, L);
+            if_phase<AB_SYZ_FINE>(t, L);
+            if_phase<AB_SYZ_COARSE>(t, L);
+        }
+        if (nlev) {
+            for (int L = nlev - 1; L >= 0; L--) {
+                if_phase<AB_SXZ_GHOST>(t, L);
+                if_phase<AB_SXZ_FINE0>(t, L);
+            }
+            JacParams jp{};
+            jp.tol_cap = 0; jp.tol_fac = 0; jp.rel_tol = M[0]->adj_rel_tol; jp.solve_only = 1;
+            jp.floor = 1024 * 2.220446049250313e-16;
+            jp.cap = JACOBI_CAP;
+            hipLaunchKernelGGL((k_jacobi_lds_batch<T>), dim3(naw), dim3(JL), (size_t)(2 * maxw * sizeof(double)), s,
+                               (const JacLds<T> *)d_jla, jp);
+            for (int L = nlev - 1; L >= 0; L--) {
+                if_phase<AB_SXZ_FINE>(t, L);
+                if_phase<AB_SXZ_COARSE>(t, L);
+            }
+        }
+        mirror_phase(0);
+        grid_phase<AG_STRESS>(t);
+        grid_phase<AG_GRAD>(t);
+        for (int L = nlev - 1; L >= 0; L--) {
+            if_phase<AB_VZ_GHOST>(t, L);
+            if_phase<AB_VZ_FINE0>(t, L);
+            const int tot = aps_tot[L * AB_NPHASE + AB_VZ_TRI];
+            if (tot)
+                hipLaunchKernelGGL((k_adj_tri_batch<T>), dim3(tot), dim3(64), 0, s,
+                                   (const AdjIfWork<T> *)(d_aw + (long)t * naw + lev_off[L]),
+                                   (const int *)(d_aps + aps_off[L * AB_NPHASE + AB_VZ_TRI]), lev_n[L]);
+            if_phase<AB_VZ_FINE>(t, L);
+            if_phase<AB_VZ_COARSE>(t, L);
+        }
+        if (inject && ra_tot)
+            hipLaunchKernelGGL((k_adj_rec_batch<T>), dim3(ra_tot), dim3(256), 0, s, (const RecAdj<T> *)d_ra, (const int *)d_ras,
+                               (int)M.size(), n);
+        mirror_phase(1);
+        mirror_phase(2);
+        grid_phase<AG_VEL>(t);
+        grid_phase<AG_GRADB>(t);
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
+    int fwd_step(int slot)
+    {
+        const long nif = ni, nm = (long)M.size();
+        return enqueue_tables(d_gF + (long)slot * ng, d_ifF + (long)slot * nif, d_jlF + (long)slot * nif, d_paF + (long)slot * nm);
+    }
+
+    int gradient(int N, const double *const *obs, double *misfit, const int *group) override
+    {
+        if (N < 1) return fail(ECSG_EINVAL, "gradient: step count out of range");
+        const int nm = (int)M.size();
+        for (int m = 0; m < nm; m++) {
+            Engine<T> *e = M[m];
+            const std::string id = "member " + std::to_string(m) + ": ";
+            if (N > e->cfg.max_steps) return fail(ECSG_EINVAL, id + "gradient: step count above max_steps");
+            if (e->nrec > 0 && (!obs || !obs[m])) return fail(ECSG_EINVAL, id + "gradient: null observed traces");
+            if (e->ck_every != M[0]->ck_every || e->ck_fmt != M[0]->ck_fmt)
+                return fail(ECSG_EINVAL, "gradient: members must share one checkpoint interval and format");
+            if (e->adj_rel_tol != M[0]->adj_rel_tol) return fail(ECSG_EINVAL, "gradient: members must share one adjoint tolerance");
+            HIPCHK(hipStreamSynchronize(e->s));
+            if (int rc = e->adj_build()) return fail(rc, id + e->st.msg);
+            if (int rc = e->build_adj_receivers()) return fail(rc, id + e->st.msg);
+        }
+        if (group)
+            for (int m = 0; m < nm; m++)
+                for (int q = 0; q < m; q++)
+                    if (group[q] == group[m]) {
+                        if (M[q]->G.size() != M[m]->G.size()) return fail(ECSG_EINVAL, "gradient: a group's members differ in shape");
+                        for (size_t g = 0; g < M[m]->G.size(); g++)
+                            for (int d = 0; d < 3; d++)
+                                if (M[q]->G[g].n[d] != M[m]->G[g].n[d])
+                                    return fail(ECSG_EINVAL, "gradient: a group's members differ in shape");
+                        break;
+                    }
+        HIPCHK(hipStreamSynchronize(s));
+        for (size_t m = 0; m < M.size(); m++)
+            if (M[m]->G[0].cur != built_cur[m])
+                if (int rc = build_tables()) return rc;
+        int K = M[0]->ck_every;
+        if (K == 0)
+            if (int rc = batch_interval(N, &K)) return rc;
+        K = std::min(K, N);
+        K_used = K;
+        const int nseg = (N + K - 1) / K;
+        const int r0 = N - (nseg - 1) * K;
+        auto seg0 = [&](int j) { return j == 0 ? 0 : r0 + (j - 1) * K; };
+        auto seglen = [&](int j) { return j == 0 ? r0 : K; };
+        MembersOnBatch on(this);
+        for (int m = 0; m < nm; m++) {
+            Engine<T> *e = M[m];
+            if (int rc = e->ck_prepare(std::max(nseg - 1, 1), K)) return fail(rc, "member " + std::to_string(m) + ": " + e->st.msg);
+            e->last_K = K;
+            if (int rc = e->reset()) return fail(rc, "member " + std::to_string(m) + ": " + e->st.msg);
+        }
+        if (int rc = build_grad_tables(K)) return rc;
+        // forward pass with checkpoints; the last segment fills the rings
+        for (int j = 0; j < nseg; j++) {
+            const int len = seglen(j);
+            if (j < nseg - 1) {
+                for (Engine<T> *e : M)
+                    if (int rc = e->ck_save(j)) return fail(rc, e->st.msg);
+                for (int t = 0; t < len; t++)
+                    if (int rc = fwd_step(0)) return rc;
+            } else {
+                for (Engine<T> *e : M)
+                    if (int rc = e->ring_save(0)) return fail(rc, e->st.msg);
+                for (int t = 0; t < len; t++)
+                    if (int rc = fwd_step(t + 1)) return rc;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        // residuals and misfits, member by member
+        for (int m = 0; m < nm; m++) {
+            Engine<T> *e = M[m];
+            const int nrec = e->nrec, ms = e->cfg.max_steps;
+            std::vector<T> tr((size_t)std::max(nrec, 1) * ms);
+            if (nrec) HIPCHK(hipMemcpy(tr.data(), e->d_traces, sizeof(T) * nrec * ms, hipMemcpyDeviceToHost));
+            std::vector<T> res((size_t)std::max(nrec, 1) * N, (T)0);
+            double J = 0;
+            for (int q = 0; q < nrec; q++)
+                for (int t = 0; t < N; t++) {
+                    const double rr = (double)tr[(size_t)q * ms + t] - obs[m][(size_t)q * N + t];
+                    res[(size_t)q * N + t] = (T)rr;
+                    J += 0.5 * rr * rr;
+                }
+            if (misfit) misfit[m] = J;
+            Engine<T>::dfree(e->d_resid);
+            if (int rc = e->upload(&e->d_resid, res)) return fail(rc, e->st.msg);
+            e->resid_steps = N;
+            if (int rc = e->adj_zero()) return fail(rc, e->st.msg);
+        }
+        if (int rc = build_rec_table()) return rc;
+        // backward over the segments
+        for (int j = nseg - 1; j >= 0; j--) {
+            const int len = seglen(j);
+            if (j < nseg - 1) {
+                for (Engine<T> *e : M) {
+                    if (int rc = e->ck_restore(j, seg0(j))) return fail(rc, e->st.msg);
+                    if (int rc = e->ring_save(0)) return fail(rc, e->st.msg);
+                }
+                for (int t = 0; t < len; t++)
+                    if (int rc = fwd_step(t + 1)) return rc;
+            }
+            for (int t = len - 1; t >= 0; t--)
+                if (int rc = adj_step(seg0(j) + t, t, true)) return rc;
+        }
+        // a model's gradient: its members' accumulators added, in member
+        // order, into the first member of its group
+        if (group)
+            for (int m = 0; m < nm; m++) {
+                int lead = m;
+                for (int q = 0; q < m; q++)
+                    if (group[q] == group[m]) { lead = q; break; }
+                if (lead == m) continue;
+                for (size_t g = 0; g < M[m]->G.size(); g++) {
+                    const long n12 = 12 * M[m]->G[g].astride;
+                    hipLaunchKernelGGL((k_add_into<T>), dim3((unsigned)std::min<long>((n12 + 255) / 256, 4096)), dim3(256), 0, s,
+                                       M[lead]->AG[g].grad, (const T *)M[m]->AG[g].grad, n12);
+                }
+            }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        for (int m = 0; m < nm; m++) {
+            Engine<T> *e = M[m];
+            for (int k = 0; k < (int)e->I.size(); k++) {
+                long long st3[3];
+                HIPCHK(hipMemcpy(st3, e->AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
+                if (st3[2]) return fail(ECSG_ENOCONV, "member " + std::to_string(m) + ": adjoint Jacobi iteration cap reached");
+            }
+            e->host_step = nseg > 1 ? r0 : N;
+            e->gprim.clear();
+            e->grad_ready = true;
+        }
         return ECSG_OK;
     }
 };
@@ -3638,5 +4084,11 @@ int ecsg_batch_step_n(ecsg_batch *b, int nsteps, int use_graph) { BATCH_CHECK(b)
 int ecsg_batch_synchronize(ecsg_batch *b) { BATCH_CHECK(b); return bret(b, b->b->sync()); }
 int ecsg_batch_set_tiling(ecsg_batch *b, int by, int zc) { BATCH_CHECK(b); return bret(b, b->b->set_tiling(by, zc)); }
 int ecsg_batch_autotune(ecsg_batch *b, int reps, int *by, int *zc) { BATCH_CHECK(b); return bret(b, b->b->autotune(reps, by, zc)); }
+int ecsg_batch_gradient(ecsg_batch *b, int nsteps, const double *const *observed, double *misfit, const int *group)
+{
+    BATCH_CHECK(b);
+    return bret(b, b->b->gradient(nsteps, observed, misfit, group));
+}
+int ecsg_batch_checkpoint_interval(const ecsg_batch *b) { return b && b->b ? b->b->last_interval() : 0; }
 
 }  // extern "C"

@@ -334,6 +334,23 @@ int ecsg_batch_synchronize(ecsg_batch *b);
 int ecsg_batch_set_tiling(ecsg_batch *b, int by, int zc);
 int ecsg_batch_autotune(ecsg_batch *b, int reps, int *by, int *zc);
 
+/* Gradients of every member at once (no counterpart in the reference): each
+ * member's ecsg_gradient -- misfit against observed[m] (member m's nrec x
+ * nsteps traces, receiver-major), forward pass from rest with checkpoints,
+ * recomputation, transposed steps -- with one launch per phase of the
+ * forward and of the transposed step across all members; every member's
+ * result equals its own ecsg_gradient bit for bit.  misfit[m] receives
+ * member m's misfit.  Members share one checkpoint interval and format
+ * (ecsg_set_checkpointing; interval 0: the largest that fits every member's
+ * ring and checkpoints in free device memory together).  group (may be
+ * NULL): group[m] names member m's model; the gradient accumulators of the
+ * other members of a group are then added, in member order and on the
+ * device, into its first member, so ecsg_get_gradient on that member returns
+ * the model's gradient (the sum over its shots); the others keep their own. */
+int ecsg_batch_gradient(ecsg_batch *b, int nsteps, const double *const *observed, double *misfit, const int *group);
+/* the checkpoint interval the last ecsg_batch_gradient used */
+int ecsg_batch_checkpoint_interval(const ecsg_batch *b);
+
 /* --- Measurement.  No counterpart in the reference. --- */
 /* Copies n FP32 values from src to dst (device pointers, disjoint) on stream
  * (a hipStream_t, NULL = the null stream), one value per lane as the stencils
