@@ -457,6 +457,47 @@ __global__ __launch_bounds__(64) void k_tri_solve(double *__restrict__ B, const 
 }
 
 
+// The same with one wavefront per line (lines of at most VZ_WAVE_LDS / 32
+// entries): the line staged in LDS by the lanes, the recurrence by lane 0 in
+// tri_solve_body's operation order, the solution written back by the lanes.
+__device__ __forceinline__ void tri_solve_wave(double *__restrict__ B, const double *__restrict__ M,
+                                               const double *__restrict__ Dp, const double *__restrict__ U, int nx2, int n,
+                                               const int I, double *sh)
+{
+    double *sb = sh, *sm = sh + n, *su = sh + 2 * n, *sd = sh + 3 * n;
+    const int lane = threadIdx.x;
+    for (int q = lane; q < n; q += 64) {
+        const long r = (long)q * nx2 + I;
+        sb[q] = B[r];
+        sm[q] = M[r];
+        su[q] = U[r];
+        sd[q] = Dp[r];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double prev = sb[0];
+        for (int q = 1; q < n; q++) {
+            prev = sb[q] - sm[q] * prev;
+            sb[q] = prev;
+        }
+        double x = prev / sd[n - 1];
+        sb[n - 1] = x;
+        for (int q = n - 2; q >= 0; q--) {
+            x = (sb[q] - su[q] * x) / sd[q];
+            sb[q] = x;
+        }
+    }
+    __syncthreads();
+    for (int q = lane; q < n; q += 64) B[(long)q * nx2 + I] = sb[q];
+}
+
+__global__ __launch_bounds__(64) void k_tri_solve_wave(double *__restrict__ B, const double *__restrict__ M,
+                                                       const double *__restrict__ Dp, const double *__restrict__ U, int nx2, int n)
+{
+    extern __shared__ double tri_sh[];
+    tri_solve_wave(B, M, Dp, U, nx2, n, (int)blockIdx.x, tri_sh);
+}
+
 // Denominator of the pointwise vy-syz ghost of each coarse (I, J), as the
 // forward kernel k_gsyz forms it.
 template <typename T>

@@ -38,6 +38,16 @@ __global__ __launch_bounds__(64) void k_gvz_solve_batch(const IfWork<T> *__restr
     gvz_solve_body(W.a, (blockIdx.x - start[m]) * 64 + (int)threadIdx.x, W.B, W.M, W.Rp, W.U);
 }
 
+// one wavefront per coarse x-line of every interface (gvz_solve_wave)
+template <typename T>
+__global__ __launch_bounds__(64) void k_gvz_solve_wave_batch(const IfWork<T> *__restrict__ w, const int *__restrict__ start, int n)
+{
+    extern __shared__ double vz_sh[];
+    const int m = batch_find(start, n, blockIdx.x);
+    const IfWork<T> &W = w[m];
+    gvz_solve_wave(W.a, (int)blockIdx.x - start[m], W.B, W.M, W.Rp, W.U, vz_sh);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gvz_fine_batch(const IfWork<T> *__restrict__ w, const int *__restrict__ start, int n)
 {

@@ -48,7 +48,7 @@ __host__ __device__ inline long adj_if_threads(int ph, int r, int nx2, int ny2)
     case AB_VZ_GHOST: return (long)nx2 * (ny2 + 1);
     case AB_VZ_FINE0: return nx1 * (ny1 - 1);
     case AB_SYZ_FINE: case AB_SXZ_FINE: case AB_VZ_FINE: return (nx1 + 1) * (ny1 + 1);
-    case AB_VZ_TRI: return nx2;
+    case AB_VZ_TRI: return nx2;  // one workgroup per line
     default: return (long)(nx2 + 1) * (ny2 + 1);  // the three coarse phases
     }
 }
@@ -73,13 +73,15 @@ __global__ __launch_bounds__(256) void k_adj_if_batch(const AdjIfWork<T> *__rest
     else if (PH == AB_VZ_COARSE) avz_coarse_body(W.ap, (const double *)W.mu_vz, t);
 }
 
-// the tridiagonal solves of the transposed vz-szz systems: 64-thread blocks
+// the tridiagonal solves of the transposed vz-szz systems: one wavefront per
+// coarse x-line (tri_solve_wave)
 template <typename T>
 __global__ __launch_bounds__(64) void k_adj_tri_batch(const AdjIfWork<T> *__restrict__ w, const int *__restrict__ start, int n)
 {
+    extern __shared__ double tri_sh[];
     const int m = batch_find(start, n, blockIdx.x);
     const AdjIfWork<T> &W = w[m];
-    tri_solve_body(W.mu_vz, W.M, W.Dp, W.U, W.a.nx2, W.a.ny2 - 1, (blockIdx.x - start[m]) * 64 + (int)threadIdx.x);
+    tri_solve_wave(W.mu_vz, W.M, W.Dp, W.U, W.a.nx2, W.a.ny2 - 1, (int)blockIdx.x - start[m], tri_sh);
 }
 
 // adjoint stencils and gradient sweeps over the tiles of every grid

@@ -1051,7 +1051,7 @@ public:
             const long nl = (long)f.nx2 * (f.ny2 - 1);
             ev_begin();
             hipLaunchKernelGGL((k_gvz_rhs<T>), dim3((nl + 255) / 256), dim3(256), 0, s, a, f.B);
-            hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Rp, f.U);
+            launch_vz_solve(a, f);
             const long nf = (long)f.r * f.nx2 * (f.r * f.ny2 - 1);
             hipLaunchKernelGGL((k_gvz_fine<T>), dim3((nf + 255) / 256), dim3(256), 0, s, a);
             ev_end(4);
@@ -1257,6 +1257,18 @@ public:
         return ECSG_OK;
     }
 
+    // the vz-szz tridiagonal solves: one wavefront per coarse x-line when the
+    // line fits in LDS, else one thread per line
+    void launch_vz_solve(const IfaceDev<T> &a, const DevIface &f)
+    {
+        const size_t lds = (size_t)32 * (f.ny2 - 1);
+        if (lds <= (size_t)VZ_WAVE_LDS)
+            hipLaunchKernelGGL((k_gvz_solve_wave<T>), dim3(f.nx2), dim3(64), lds, s, a, (const double *)f.B, (const double *)f.M,
+                               (const double *)f.Rp, (const double *)f.U);
+        else
+            hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Rp, f.U);
+    }
+
     IfaceDev<T> iface_dev(int k) const
     {
         IfaceDev<T> a{};
@@ -1283,7 +1295,7 @@ public:
         const long nl = (long)f.nx2 * (f.ny2 - 1);
         ev_begin();
         hipLaunchKernelGGL((k_gvz_rhs<T>), dim3((nl + 255) / 256), dim3(256), 0, s, a, f.B);
-        hipLaunchKernelGGL((k_gvz_solve<T>), dim3((f.nx2 + 63) / 64), dim3(64), 0, s, a, f.B, f.M, f.Rp, f.U);
+        launch_vz_solve(a, f);
         const long nf = (long)f.r * f.nx2 * (f.r * f.ny2 - 1);
         hipLaunchKernelGGL((k_gvz_fine<T>), dim3((nf + 255) / 256), dim3(256), 0, s, a);
         ev_end(4);
@@ -2413,8 +2425,12 @@ public:
             const int nx1 = a.r * a.nx2, ny1 = a.r * a.ny2;
             hipLaunchKernelGGL((k_avz_ghost<T>), dim3(nb((long)a.nx2 * (a.ny2 + 1))), dim3(256), 0, s, a, AI[k].mu_vz);
             hipLaunchKernelGGL((k_avz_fine0<T>), dim3(nb((long)nx1 * (ny1 - 1))), dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_tri_solve, dim3((I[k].nx2 + 63) / 64), dim3(64), 0, s, AI[k].mu_vz, I[k].M, I[k].Dp, I[k].U,
-                               I[k].nx2, I[k].ny2 - 1);
+            if ((size_t)32 * (I[k].ny2 - 1) <= (size_t)VZ_WAVE_LDS)
+                hipLaunchKernelGGL(k_tri_solve_wave, dim3(I[k].nx2), dim3(64), (size_t)32 * (I[k].ny2 - 1), s, AI[k].mu_vz,
+                                   (const double *)I[k].M, (const double *)I[k].Dp, (const double *)I[k].U, I[k].nx2, I[k].ny2 - 1);
+            else
+                hipLaunchKernelGGL(k_tri_solve, dim3((I[k].nx2 + 63) / 64), dim3(64), 0, s, AI[k].mu_vz, I[k].M, I[k].Dp, I[k].U,
+                                   I[k].nx2, I[k].ny2 - 1);
             hipLaunchKernelGGL((k_avz_fine<T>), dim3(nb((long)(nx1 + 1) * (ny1 + 1))), dim3(256), 0, s, a, (const double *)AI[k].mu_vz);
             hipLaunchKernelGGL((k_avz_coarse<T>), dim3(nb((long)(a.nx2 + 1) * (a.ny2 + 1))), dim3(256), 0, s, a,
                                (const double *)AI[k].mu_vz);
@@ -2958,6 +2974,7 @@ public:
     int ni = 0, itot[5] = {0, 0, 0, 0, 0};
     JacLds<T> *d_jl = nullptr;
     long maxw = 0;
+    long maxn_vz = 0;  // longest vz-szz line (the wavefront solves' LDS)
     PointArgs<T> *d_pa = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -2998,6 +3015,8 @@ public:
             if (e->cfg.imode != 1 && !e->I.empty()) return fail(ECSG_EINVAL, id + "batched composite models use the energy-conserving interface (imode 1)");
             if (e->cfg.radius != 1) return fail(ECSG_EINVAL, id + "batched models are radius 1");
             if (!e->I.empty() && !e->lds_ok) return fail(ECSG_EINVAL, id + "interface too large for the LDS Jacobi of a batch");
+            for (const DevIface &f : e->I)
+                if ((size_t)32 * (f.ny2 - 1) > (size_t)VZ_WAVE_LDS) return fail(ECSG_EINVAL, id + "vz-szz line too long for a batch");
         }
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
@@ -3011,6 +3030,7 @@ public:
         built_cur.clear();
         for (Engine<T> *e : M) built_cur.push_back(e->G[0].cur);
         maxw = 0;
+        maxn_vz = 0;
         if (int rc = build_grids()) return rc;
         // interfaces, Jacobi descriptors and point kernels
         std::vector<IfWork<T>> ifs;
@@ -3027,11 +3047,12 @@ public:
                 w.B = f.B; w.M = f.M; w.Rp = f.Rp; w.U = f.U; w.Bs = f.Bs; w.absp = f.absp;
                 ifs.push_back(w);
                 const long r = f.r;
-                const long nb[5] = {((long)f.nx2 * (f.ny2 - 1) + 255) / 256, ((long)f.nx2 + 63) / 64,
+                const long nb[5] = {((long)f.nx2 * (f.ny2 - 1) + 255) / 256, (long)f.nx2,
                                     (r * f.nx2 * (r * f.ny2 - 1) + 255) / 256, (long)f.nabs, ((long)f.nx2 * f.ny2 + 255) / 256};
                 for (int q = 0; q < 5; q++) { st[q].push_back(acc[q]); acc[q] += (int)nb[q]; }
                 jl.push_back(e->jac_lds(k, false));
                 maxw = std::max(maxw, (long)(f.nx2 + 1) * (f.ny2 + 1));
+                maxn_vz = std::max(maxn_vz, (long)(f.ny2 - 1));
             }
         }
         ni = (int)ifs.size();
@@ -3103,7 +3124,7 @@ public:
         hipLaunchKernelGGL((k_recsrc_batch<T>), dim3((int)M.size()), dim3(256), 0, s, pa);
         if (ni) {
             hipLaunchKernelGGL((k_gvz_rhs_batch<T>), dim3(itot[0]), dim3(256), 0, s, w, (const int *)d_is[0], ni);
-            hipLaunchKernelGGL((k_gvz_solve_batch<T>), dim3(itot[1]), dim3(64), 0, s, w, (const int *)d_is[1], ni);
+            hipLaunchKernelGGL((k_gvz_solve_wave_batch<T>), dim3(itot[1]), dim3(64), (size_t)(32 * maxn_vz), s, w, (const int *)d_is[1], ni);
             hipLaunchKernelGGL((k_gvz_fine_batch<T>), dim3(itot[2]), dim3(256), 0, s, w, (const int *)d_is[2], ni);
         }
         launch_stencils(false, g);
@@ -3403,7 +3424,7 @@ public:
                     const DevIface &f = ek.first->I[ek.second];
                     const long nt = adj_if_threads(ph, f.r, f.nx2, f.ny2);
                     aps.push_back(acc);
-                    acc += (int)((nt + (ph == AB_VZ_TRI ? 63 : 255)) / (ph == AB_VZ_TRI ? 64 : 256));
+                    acc += (int)(ph == AB_VZ_TRI ? nt : (nt + 255) / 256);
                 }
                 aps_tot[L * AB_NPHASE + ph] = acc;
             }
@@ -3515,7 +3536,7 @@ public:
             if_phase<AB_VZ_FINE0>(t, L);
             const int tot = aps_tot[L * AB_NPHASE + AB_VZ_TRI];
             if (tot)
-                hipLaunchKernelGGL((k_adj_tri_batch<T>), dim3(tot), dim3(64), 0, s,
+                hipLaunchKernelGGL((k_adj_tri_batch<T>), dim3(tot), dim3(64), (size_t)(32 * maxn_vz), s,
                                    (const AdjIfWork<T> *)(d_aw + (long)t * naw + lev_off[L]),
                                    (const int *)(d_aps + aps_off[L * AB_NPHASE + AB_VZ_TRI]), lev_n[L]);
             if_phase<AB_VZ_FINE>(t, L);

@@ -606,6 +606,61 @@ __global__ __launch_bounds__(64) void k_gvz_solve(IfaceDev<T> a, double *__restr
     gvz_solve_body(a, blockIdx.x * blockDim.x + threadIdx.x, B, M, Rp, U);
 }
 
+// The same solve with one wavefront per coarse x-line I (the form used
+// whenever the line fits in LDS, 32 (ny2-1) bytes <= VZ_WAVE_LDS): the 64
+// lanes stage the line's right-hand side and elimination factors in LDS, lane
+// 0 runs the two sweeps of the recurrence there -- operation for operation as
+// gvz_solve_body -- and the lanes write the ghosts vz2(I, J, -1) back.  The
+// line's chain then waits on LDS instead of HBM latencies.
+constexpr int VZ_WAVE_LDS = 65536;
+template <typename T>
+__device__ __forceinline__ void gvz_solve_wave(const IfaceDev<T> &a, const int I, const double *__restrict__ B,
+                                               const double *__restrict__ M, const double *__restrict__ Rp,
+                                               const double *__restrict__ U, double *sh)
+{
+    const int nx2 = a.nx2, n = a.ny2 - 1;
+    double *sb = sh, *sm = sh + n, *su = sh + 2 * n, *sr = sh + 3 * n;
+    const int lane = threadIdx.x;
+    for (int q = lane; q < n; q += 64) {
+        const long r = (long)q * nx2 + I;
+        sb[q] = B[r];
+        sm[q] = M[r];
+        su[q] = U[r];
+        sr[q] = Rp[r];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double prev = sb[0];
+        for (int q = 1; q < n; q++) {
+            prev = sb[q] - sm[q] * prev;
+            sb[q] = prev;
+        }
+        double x = prev * sr[n - 1];
+        sb[n - 1] = x;
+        for (int q = n - 2; q >= 0; q--) {
+            x = (sb[q] - su[q] * x) * sr[q];
+            sb[q] = x;
+        }
+    }
+    __syncthreads();
+    T *vz2 = a.C.f[2];
+    T *vz2s = a.C.st[2];
+    for (int q = lane; q < n; q += 64) {
+        const T v = (T)sb[q];
+        vz2[FIDX(a.C, I, q + 1, -1)] = v;
+        if (vz2s) vz2s[FIDX(a.C, I, q + 1, -1)] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_gvz_solve_wave(IfaceDev<T> a, const double *__restrict__ B,
+                                                       const double *__restrict__ M, const double *__restrict__ Rp,
+                                                       const double *__restrict__ U)
+{
+    extern __shared__ double vz_sh[];
+    gvz_solve_wave(a, (int)blockIdx.x, B, M, Rp, U, vz_sh);
+}
+
 // Fine ghosts vz1(., ., nz1): injection in x, hat in y (update_ghost_vz.cpp:
 // 231-248 generalised).  One thread per fine (i, j) with j in [1, ny1-1].
 template <typename T>
