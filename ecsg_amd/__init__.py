@@ -37,7 +37,7 @@ ABI_SYMBOLS = (
     "ecsg_batch_create", "ecsg_batch_destroy", "ecsg_batch_last_error", "ecsg_batch_step_n", "ecsg_batch_synchronize",
     "ecsg_batch_set_tiling", "ecsg_batch_autotune", "ecsg_field_avgabs", "ecsg_sbp_residuals",
     "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_stream_probe_floats", "ecsg_stream_probe",
-    "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed", "ecsg_set_barrier_timeout",
+    "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed", "ecsg_set_barrier_timeout", "ecsg_set_jacobi_tol",
     "ecsg_set_params_staged", "ecsg_cfl_dt", "ecsg_cfl_dt_noalloc", "ecsg_batch_gradient",
     "ecsg_batch_checkpoint_interval",
 )
@@ -119,6 +119,7 @@ def load_library(path: Optional[str] = None):
     L.ecsg_overlap_active.argtypes = [vp]
     L.ecsg_step_n_timed.argtypes = [vp, i, P(d)]
     L.ecsg_set_barrier_timeout.argtypes = [vp, i]
+    L.ecsg_set_jacobi_tol.argtypes = [vp, C.c_double, C.c_double]
     L.ecsg_set_params_staged.argtypes = [vp, i, P(d), P(P(d))]
     L.ecsg_cfl_dt.argtypes = [P(EcsgConfig), P(P(d)), P((P(d) * 9)), i, d, P(d)]
     L.ecsg_cfl_dt_noalloc.argtypes = [P(EcsgConfig), P(vp), P(vp * 9), i, d, vp, vp]
@@ -195,7 +196,7 @@ class Simulation:
 
     def __init__(self, m, precision: str = "f64", check_energy: bool = True, max_steps: Optional[int] = None,
                  vy_mirror_all: bool = False, sxz_solver: str = "jacobi", ii_vz_xend: bool = False,
-                 staged: bool = False):
+                 staged: bool = False, jacobi_tol=None):
         self.L = load_library()
         self.model = m
         cfg = EcsgConfig()
@@ -212,7 +213,7 @@ class Simulation:
         cfg.precision = {"f32": F32, "f64": F64}[precision]
         cfg.max_steps = int(m.nsteps if max_steps is None else max_steps)
         cfg.vy_mirror_all = int(vy_mirror_all)
-        cfg.sxz_solver = {"jacobi": 0, "chebyshev": 1, "jacobi_lds": 2}[sxz_solver]
+        cfg.sxz_solver = {"jacobi": 0, "chebyshev": 1, "jacobi_lds": 2, "direct": 3}[sxz_solver]
         cfg.ii_vz_xend = int(ii_vz_xend)
         self.max_steps = cfg.max_steps
         self.precision = precision
@@ -225,6 +226,8 @@ class Simulation:
             self.h = None
             raise EcsgError("ecsg_create failed (%d): %s" % (rc, msg))
         _OPEN.add(self)
+        if jacobi_tol is not None:
+            self.set_jacobi_tol(*jacobi_tol)
         for g in range(m.num_grids):
             if staged:
                 self.set_params_staged(g)
@@ -268,6 +271,11 @@ class Simulation:
     def set_barrier_timeout(self, ms: int):
         """Longest wait at one grid barrier of the cooperative Jacobi solve."""
         self._chk(self.L.ecsg_set_barrier_timeout(self.h, int(ms)))
+
+    def set_jacobi_tol(self, tol_cap: float, tol_fac: float):
+        """Exit tolerance min(tol_cap, tol_fac * AvgAbs(B)) of the forward
+        Jacobi sweeps (defaults: 1e-3, 1e-4 in f32; 1e-12, 1e-5 in f64)."""
+        self._chk(self.L.ecsg_set_jacobi_tol(self.h, float(tol_cap), float(tol_fac)))
 
     def close(self):
         if getattr(self, "h", None):

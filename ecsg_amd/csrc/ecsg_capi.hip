@@ -106,6 +106,87 @@ const int ST_POS[9][3] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0},
 
 constexpr int ZCHUNK = 8;
 constexpr int JACOBI_CAP = 100000;
+constexpr size_t DIRECT_MAX_BYTES = (size_t)512 << 20;  // direct vx-sxz factors per interface
+
+// Inverse of the dense n x n row-major matrix a (destroyed), Gauss-Jordan
+// elimination with partial pivoting; false when a pivot column is zero.
+bool invert_dense(std::vector<double> &a, std::vector<double> &inv, int n)
+{
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        double best = std::fabs(a[(size_t)c * n + c]);
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs(a[(size_t)r * n + c]) > best) { best = std::fabs(a[(size_t)r * n + c]); p = r; }
+        if (!(best > 0) || !std::isfinite(best)) return false;
+        if (p != c)
+            for (int j = 0; j < n; j++) {
+                std::swap(a[(size_t)p * n + j], a[(size_t)c * n + j]);
+                std::swap(inv[(size_t)p * n + j], inv[(size_t)c * n + j]);
+            }
+        const double piv = a[(size_t)c * n + c];
+        for (int j = c; j < n; j++) a[(size_t)c * n + j] /= piv;
+        for (int j = 0; j < n; j++) inv[(size_t)c * n + j] /= piv;
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * n + c];
+            if (f == 0) continue;
+            for (int j = c; j < n; j++) a[(size_t)r * n + j] -= f * a[(size_t)c * n + j];
+            for (int j = 0; j < n; j++) inv[(size_t)r * n + j] -= f * inv[(size_t)c * n + j];
+        }
+    }
+    return true;
+}
+
+// Factors of the direct vx-sxz solve (k_sxz_direct): the inverse Schur
+// complements W_J = S_J^-1 of the block-tridiagonal 9-band matrix A (band m
+// at A + m*nn couples unknown (I, J) to (I + m/3 - 1, J + m%3 - 1), rows of
+// n0 unknowns, n1 rows): S_0 = D_0, S_J = D_J - L_J W_{J-1} U_{J-1}, each W_J
+// stored column-major, entry (i, k) at W + J*n0*n0 + k*n0 + i.
+bool sxz_schur_inverses(const std::vector<double> &A, int n0, int n1, std::vector<double> &W)
+{
+    const size_t nn = (size_t)n0 * n1, bs = (size_t)n0 * n0;
+    W.assign((size_t)n1 * bs, 0.0);
+    auto band = [&](int m, int J, int i) { return A[m * nn + (size_t)J * n0 + i]; };
+    std::vector<double> S(bs), M(bs), inv;
+    for (int J = 0; J < n1; J++) {
+        std::fill(S.begin(), S.end(), 0.0);
+        for (int i = 0; i < n0; i++) {
+            if (i > 0) S[(size_t)i * n0 + i - 1] = band(1, J, i);
+            S[(size_t)i * n0 + i] = band(4, J, i);
+            if (i + 1 < n0) S[(size_t)i * n0 + i + 1] = band(7, J, i);
+        }
+        if (J > 0) {
+            const double *Wp = &W[(size_t)(J - 1) * bs];
+            // M = W_{J-1} U_{J-1}, U_{J-1}(k, k + d) = band(3(d+1)+2, J-1, k)
+            for (int i = 0; i < n0; i++)
+                for (int c = 0; c < n0; c++) {
+                    double acc = 0;
+                    for (int d = -1; d <= 1; d++) {
+                        const int k = c - d;
+                        if (k < 0 || k >= n0) continue;
+                        acc += Wp[(size_t)k * n0 + i] * band(3 * (d + 1) + 2, J - 1, k);
+                    }
+                    M[(size_t)i * n0 + c] = acc;
+                }
+            // S -= L_J M, L_J(i, i + d) = band(3(d+1), J, i)
+            for (int i = 0; i < n0; i++)
+                for (int d = -1; d <= 1; d++) {
+                    const int k = i + d;
+                    if (k < 0 || k >= n0) continue;
+                    const double l = band(3 * (d + 1), J, i);
+                    if (l == 0) continue;
+                    for (int c = 0; c < n0; c++) S[(size_t)i * n0 + c] -= l * M[(size_t)k * n0 + c];
+                }
+        }
+        if (!invert_dense(S, inv, n0)) return false;
+        double *Wj = &W[(size_t)J * bs];
+        for (int i = 0; i < n0; i++)
+            for (int k = 0; k < n0; k++) Wj[(size_t)k * n0 + i] = inv[(size_t)i * n0 + k];
+    }
+    return true;
+}
 
 struct Status {
     int code = ECSG_OK;
@@ -164,6 +245,7 @@ public:
     virtual int set_overlap(int on) = 0;
     virtual int overlap_active() const = 0;
     virtual int set_barrier_timeout(int ms) = 0;
+    virtual int set_jacobi_tol(double cap, double fac) = 0;
 };
 
 #define HIPCHK(x)                                                                                  \
@@ -207,6 +289,7 @@ struct DevIface {
     double *Rp = nullptr;                                            // 1 / Dp
     double *Bs = nullptr, *A = nullptr, *absp = nullptr;             // sxz system
     double *X0 = nullptr, *X1 = nullptr, *jpart = nullptr;
+    double *W = nullptr;  // direct sxz mode: inverse Schur complements (k_sxz_direct)
     long long *stats = nullptr;
     int nabs = 0, coop = 0, tiles = 0;
     double rho = 0;  // Gershgorin bound of the Jacobi iteration matrix
@@ -259,6 +342,9 @@ public:
     long lds_pref_maxw = LDS_PREF_MAXW;
     // the single-model step solves the vx-sxz systems with k_jacobi_lds
     bool use_lds() const { return lds_ok && lds_maxw <= lds_pref_maxw; }
+    // the step's vx-sxz solves hold no grid barrier (one workgroup per
+    // interface: LDS Jacobi or the direct solve)
+    bool no_grid_barrier() const { return cfg.sxz_solver == 3 || (use_lds() && cfg.sxz_solver == 0); }
     long lds_maxw = 0;
     // fused velocity+stress pass on double-buffered fields
     int use_fused = 0;
@@ -295,7 +381,7 @@ public:
         for (auto &f : I) {
             hipFree(f.B); hipFree(f.M); hipFree(f.Dp); hipFree(f.U); hipFree(f.Rp);
             hipFree(f.Bs); hipFree(f.A); hipFree(f.absp); hipFree(f.X0); hipFree(f.X1);
-            hipFree(f.jpart); hipFree(f.stats);
+            hipFree(f.jpart); hipFree(f.stats); hipFree(f.W);
         }
         hipFree(d_bar);
         hipFree(d_rec); hipFree(d_traces); hipFree(d_step); hipFree(d_energy); hipFree(d_fix); hipFree(d_diag); hipFree(e_save); hipFree(d_sbp);
@@ -321,8 +407,8 @@ public:
     int init(const ecsg_config &c) override
     {
         cfg = c;
-        if (c.sxz_solver < 0 || c.sxz_solver > 2)
-            return fail(ECSG_EINVAL, "sxz_solver must be 0 (Jacobi), 1 (Chebyshev) or 2 (Jacobi, one workgroup)");
+        if (c.sxz_solver < 0 || c.sxz_solver > 3)
+            return fail(ECSG_EINVAL, "sxz_solver must be 0 (Jacobi), 1 (Chebyshev), 2 (Jacobi, one workgroup) or 3 (direct)");
         if (c.sxz_solver == 2) {
             // the same sweeps, one workgroup per interface wherever the iterate fits in LDS
             cfg.sxz_solver = 0;
@@ -431,6 +517,16 @@ public:
             const size_t np = std::max(4 * (size_t)f.coop + 1, (size_t)2 * f.tiles * 2 * SJ);
             if (int rc = dalloc(&f.jpart, np)) return rc;
             if (int rc = dalloc(&f.stats, 3)) return rc;
+            if (cfg.sxz_solver == 3) {
+                const long n0 = f.nx2 - 1, n1 = f.ny2 - 1;
+                if (n0 > JD_MAXN0)
+                    return fail(ECSG_EINVAL, "interface " + std::to_string(k) + ": coarse rows of more than " +
+                                                 std::to_string(JD_MAXN0) + " unknowns for the direct vx-sxz solve");
+                if ((double)n1 * n0 * n0 * sizeof(double) > DIRECT_MAX_BYTES)
+                    return fail(ECSG_EINVAL, "interface " + std::to_string(k) + ": direct vx-sxz factors above " +
+                                                 std::to_string(DIRECT_MAX_BYTES >> 20) + " MiB");
+                if (int rc = dalloc(&f.W, (size_t)n1 * n0 * n0)) return rc;
+            }
         }
         for (int k0 = 0; k0 < (int)I.size(); k0 += MAXI) {
             int tot = 0;
@@ -851,6 +947,12 @@ public:
             rho = std::max(rho, off / A[(size_t)4 * nn + q]);
         }
         f.rho = rho;
+        if (cfg.sxz_solver == 3) {
+            std::vector<double> W;
+            if (!sxz_schur_inverses(A, n0, n1, W))
+                return fail(ECSG_EINVAL, "interface " + std::to_string(k) + ": singular vx-sxz block in the direct factorisation");
+            HIPCHK(hipMemcpy(f.W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice));
+        }
         f.ready = true;
         return ECSG_OK;
     }
@@ -928,6 +1030,19 @@ public:
         bar_limit = (unsigned long long)ms * (unsigned long long)khz;
         return ECSG_OK;
     }
+    // Exit tolerance of the forward Jacobi sweeps, tol = min(cap, fac *
+    // AvgAbs(B)) as update_ghost_sxz.cpp:229-236 forms it; the defaults are
+    // that line's constants of the storage precision (1e-3, 1e-4 single;
+    // 1e-12, 1e-5 double), the solve itself runs in double either way
+    int set_jacobi_tol(double cap, double fac) override
+    {
+        if (!(cap >= 0) || !(fac >= 0) || !std::isfinite(cap) || !std::isfinite(fac))
+            return fail(ECSG_EINVAL, "Jacobi tolerance factors must be finite and >= 0");
+        tol_cap = cap;
+        tol_fac = fac;
+        invalidate_graph();
+        return ECSG_OK;
+    }
     // A timed-out grid barrier: clear the state for the next launch and
     // report it (the solve it belonged to ended early, so the state is wrong)
     int check_barrier()
@@ -947,7 +1062,7 @@ public:
     // hold; such launches get the GPU to themselves (serial step).
     bool coop_one_per_cu() const
     {
-        if (use_lds()) return true;  // no cooperative launch in the step
+        if (no_grid_barrier()) return true;
         for (int k0 = 0; k0 < (int)I.size(); k0 += MAXI) {
             long tot = 0;
             for (int k = k0; k < std::min((int)I.size(), k0 + MAXI); k++)
@@ -1078,12 +1193,7 @@ public:
             ev_begin();
             for (int k = 0; k + 1 < ng; k++)
                 if (int rc = launch_sxz_rhs(k)) return rc;
-            if (use_lds() && cfg.sxz_solver == 0) {
-                if (int rc = launch_jacobi_lds(false)) return rc;
-            } else {
-                for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
-                    if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
-            }
+            if (int rc = launch_sxz_solves()) return rc;
             ev_end(4);
         }
         for (int k = 0; k + 1 < ng; k++)
@@ -1097,7 +1207,7 @@ public:
         if (sets_dirty)
             if (int rc = sync_sets()) return rc;
         int left = n;
-        const bool can_graph = use_graph && !graph_failed && !prof && (I.empty() || (use_lds() && cfg.sxz_solver == 0));
+        const bool can_graph = use_graph && !graph_failed && !prof && (I.empty() || no_grid_barrier());
         if (can_graph) {
             if (G[0].cur == 1 && left > 0) {
                 if (int rc = enqueue_fused_step()) return rc;
@@ -1404,7 +1514,13 @@ public:
             jb.bar_limit = bar_limit;
             HIPCHK(launch_cooperative((const void *)k_jacobi<T>, dim3(blk), dim3(JT), args, s));
         }
-        // fine ghosts from the coarse ghosts just written
+        return launch_sxz_fine(jb, k0, n);
+    }
+
+    // fine ghosts of interfaces [k0, k0+n) from the coarse ghosts just
+    // written (n <= MAXI; jb's ghost views filled)
+    int launch_sxz_fine(const JacBatch<T> &jb, int k0, int n)
+    {
         long nfmax = 0;
         for (int q = 0; q < n; q++) {
             const long r = I[k0 + q].r;
@@ -1413,6 +1529,51 @@ public:
         if (nfmax > (1L << 31) - 256) return fail(ECSG_EINVAL, "interface too large for the fine ghost kernel");
         hipLaunchKernelGGL((k_jacobi_fine<T>), dim3((unsigned)((nfmax + 255) / 256), n), dim3(256), 0, s, jb);
         HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+
+    // Direct vx-sxz solves of interfaces [k0, k0+n) (one workgroup each),
+    // then their fine ghosts
+    int launch_sxz_direct(int k0, int n)
+    {
+        SxzDirectSet<T> set{};
+        for (int q = 0; q < n; q++) {
+            DevIface &f = I[k0 + q];
+            const DevGrid<T> &C = G[k0 + q + 1];
+            SxzDirect<T> &D = set.it[q];
+            D.A = f.A; D.W = f.W; D.B = f.Bs; D.X0 = f.X0; D.Y = f.X1;
+            D.sxz2 = C.dev.f[7]; D.sxz2s = C.dev.st[7];
+            D.o2 = C.o; D.P2 = C.P; D.S22 = C.S2;
+            D.nx2 = f.nx2; D.ny2 = f.ny2;
+        }
+        hipLaunchKernelGGL((k_sxz_direct<T>), dim3(n), dim3(JD), 0, s, set);
+        HIPCHK(hipGetLastError());
+        for (int q0 = 0; q0 < n; q0 += MAXI) {
+            const int m = std::min(MAXI, n - q0);
+            JacBatch<T> jb{};
+            for (int q = 0; q < m; q++) {
+                const int k = k0 + q0 + q;
+                JacItf<T> &J = jb.it[q];
+                J.sxz1 = G[k].dev.f[7]; J.sxz2 = G[k + 1].dev.f[7];
+                J.sxz1s = G[k].dev.st[7]; J.sxz2s = G[k + 1].dev.st[7];
+                J.o1 = G[k].o; J.P1 = G[k].P; J.S21 = G[k].S2;
+                J.o2 = G[k + 1].o; J.P2 = G[k + 1].P; J.S22 = G[k + 1].S2;
+                J.nx2 = I[k].nx2; J.ny2 = I[k].ny2; J.nz1 = I[k].nz1; J.r = I[k].r;
+            }
+            jb.n = m;
+            if (int rc = launch_sxz_fine(jb, k0 + q0, m)) return rc;
+        }
+        return ECSG_OK;
+    }
+
+    // the vx-sxz solves of every interface, in the configured form
+    int launch_sxz_solves()
+    {
+        const int ni = (int)I.size();
+        if (cfg.sxz_solver == 3) return launch_sxz_direct(0, ni);
+        if (use_lds() && cfg.sxz_solver == 0) return launch_jacobi_lds(false);
+        for (int k0 = 0; k0 < ni; k0 += MAXI)
+            if (int rc = launch_jacobi(k0, std::min(MAXI, ni - k0))) return rc;
         return ECSG_OK;
     }
 
@@ -1429,6 +1590,11 @@ public:
         }
         ev_begin();
         if (int rc = launch_sxz_rhs(k)) return rc;
+        if (cfg.sxz_solver == 3) {
+            if (int rc = launch_sxz_direct(k, 1)) return rc;
+            ev_end(4);
+            return ECSG_OK;
+        }
         if (use_lds() && cfg.sxz_solver == 0) {
             JacLdsSet<T> set{};
             set.it[0] = jac_lds(k, false);
@@ -1644,12 +1810,7 @@ public:
             ev_begin();
             for (int k = 0; k + 1 < ng; k++)
                 if (int rc = launch_sxz_rhs(k)) return rc;
-            if (use_lds() && cfg.sxz_solver == 0) {
-                if (int rc = launch_jacobi_lds(false)) return rc;
-            } else {
-                for (int k0 = 0; k0 + 1 < ng; k0 += MAXI)
-                    if (int rc = launch_jacobi(k0, std::min(MAXI, ng - 1 - k0))) return rc;
-            }
+            if (int rc = launch_sxz_solves()) return rc;
             ev_end(4);
         }
         for (int k = 0; k + 1 < ng; k++)
@@ -1873,7 +2034,7 @@ public:
         // every other handle's through a process-wide event, which a captured
         // graph cannot hold, so composite models with interfaces too large
         // for the LDS solver run eagerly.
-        if (use_graph && !graph_failed && !prof && (I.empty() || cfg.imode == 0 || (use_lds() && cfg.sxz_solver == 0))) {
+        if (use_graph && !graph_failed && !prof && (I.empty() || cfg.imode == 0 || no_grid_barrier())) {
             if (!gexec) {
                 if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                     int rc = enqueue_step();
@@ -3012,6 +3173,8 @@ public:
             if (!e->all_params()) return fail(ECSG_ESTATE, id + "parameters not set for every grid");
             if (e->cfg.check_energy) return fail(ECSG_EINVAL, id + "batched models run without the energy check");
             if (e->cfg.sxz_solver != 0) return fail(ECSG_EINVAL, id + "batched models use the Jacobi sweeps (sxz_solver 0)");
+            if (e->tol_cap != M[0]->tol_cap || e->tol_fac != M[0]->tol_fac)
+                return fail(ECSG_EINVAL, id + "members must share one Jacobi tolerance");
             if (e->cfg.imode != 1 && !e->I.empty()) return fail(ECSG_EINVAL, id + "batched composite models use the energy-conserving interface (imode 1)");
             if (e->cfg.radius != 1) return fail(ECSG_EINVAL, id + "batched models are radius 1");
             if (!e->I.empty() && !e->lds_ok) return fail(ECSG_EINVAL, id + "interface too large for the LDS Jacobi of a batch");
@@ -3990,6 +4153,11 @@ int ecsg_step_n_timed(ecsg_ctx *ctx, int nsteps, double ms_out[2])
 }
 int ecsg_overlap_active(const ecsg_ctx *ctx) { return ctx && ctx->e ? ctx->e->overlap_active() : 0; }
 int ecsg_set_barrier_timeout(ecsg_ctx *ctx, int ms) { CTX_CHECK(ctx); return ret(ctx, ctx->e->set_barrier_timeout(ms)); }
+int ecsg_set_jacobi_tol(ecsg_ctx *ctx, double tol_cap, double tol_fac)
+{
+    CTX_CHECK(ctx);
+    return ret(ctx, ctx->e->set_jacobi_tol(tol_cap, tol_fac));
+}
 int ecsg_steps_done(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->steps_done() : -1; }
 int ecsg_synchronize(ecsg_ctx *ctx) { CTX_CHECK(ctx); return ret(ctx, ctx->e->sync()); }
 int ecsg_field_dims(const ecsg_ctx *ctx, int g, int sel, int len[3])

@@ -1349,6 +1349,122 @@ __global__ __launch_bounds__(256) void k_jacobi_fine(JacBatch<T> jb)
     if (J.sxz1s) J.sxz1s[qf] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Direct solve of the vx-sxz system (the reference solves it only by the
+// Jacobi sweeps of update_ghost_sxz.cpp:229-236, 314-350; the matrix is the
+// one those sweeps use, update_ghost_sxz.cpp:96-133 in ratio-r form).  The
+// unknowns of coarse row J (n0 = nx2-1 of them) form block J; the 9 bands
+// make the matrix block tridiagonal: diagonal blocks D_J (bands 1, 4, 7),
+// blocks L_J coupling row J to row J-1 (bands 0, 3, 6) and U_J coupling it
+// to row J+1 (bands 2, 5, 8), each tridiagonal in I.  The host forms the
+// Schur complements S_0 = D_0, S_J = D_J - L_J S_{J-1}^-1 U_{J-1} once per
+// parameter set and stores W_J = S_J^-1 (dense, column-major: entry (I, k)
+// at W + J*n0*n0 + k*n0 + I).  Per solve, one workgroup per interface:
+//   y_0 = b_0,             y_J = b_J - L_J (W_{J-1} y_{J-1}),
+//   x_{n1-1} = W y_{n1-1}, x_J = W_J (y_J - U_J x_{J+1}),
+// block elimination without pivoting between blocks, which the system's
+// symmetry and diagonal dominance make stable.
+template <typename T>
+struct SxzDirect {
+    const double *A;     // 9 bands, band k at A + k*nn, [J-1][I-1]
+    const double *W;     // inverse Schur complements
+    const double *B;     // right-hand side [J-1][I-1]
+    double *X0;          // solution out, (nx2+1)(ny2+1), zero border
+    double *Y;           // nn entries: the forward sweep's y
+    T *sxz2, *sxz2s;     // coarse ghosts sxz2(I, J, -1), optional second destination
+    long o2, P2, S22;
+    int nx2, ny2;
+};
+
+template <typename T>
+struct SxzDirectSet {
+    SxzDirect<T> it[MAXG - 1];
+};
+
+constexpr int JD = 1024;            // threads of the direct-solve workgroup
+constexpr int JD_MAXN0 = JD;        // longest coarse row (unknowns per block)
+
+// part[p][I] = sum over k of part p of W(I, k) v[k], W column-major n0 x n0;
+// thread (p, I) = (tid / n0, tid % n0), parts of KS consecutive k
+__device__ __forceinline__ void dense_mv_parts(const double *__restrict__ W, const double *v, double *part, int n0, int P,
+                                               int KS)
+{
+    const int tid = threadIdx.x;
+    const int p = tid / n0, I = tid - p * n0;
+    if (p < P) {
+        const int k0 = p * KS, k1 = min(n0, k0 + KS);
+        double acc = 0;
+        for (int k = k0; k < k1; k++) acc += W[(long)k * n0 + I] * v[k];
+        part[p * n0 + I] = acc;
+    }
+}
+
+// the parts of row I summed in order p = 0, 1, ...
+__device__ __forceinline__ double dense_mv_sum(const double *part, int n0, int P, int I)
+{
+    double s = part[I];
+    for (int p = 1; p < P; p++) s += part[p * n0 + I];
+    return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(JD) void k_sxz_direct(SxzDirectSet<T> set)
+{
+    __shared__ double va[JD_MAXN0], vb[JD_MAXN0], part[JD];
+    const SxzDirect<T> &D = set.it[blockIdx.x];
+    const int n0 = D.nx2 - 1, n1 = D.ny2 - 1;
+    const long nn = (long)n0 * n1, w = D.nx2 + 1, bs = (long)n0 * n0;
+    const int P = min(JD / n0, n0), KS = (n0 + P - 1) / P;
+    const int tid = threadIdx.x;
+    const double *__restrict__ A = D.A;
+    const double *__restrict__ B = D.B;
+    double *Y = D.Y;
+    // forward sweep; vb holds y_{J-1}, va the product W_{J-1} y_{J-1}
+    if (tid < n0) {
+        const double y = B[tid];
+        Y[tid] = y;
+        vb[tid] = y;
+    }
+    for (int J = 1; J < n1; J++) {
+        __syncthreads();
+        dense_mv_parts(D.W + (long)(J - 1) * bs, vb, part, n0, P, KS);
+        __syncthreads();
+        if (tid < n0) va[tid] = dense_mv_sum(part, n0, P, tid);
+        __syncthreads();
+        if (tid < n0) {
+            const long q = (long)J * n0 + tid;
+            double y = B[q] - A[3 * nn + q] * va[tid];
+            if (tid > 0) y -= A[q] * va[tid - 1];
+            if (tid + 1 < n0) y -= A[6 * nn + q] * va[tid + 1];
+            Y[q] = y;
+            vb[tid] = y;
+        }
+    }
+    // backward sweep; vb holds y_J - U_J x_{J+1}, va x_J
+    for (int J = n1 - 1; J >= 0; J--) {
+        __syncthreads();
+        dense_mv_parts(D.W + (long)J * bs, vb, part, n0, P, KS);
+        __syncthreads();
+        if (tid < n0) {
+            const double x = dense_mv_sum(part, n0, P, tid);
+            va[tid] = x;
+            const long c = (long)(J + 1) * w + tid + 1;
+            D.X0[c] = x;
+            const long idx = D.o2 - D.S22 + (long)(J + 1) * D.P2 + tid + 1;
+            D.sxz2[idx] = (T)x;
+            if (D.sxz2s) D.sxz2s[idx] = (T)x;
+        }
+        __syncthreads();
+        if (J > 0 && tid < n0) {
+            const long q = (long)(J - 1) * n0 + tid;
+            double t = Y[q] - A[5 * nn + q] * va[tid];
+            if (tid > 0) t -= A[2 * nn + q] * va[tid - 1];
+            if (tid + 1 < n0) t -= A[8 * nn + q] * va[tid + 1];
+            vb[tid] = t;
+        }
+    }
+}
+
 // vy-syz pair: pointwise coarse ghost and its fine children (derived ratio-r
 // form of update_ghost_syz.cpp:162-238).
 template <typename T>

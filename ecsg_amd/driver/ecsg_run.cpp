@@ -19,8 +19,12 @@
 //                       c23_<i> c13_<i> c12_<i> c44_<i> c55_<i> c66_<i>
 //                       (code convention c44 = 2*C44) instead of s11/s13
 //   vy_mirror_all=y     mirror vy across the y walls on every plane
-//   sxz_solver=0|1|2    vx-sxz interface system: Jacobi, Chebyshev-accelerated, or
-//                       Jacobi by one workgroup per interface (up to 9728 entries)
+//   sxz_solver=0|1|2|3  vx-sxz interface system: Jacobi, Chebyshev-accelerated,
+//                       Jacobi by one workgroup per interface (up to 9728
+//                       entries), or the direct block elimination
+//   jacobi_tol=<cap>,<fac>  exit tolerance min(cap, fac * AvgAbs(B)) of the
+//                       Jacobi sweeps (default: update_ghost_sxz.cpp:229-236's
+//                       constants of the storage precision)
 //   ii_vz_xend=y        imode=0: the working copy's form of the x-end vz ghost
 //                       edge (update_ghost_vz.cpp:98-107) instead of the
 //                       committed one the golden II traces were made with
@@ -323,6 +327,19 @@ int run(int argc, char **argv)
         for (int k = 0; k < 9; k++) sp[k] = aniso ? gp[i].st[k].data() : gp[i].st[k == 4 ? 4 : 0].data();
         C.chk(ecsg_set_params(C.h, i, gp[i].buoy.data(), sp, aniso ? 0 : 1), ("grid " + std::to_string(i + 1) + " parameters").c_str());
         gp[i] = GridParams();
+    }
+    if (P.has("jacobi_tol")) {
+        const std::string v = P.str("jacobi_tol", "");
+        const size_t comma = v.find(',');
+        if (comma == std::string::npos) throw Fail("jacobi_tol must be <cap>,<fac>");
+        double cap = 0, fac = 0;
+        try {
+            cap = std::stod(v.substr(0, comma));
+            fac = std::stod(v.substr(comma + 1));
+        } catch (...) {
+            throw Fail("jacobi_tol=" + v + ": not two numbers");
+        }
+        C.chk(ecsg_set_jacobi_tol(C.h, cap, fac), "jacobi_tol");
     }
     C.chk(ecsg_set_source(C.h, src_loc[0], src_loc[1], src_loc[2], src_loc[3], 0, src_trace.data(), (int)src_trace.size()), "source");
     C.chk(ecsg_set_receivers(C.h, n_rec, rec_loc.data()), "receivers");

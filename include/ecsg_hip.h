@@ -76,7 +76,14 @@ typedef struct {
                                            Chebyshev acceleration (same fixed point and exit test,
                                            fewer sweeps); 2 = the Jacobi sweeps of 0, one workgroup
                                            per interface up to 9728 entries (graph-capturable; for
-                                           many models stepped at once on their own streams) */
+                                           many models stepped at once on their own streams);
+                                           3 = a direct solve (no counterpart in the reference):
+                                           block elimination over the coarse rows with the inverse
+                                           Schur complements factored once per parameter set, one
+                                           workgroup per interface, graph-capturable; coarse rows of
+                                           at most 1024 unknowns and at most 512 MiB of factors per
+                                           interface.  The transposed solves of the gradient stay
+                                           Jacobi sweeps (to the adjoint tolerance). */
     int ii_vz_xend;                     /* imode 0 only: the x-end edge of the fine vz ghosts
                                            (update_ghost_vz.cpp:98-107).  0: left unwritten, as the
                                            committed version of that line (whose stores fall outside
@@ -300,6 +307,15 @@ int ecsg_overlap_active(const ecsg_ctx *ctx);
  * invalid.  0 makes the first barrier of every solve time out (a test of
  * the error path). */
 int ecsg_set_barrier_timeout(ecsg_ctx *ctx, int ms);
+
+/* Exit tolerance of the forward vx-sxz Jacobi sweeps (sxz_solver 0, 1, 2):
+ * the sweeps stop when the update norm is at most
+ * min(tol_cap, tol_fac * AvgAbs(B)), as update_ghost_sxz.cpp:229-236 forms it.
+ * The defaults are that line's constants of the storage precision: 1e-3 and
+ * 1e-4 for ECSG_F32, 1e-12 and 1e-5 for ECSG_F64.  The solve runs in double
+ * in both precisions, so an ECSG_F32 handle may take the double-precision
+ * pair (1e-12, 1e-5).  Members of one batch must share the tolerance. */
+int ecsg_set_jacobi_tol(ecsg_ctx *ctx, double tol_cap, double tol_fac);
 
 /* ecsg_step_n's velocity and stress updates in one fused pass (default off;
  * used when the energy check is off): the pass reads the state before the

@@ -93,6 +93,26 @@ def single(n=(8, 7, 6), h=10.0, nsteps=40, seed=0, iso=False, top_bc=1, src_len=
                  name="test_single")
 
 
+def sxz_sparse(A):
+    """The vx-sxz interface matrix as a sparse matrix from its 9 bands
+    A[k, J, I] (band k couples unknown (I, J) to (I + k//3 - 1, J + k%3 - 1);
+    unknowns numbered J * n0 + I)."""
+    import scipy.sparse as sp
+    _, n1, n0 = A.shape
+    N = n0 * n1
+    rows, cols, vals = [], [], []
+    for k in range(9):
+        dI, dJ = k // 3 - 1, k % 3 - 1
+        for j in range(n1):
+            for i in range(n0):
+                ii, jj = i + dI, j + dJ
+                if 0 <= ii < n0 and 0 <= jj < n1 and A[k, j, i] != 0:
+                    rows.append(j * n0 + i)
+                    cols.append(jj * n0 + ii)
+                    vals.append(A[k, j, i])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+
+
 def rel(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)

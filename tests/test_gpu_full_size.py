@@ -79,7 +79,19 @@ def test_full_size_overlap_bitwise(full_model):
     assert a[3] == b[3]
 
 
-def test_two_grid_config_fp32_against_fp64_oracle():
+@pytest.fixture(scope="module")
+def two_grid():
+    """The two-grid configuration and its FP64 oracle traces (CPU, OpenMP)."""
+    m = M.two_grid_workload()
+    o = Oracle(m, prec="f64", eci_mode=1, check_energy=False)
+    o.run(m.nsteps)
+    ref = o.traces()
+    o.close()
+    assert np.abs(ref).max() > 0
+    return m, ref
+
+
+def test_two_grid_config_fp32_against_fp64_oracle(two_grid):
     """The two-grid ratio-3 configuration at full size (241x241x61 h 5 m over
     81x81x121 h 15 m, smoothed +-5 % orthorhombic perturbation, Ricker 20 Hz
     at 30 m, 1500 steps): FP32 device traces against the FP64 oracle (CPU,
@@ -89,12 +101,7 @@ def test_two_grid_config_fp32_against_fp64_oracle():
     against 80 sweeps) -- its norm and the AvgAbs tolerance are sums taken in
     another order, and this step's norm lies within their rounding of the
     tolerance; the extra sweep moves the ghosts by less than the tolerance."""
-    m = M.two_grid_workload()
-    o = Oracle(m, prec="f64", eci_mode=1, check_energy=False)
-    o.run(m.nsteps)
-    ref = o.traces()
-    o.close()
-    assert np.abs(ref).max() > 0
+    m, ref = two_grid
     s = ecsg_amd.Simulation(m, precision="f32", check_energy=False)
     s.step(m.nsteps)
     s.sync()
@@ -108,6 +115,30 @@ def test_two_grid_config_fp32_against_fp64_oracle():
     s.close()
     print("two-grid 1500 steps: FP32 %.3e, FP64 %.3e relative L2 against the FP64 oracle" % (err, err64))
     assert err64 <= 1e-6, err64                 # measured: 8.7e-16
+
+
+def test_two_grid_config_direct_and_tight_jacobi(two_grid):
+    """The two-grid configuration's two interface modes against the FP64
+    oracle: the direct vx-sxz solve (FP32 and FP64) and, in FP32, the Jacobi
+    sweeps to tolerance 1e-12 (the double-precision pair (1e-12, 1e-5) of
+    update_ghost_sxz.cpp:234)."""
+    m, ref = two_grid
+    err = {}
+    sweeps = {}
+    for name, prec, kw in (("f32 direct", "f32", {"sxz_solver": "direct"}),
+                           ("f32 jacobi 1e-12", "f32", {"jacobi_tol": (1e-12, 1e-5)}),
+                           ("f64 direct", "f64", {"sxz_solver": "direct"})):
+        s = ecsg_amd.Simulation(m, precision=prec, check_energy=False, **kw)
+        s.step(m.nsteps)
+        s.sync()
+        err[name] = rel(s.traces(), ref)
+        sweeps[name] = s.jacobi_stats(0)[0]
+        s.close()
+    print("two-grid 1500 steps, relative L2 against the FP64 oracle: %s; sweeps %s" % (err, sweeps))
+    assert err["f32 direct"] <= 1e-4, err
+    assert err["f32 jacobi 1e-12"] <= 1e-4, err
+    assert err["f64 direct"] <= 1e-6, err
+    assert sweeps["f32 direct"] == 0 and sweeps["f32 jacobi 1e-12"] > m.nsteps
 
 
 def test_uniform_config_fp64_against_oracle():

@@ -9,11 +9,10 @@ from fractions import Fraction
 
 import numpy as np
 import pytest
-import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from ecsg_amd.model import Grid, Model, cfl_dt, ricker
-from helpers import composite, rel, single
+from helpers import composite, rel, single, sxz_sparse
 from oracle_lib import Oracle, fd_coeff
 
 
@@ -155,19 +154,7 @@ def test_sxz_system_symmetric_dominant_and_solved(r):
     o = Oracle(m, eci_mode=1)
     o.run(m.nsteps)
     A, B = o.sxz_system(0)
-    n1, n0 = B.shape
-    N = n0 * n1
-    rows, cols, vals = [], [], []
-    for k in range(9):
-        dI, dJ = k // 3 - 1, k % 3 - 1
-        for j in range(n1):
-            for i in range(n0):
-                ii, jj = i + dI, j + dJ
-                if 0 <= ii < n0 and 0 <= jj < n1 and A[k, j, i] != 0:
-                    rows.append(j * n0 + i)
-                    cols.append(jj * n0 + ii)
-                    vals.append(A[k, j, i])
-    M = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    M = sxz_sparse(A)
     assert abs(M - M.T).max() <= 1e-15 * abs(M).max()
     d = M.diagonal()
     off = np.asarray(abs(M).sum(axis=1)).ravel() - np.abs(d)
