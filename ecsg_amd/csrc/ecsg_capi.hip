@@ -348,7 +348,10 @@ public:
     long lds_maxw = 0;
     // fused velocity+stress pass on double-buffered fields
     int use_fused = 0;
-    int fby = 8;                 // rows per fused workgroup (4 or 8)
+    int fused_form = 4;          // rows per wavefront of k_fused_w (2, 4 or 8)
+    int fused_zc = 0;            // z planes per fused tile (0: the stencil tiling's)
+    int fused_lean = 1;          // interior tiles by k_fused_lean
+    static constexpr int FRL = sizeof(T) == 4 ? 4 : 2;  // rows per wavefront of k_fused_lean
     bool sets_dirty = false;     // field sets may differ in entries no kernel writes
     hipGraphExec_t gexec_f = nullptr;
     hipGraph_t graph_f = nullptr;
@@ -425,7 +428,9 @@ public:
         if (sizeof(T) == 8) { tol_cap = 1e-12; tol_fac = 1e-5; } else { tol_cap = 1e-3; tol_fac = 1e-4; }
         if (const char *e = getenv("ECSG_TILE_BY")) tile_by = atoi(e);
         if (const char *e = getenv("ECSG_TILE_ZC")) tile_zc = atoi(e);
-        if (const char *e = getenv("ECSG_FUSED_ROWS")) fby = atoi(e) == 4 ? 4 : 8;
+        if (const char *e = getenv("ECSG_FUSED_FORM")) fused_form = atoi(e) == 8 ? 8 : atoi(e) == 2 ? 2 : 4;
+        if (const char *e = getenv("ECSG_FUSED_ZC")) fused_zc = std::max(0, atoi(e));
+        if (const char *e = getenv("ECSG_FUSED_LEAN")) fused_lean = atoi(e) != 0;
         if (tile_by != 1 && tile_by != 2 && tile_by != 4 && tile_by != 8) return fail(ECSG_EINVAL, "tile height must be 1, 2, 4 or 8");
         if (tile_zc < 1) return fail(ECSG_EINVAL, "z chunk must be >= 1");
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -1097,10 +1102,24 @@ public:
         for (int f = 0; f < 9; f++) a.o[f] = o + (long)f * d.astride;
         a.defer0 = g > 0;
         a.defern = g + 1 < (int)G.size();
-        a.zc = d.dev.zc;
-        a.gx = (d.n[0] + 1 + FXO - 1) / FXO;
-        a.gy = (d.n[1] + 1 + fby - 1) / fby;
-        a.gz = (d.n[2] + 1 + a.zc - 1) / a.zc;
+        a.zc = fused_zc > 0 ? fused_zc : d.dev.zc;
+        {
+            const int nx = d.n[0], ny = d.n[1], nz = d.n[2];
+            a.gx = (nx + 1 + 63) / 64;
+            a.gy = (ny + 1 + 4 * fused_form - 1) / (4 * fused_form);
+            // k_fused_lean's planes [1, nz-2), columns [64, 64 (gxl+1)) and
+            // rows [4 FRL, 4 FRL (gyl+1)): every halo entry inside the walls
+            a.zlo = nz >= 4 ? 1 : 0;
+            a.zhi = nz >= 4 ? nz - 2 : 0;
+            a.nzc = (a.zhi - a.zlo + a.zc - 1) / a.zc;
+            a.gz = a.nzc + 2;
+            const int gxl = nx >= 129 ? (nx - 65) / 64 : 0;
+            const int gyl = std::max(0, (ny - 1) / (4 * FRL) - 1);
+            const bool lean = fused_lean && gxl > 0 && gyl > 0 && a.nzc > 0 &&
+                              (double)d.astride * sizeof(T) < 4294967296.0;
+            a.cx0 = 64; a.cx1 = lean ? 64 * (gxl + 1) : 64;
+            a.ry0 = 4 * FRL; a.ry1 = lean ? 4 * FRL * (gyl + 1) : 4 * FRL;
+        }
         a.src_p = -1;
         if (g == src_g && src_len > 0) {
             a.src_p = d.o + (long)src_loc[2] * d.S2 + (long)src_loc[1] * d.P + src_loc[0];
@@ -1110,12 +1129,6 @@ public:
         }
         a.step = d_step;
         return a;
-    }
-
-    template <int FB>
-    void fused_launch(const FusedDev<T> &a)
-    {
-        hipLaunchKernelGGL((k_fused<T, FB>), dim3(a.gx * a.gy * a.gz), dim3(64, FB + 2), 0, s, a);
     }
 
     int sync_sets()
@@ -1136,8 +1149,25 @@ public:
         for (int g = 0; g < ng; g++) {
             ev_begin();
             FusedDev<T> a = fused_dev(g);
-            if (fby == 4) fused_launch<4>(a);
-            else fused_launch<8>(a);
+            const unsigned nt = (unsigned)(a.gx * a.gy * a.gz);
+            if (a.cx1 > a.cx0) {
+                const DevGrid<T> &d = G[g];
+                FusedLean<T> L{};
+                L.A = a.g.f[0];
+                L.B = a.o[0];
+                L.Pm = d.base + 9 * d.astride;
+                L.as = (unsigned)d.astride; L.P = (unsigned)d.P; L.S2 = (unsigned)d.S2; L.o = (unsigned)d.o;
+                L.cx = d.dev.cx; L.cy = d.dev.cy; L.cz = d.dev.cz;
+                L.bx0 = 1; L.by0 = 1;
+                L.gxl = a.cx1 / 64 - 1; L.gyl = a.ry1 / (4 * FRL) - 1;
+                L.zlo = a.zlo; L.zhi = a.zhi; L.zc = a.zc; L.nzc = a.nzc;
+                L.src_p = a.src_p >= 0 ? (unsigned)a.src_p : 0xffffffffu;
+                L.src_type = a.src_type; L.src_len = a.src_len; L.src_tr = a.src_tr; L.step = a.step;
+                hipLaunchKernelGGL((k_fused_lean<T, FRL>), dim3((unsigned)(L.gxl * L.gyl * L.nzc)), dim3(256), 0, s, L);
+            }
+            if (fused_form == 2) hipLaunchKernelGGL((k_fused_w<T, 2>), dim3(nt), dim3(256), 0, s, a);
+            else if (fused_form == 8) hipLaunchKernelGGL((k_fused_w<T, 8>), dim3(nt), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_fused_w<T, 4>), dim3(nt), dim3(256), 0, s, a);
             ev_end(g == 0 ? 0 : 2);
         }
         HIPCHK(hipGetLastError());

@@ -38,6 +38,20 @@ def _model(case, nsteps):
         m = composite((3,), coarse=(6, 5, 4), nz_fine=(5,), nsteps=nsteps, seed=9)
         m.src_loc = (0, 0, 0)
         return m
+    # grids large enough for interior tiles (k_fused_lean: columns 64.., rows
+    # 4 x rows per wavefront.., planes 1 .. nz-3); the vx force sits where a
+    # lean tile's halo column (x = 128), halo row (y = 32) and z chunk start
+    # (z = 9 with 8-plane chunks) meet
+    if case == "big_single_vx":
+        m = composite((), coarse=(200, 70, 12), nz_fine=(), nsteps=nsteps, seed=10)
+        m.src_loc = (128, 32, 9)
+        return m
+    if case == "big_r3_explosive":   # explosive source inside the fine grid's interior tiles
+        m = composite((3,), coarse=(70, 30, 5), nz_fine=(12,), nsteps=nsteps, seed=11, src_type=1)
+        m.src_loc = (100, 40, 6)
+        return m
+    if case == "big_r2r3_rigid":
+        return composite((2, 3), coarse=(24, 12, 4), nz_fine=(11, 6), top_bc=0, nsteps=nsteps, seed=12)
     raise ValueError(case)
 
 
@@ -59,7 +73,7 @@ def _run(m, precision, fused, graph, nsteps, random_start=False):
 
 
 CASES = ["single_free", "single_rigid", "r2", "r3", "r3_rigid", "r2r3", "src_fine_plane", "src_coarse_plane",
-         "src_corner"]
+         "src_corner", "big_single_vx", "big_r3_explosive", "big_r2r3_rigid"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -76,7 +90,7 @@ def test_fused_pass_equals_separate_passes(case, precision):
     assert np.abs(ref[1]).max() > 0
 
 
-@pytest.mark.parametrize("case", ["r3", "r2r3", "single_rigid"])
+@pytest.mark.parametrize("case", ["r3", "r2r3", "single_rigid", "big_single_vx", "big_r2r3_rigid"])
 def test_fused_pass_from_random_state(case):
     """Random values in every entry (halos too) before stepping."""
     nsteps = 9
