@@ -97,6 +97,15 @@ static Layout grid_layout(const int n[3], long H, long elem_bytes)
 
 namespace {
 
+// every index a radius-1 stencil sweep of this grid forms (halo planes and
+// the one-entry neighbours included) has a byte offset below 2^32: the
+// sweeps then use 32-bit offsets (k_vel / k_stress with IT = unsigned)
+template <typename T>
+bool fits32(const ecsg::GridDev<T> &v)
+{
+    return (double)(v.o + (long)(v.nz + 3) * v.S2) * sizeof(T) < 4294967296.0;
+}
+
 // wavefield / stiffness positions (init_del_sim.cpp:163-173,
 // init_del_par_field.cpp:82-88 extended to the nine stiffnesses)
 const int SIM_POS[9][3] = {{0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 0, 0}, {1, 0, 0},
@@ -1303,18 +1312,26 @@ public:
     template <int BYv>
     void vel_launch(const GridDev<T> &v, double *ke, bool en, const int *srcp)
     {
-        if (en)
-            hipLaunchKernelGGL((k_vel<T, true, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, ke, srcp, d_src_old);
-        else
-            hipLaunchKernelGGL((k_vel<T, false, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, ke, srcp, d_src_old);
+        const dim3 gr(v.nblk), bl(BX, BYv);
+        if (fits32(v)) {
+            if (en) hipLaunchKernelGGL((k_vel<T, true, BYv, unsigned>), gr, bl, 0, s, v, ke, srcp, d_src_old);
+            else hipLaunchKernelGGL((k_vel<T, false, BYv, unsigned>), gr, bl, 0, s, v, ke, srcp, d_src_old);
+        } else {
+            if (en) hipLaunchKernelGGL((k_vel<T, true, BYv, long>), gr, bl, 0, s, v, ke, srcp, d_src_old);
+            else hipLaunchKernelGGL((k_vel<T, false, BYv, long>), gr, bl, 0, s, v, ke, srcp, d_src_old);
+        }
     }
     template <int BYv>
     void stress_launch(const GridDev<T> &v, double *se, bool en)
     {
-        if (en)
-            hipLaunchKernelGGL((k_stress<T, true, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, se);
-        else
-            hipLaunchKernelGGL((k_stress<T, false, BYv>), dim3(v.nblk), dim3(BX, BYv), 0, s, v, se);
+        const dim3 gr(v.nblk), bl(BX, BYv);
+        if (fits32(v)) {
+            if (en) hipLaunchKernelGGL((k_stress<T, true, BYv, unsigned>), gr, bl, 0, s, v, se);
+            else hipLaunchKernelGGL((k_stress<T, false, BYv, unsigned>), gr, bl, 0, s, v, se);
+        } else {
+            if (en) hipLaunchKernelGGL((k_stress<T, true, BYv, long>), gr, bl, 0, s, v, se);
+            else hipLaunchKernelGGL((k_stress<T, false, BYv, long>), gr, bl, 0, s, v, se);
+        }
     }
     // radius-1 stencil of grid g over the z planes [zb, ze) only (no energy)
     int stencil_part(int g, int zb, int ze, bool vel)
@@ -3160,6 +3177,7 @@ public:
     GridDev<T> *d_g = nullptr;
     int *d_gs = nullptr;
     int ng = 0, gtot = 0;
+    bool all32 = true;  // every member grid takes the 32-bit-offset stencil sweeps
     IfWork<T> *d_if = nullptr;
     int *d_is[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int ni = 0, itot[5] = {0, 0, 0, 0, 0};
@@ -3282,6 +3300,8 @@ public:
             }
         ng = (int)gs.size();
         gtot = acc;
+        all32 = true;
+        for (const auto &v : gs) all32 = all32 && fits32(v);
         if (int rc = upload(&d_g, gs)) return rc;
         if (int rc = upload(&d_gs, st)) return rc;
         drop_graph();
@@ -3291,10 +3311,14 @@ public:
     template <int BYv>
     void stencils(bool vel, const GridDev<T> *g)
     {
-        if (vel)
-            hipLaunchKernelGGL((k_vel_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, g, (const int *)d_gs, ng, gtot);
-        else
-            hipLaunchKernelGGL((k_stress_batch<T, BYv>), dim3(gtot), dim3(BX, BYv), 0, s, g, (const int *)d_gs, ng, gtot);
+        const dim3 gr(gtot), bl(BX, BYv);
+        if (all32) {
+            if (vel) hipLaunchKernelGGL((k_vel_batch<T, BYv, unsigned>), gr, bl, 0, s, g, (const int *)d_gs, ng, gtot);
+            else hipLaunchKernelGGL((k_stress_batch<T, BYv, unsigned>), gr, bl, 0, s, g, (const int *)d_gs, ng, gtot);
+        } else {
+            if (vel) hipLaunchKernelGGL((k_vel_batch<T, BYv, long>), gr, bl, 0, s, g, (const int *)d_gs, ng, gtot);
+            else hipLaunchKernelGGL((k_stress_batch<T, BYv, long>), gr, bl, 0, s, g, (const int *)d_gs, ng, gtot);
+        }
     }
     void launch_stencils(bool vel, const GridDev<T> *g = nullptr)
     {
