@@ -275,6 +275,7 @@ struct DevGrid {
     int iso = 0;
     double *ke = nullptr, *se = nullptr;  // energy partials, nblk each
     int by = 4;                           // workgroup height of the stencil tiles
+    int byv = 0;                          // of the overlapped step's velocity sweeps (0: by)
     // host copies (double of the T-rounded values) of the planes the
     // interface matrices need: buoyancy and c33 on iz = 0 and iz = nz
     std::vector<double> b_top, b_bot, c33_top, c33_bot;
@@ -436,11 +437,14 @@ public:
         if (c.max_steps < 1) return fail(ECSG_EINVAL, "max_steps must be >= 1");
         if (sizeof(T) == 8) { tol_cap = 1e-12; tol_fac = 1e-5; } else { tol_cap = 1e-3; tol_fac = 1e-4; }
         if (const char *e = getenv("ECSG_TILE_BY")) tile_by = atoi(e);
+        if (const char *e = getenv("ECSG_TILE_BYV")) tile_byv = atoi(e);
         if (const char *e = getenv("ECSG_TILE_ZC")) tile_zc = atoi(e);
         if (const char *e = getenv("ECSG_FUSED_FORM")) fused_form = atoi(e) == 8 ? 8 : atoi(e) == 2 ? 2 : 4;
         if (const char *e = getenv("ECSG_FUSED_ZC")) fused_zc = std::max(0, atoi(e));
         if (const char *e = getenv("ECSG_FUSED_LEAN")) fused_lean = atoi(e) != 0;
         if (tile_by != 1 && tile_by != 2 && tile_by != 4 && tile_by != 8) return fail(ECSG_EINVAL, "tile height must be 1, 2, 4 or 8");
+        if (tile_byv != 0 && tile_byv != 1 && tile_byv != 2 && tile_byv != 4 && tile_byv != 8)
+            return fail(ECSG_EINVAL, "velocity tile height must be 0, 1, 2, 4 or 8");
         if (tile_zc < 1) return fail(ECSG_EINVAL, "z chunk must be >= 1");
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         if (const char *e = getenv("ECSG_OVERLAP")) overlap_on = atoi(e) != 0 ? 1 : 0;
@@ -478,6 +482,7 @@ public:
             // vy y-wall mirror planes: [0, nz-1] as the reference loops, or all
             v.zy = c.vy_mirror_all ? d.n[2] : d.n[2] - 1;
             set_tiling(d, tile_by, tile_zc);
+            d.byv = tile_byv;
             if (c.radius > 1) {
                 setup_radius(d, c, top, bottom);
                 v.nblk = d.rblk;  // energy partials are per radius-R workgroup
@@ -699,7 +704,15 @@ public:
     }
 
     // stencil tiling: BX x by threads per workgroup, zc planes per thread
-    int tile_by = 4, tile_zc = ZCHUNK;
+    // The overlapped step's velocity sweeps use 64 x 2 workgroups
+    // (tile_byv).  On the three-grid workload (1207 x 1207 x 81 finest grid,
+    // FP32) the step takes 4.29 against 4.31 ms with 64 x 4: the interior
+    // velocity sweep then runs while the Jacobi launch beside it is still
+    // becoming resident (two-wavefront workgroups free room on two SIMDs of
+    // a CU at a time), 1.26 against 1.39 ms, and the solve's sweeps move
+    // beside the stress sweep, 2.26 against 2.17 ms.  64 x 4 stays best for
+    // the stress sweeps (2.27 ms with 64 x 2).
+    int tile_by = 4, tile_zc = ZCHUNK, tile_byv = 2;
     static void set_tiling(DevGrid<T> &d, int by, int zc)
     {
         GridDev<T> &v = d.dev;
@@ -1341,12 +1354,15 @@ public:
         GridDev<T> v = d.dev;
         v.zb = zb;
         v.ze = ze;
+        // the velocity sweep's own workgroup height (byv), if set
+        const int bv = vel && d.byv > 0 ? d.byv : d.by;
+        v.gy = (d.n[1] + 1 + bv - 1) / bv;
         v.nblk = v.gx * v.gy * ((ze - zb + v.zc - 1) / v.zc);
         // kinds 0 / 1: the interior launch of grid 0, the long one
         const bool main_part = g == 0 && zb == 0;
         ev_begin(live && main_part);
         if (vel) {
-            switch (d.by) {
+            switch (bv) {
             case 1: vel_launch<1>(v, d.ke, false, nullptr); break;
             case 2: vel_launch<2>(v, d.ke, false, nullptr); break;
             case 8: vel_launch<8>(v, d.ke, false, nullptr); break;

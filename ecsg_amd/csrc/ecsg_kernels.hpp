@@ -183,38 +183,45 @@ __device__ __forceinline__ void vel_tile(const GridDev<T> &g, const int tile, do
         T b0 = (*at(b, p));
         T b1 = (*at(b, p + 1));
         for (int iz = z0; iz < z1; ++iz, p += S2) {
+            // Every operand of the plane is loaded before any store (all of
+            // them lie inside the halo-padded arrays for ix <= nx, iy <= ny,
+            // iz <= nz), so the loads of a plane are in flight together
+            // instead of in three dependent groups behind the stores, which
+            // the compiler may not move loads across.
             const T sxz_c = (*at(sxz, p));
             const T syz_c = (*at(syz, p));
             const T szz_n = (*at(szz, p + S2));
             const T b0n = (*at(b, p + S2));
             const T b1n = (*at(b, p + S2 + 1));
+            const T vx_o = ldc(vx, p);
+            const T vy_o = ldc(vy, p);
+            const T vz_o = ldc(vz, p);
+            const T sxx_c = (*at(sxx, p)), sxx_w = (*at(sxx, p - 1));
+            const T sxy_c = (*at(sxy, p)), sxy_s = (*at(sxy, p - P)), sxy_e = (*at(sxy, p + 1));
+            const T syy_c = (*at(syy, p)), syy_n = (*at(syy, p + P));
+            const T sxz_e = (*at(sxz, p + 1));
+            const T syz_s = (*at(syz, p - P));
+            const T bP = (*at(b, p + P));
+            const T bP1 = (*at(b, p + P + 1));
             const bool zin = iz >= zlo && iz <= zhi;
             // vx, update_vel.cpp:30-37
-            const T vx_o = ldc(vx, p);
             T vx_c = vx_o;
             if (ok_x && zin) {
-                vx_c = vx_o + b0 * (cx * ((*at(sxx, p)) - (*at(sxx, p - 1))) + cy * ((*at(sxy, p)) - (*at(sxy, p - P))) + cz * (sxz_c - sxz_m));
+                vx_c = vx_o + b0 * (cx * (sxx_c - sxx_w) + cy * (sxy_c - sxy_s) + cz * (sxz_c - sxz_m));
                 stc(vx, p, vx_c);
             }
             // vy, update_vel.cpp:43-53
-            const T vy_o = ldc(vy, p);
             T vy_c = vy_o;
-            T bP = 0, bP1 = 0;
-            if (ok_y) {
-                bP = (*at(b, p + P));
-                bP1 = (*at(b, p + P + 1));
-                if (zin) {
-                    const T bav = T(0.25) * (b0 + b1 + bP + bP1);
-                    vy_c = vy_o + bav * (cx * ((*at(sxy, p + 1)) - (*at(sxy, p))) + cz * (syz_c - syz_m) + cy * ((*at(syy, p + P)) - (*at(syy, p))));
-                    stc(vy, p, vy_c);
-                }
+            if (ok_y && zin) {
+                const T bav = T(0.25) * (b0 + b1 + bP + bP1);
+                vy_c = vy_o + bav * (cx * (sxy_e - sxy_c) + cz * (syz_c - syz_m) + cy * (syy_n - syy_c));
+                stc(vy, p, vy_c);
             }
             // vz, update_vel.cpp:59-69
-            const T vz_o = ldc(vz, p);
             T vz_c = vz_o;
             if (ok_z && iz <= nz - 1) {
                 const T bav = T(0.25) * (b0 + b0n + b1 + b1n);
-                vz_c = vz_o + bav * (cx * ((*at(sxz, p + 1)) - sxz_c) + cz * (szz_n - szz_c) + cy * (syz_c - (*at(syz, p - P))));
+                vz_c = vz_o + bav * (cx * (sxz_e - sxz_c) + cz * (szz_n - szz_c) + cy * (syz_c - syz_s));
                 stc(vz, p, vz_c);
             }
             // ghost velocities for radius 1 (update_vel.cpp:75-246), same
@@ -362,15 +369,23 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
         T vx_c = (*at(vx, p));
         T vy_c = (*at(vy, p));
         for (int iz = z0; iz < z1; ++iz, p += S2) {
+            // Every operand of the plane is loaded before any store (all of
+            // them lie inside the halo-padded arrays, parameters included,
+            // for ix <= nx, iy <= ny, iz <= nz), so the plane's loads are in
+            // flight together instead of in four groups behind the stores.
             const T vx_n = (*at(vx, p + S2));
             const T vy_n = (*at(vy, p + S2));
             const T vz_c = (*at(vz, p));
+            const T sxx_o = ldc(sxx, p), syy_o = ldc(syy, p), szz_o = ldc(szz, p);
+            const T c11 = ldc(C11, p), c22 = ldc(C22, p), c33 = ldc(C33, p), c23 = ldc(C23, p), c13 = ldc(C13, p),
+                    c12 = ldc(C12, p);
+            const T vx_e = (*at(vx, p + 1)), vy_s = (*at(vy, p - P));
+            const T syz_o = ldc(syz, p), c44 = ldc(C44, p), vz_n = (*at(vz, p + P));
+            const T sxz_o = ldc(sxz, p), c55 = ldc(C55, p), vz_w = (*at(vz, p - 1));
+            const T sxy_o = ldc(sxy, p), c66 = ldc(C66, p), vx_nn = (*at(vx, p + P)), vy_w = (*at(vy, p - 1));
             if (ok_n) {
-                const T sxx_o = ldc(sxx, p), syy_o = ldc(syy, p), szz_o = ldc(szz, p);
-                const T c11 = ldc(C11, p), c22 = ldc(C22, p), c33 = ldc(C33, p), c23 = ldc(C23, p), c13 = ldc(C13, p),
-                        c12 = ldc(C12, p);
-                const T dvx = (*at(vx, p + 1)) - vx_c;
-                const T dvy = vy_c - (*at(vy, p - P));
+                const T dvx = vx_e - vx_c;
+                const T dvy = vy_c - vy_s;
                 if (g.free_top && iz == 0) {
                     // update_stress.cpp:37-65
                     const T a = sxx_o + ((c11 - c13 * c13 / c33) * cx * dvx + (c12 - c13 * c23 / c33) * cy * dvy);
@@ -400,9 +415,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
             }
             // syz, update_stress.cpp:103-109
             if (ok_yz && iz <= nz - 1) {
-                const T syz_o = ldc(syz, p);
-                const T c44 = ldc(C44, p);
-                const T syz_n = syz_o + T(0.5) * c44 * (cz * (vy_n - vy_c) + cy * ((*at(vz, p + P)) - vz_c));
+                const T syz_n = syz_o + T(0.5) * c44 * (cz * (vy_n - vy_c) + cy * (vz_n - vz_c));
                 stc(syz, p, syz_n);
                 if (g.free_top && iz == 0) (*at(syz, p - S2)) = -syz_n;  // update_stress.cpp:264-267
                 if (sv) {
@@ -413,9 +426,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
             }
             // sxz, update_stress.cpp:115-121
             if (iz <= nz - 1) {
-                const T sxz_o = ldc(sxz, p);
-                const T c55 = ldc(C55, p);
-                const T sxz_n = sxz_o + T(0.5) * c55 * (cz * (vx_n - vx_c) + cx * (vz_c - (*at(vz, p - 1))));
+                const T sxz_n = sxz_o + T(0.5) * c55 * (cz * (vx_n - vx_c) + cx * (vz_c - vz_w));
                 stc(sxz, p, sxz_n);
                 if (g.free_top && iz == 0 && ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1) {
                     (*at(sxz, p - S2)) = -sxz_n;  // update_stress.cpp:274-277
@@ -429,9 +440,7 @@ __device__ __forceinline__ void stress_tile(const GridDev<T> &g, const int tile,
             }
             // sxy, update_stress.cpp:127-133
             if (ok_xy) {
-                const T sxy_o = ldc(sxy, p);
-                const T c66 = ldc(C66, p);
-                const T sxy_n = sxy_o + T(0.5) * c66 * (cy * ((*at(vx, p + P)) - vx_c) + cx * (vy_c - (*at(vy, p - 1))));
+                const T sxy_n = sxy_o + T(0.5) * c66 * (cy * (vx_nn - vx_c) + cx * (vy_c - vy_w));
                 stc(sxy, p, sxy_n);
                 if (sv) stc(g.st[8], p, sxy_n);
                 if (EN) {

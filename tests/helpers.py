@@ -184,3 +184,31 @@ def sbp_residuals(F, n, r=1):
     B = (at(vz, x2, y2, z2) - at(vz, x2, y2 - 1, z2)) * at(syz, x2, y2, z2)
     two(A, B)
     return np.array(res), np.array(mag)
+
+
+HIP_ATTR_MULTIPROCESSOR_COUNT = 63   # hipDeviceAttributeMultiprocessorCount (hip_runtime_api.h, ROCm 7.2)
+
+
+def cu_count(device: int = 0) -> int:
+    """Compute units of a device, asked of the HIP runtime libecsg_hip.so
+    runs on: the libamdhip64 already mapped into this process (ROCm's, or
+    torch's bundled copy when torch was loaded first; a second runtime does
+    not start in a process where one already runs)."""
+    import ctypes
+    import ecsg_amd
+    ecsg_amd.load_library()
+    path = None
+    with open("/proc/self/maps") as fh:
+        for line in fh:
+            f = line.split()[-1]
+            if "libamdhip64.so" in f:
+                path = f
+                break
+    if path is None:
+        raise RuntimeError("libamdhip64 is not loaded")
+    hip = ctypes.CDLL(path)
+    v = ctypes.c_int(0)
+    rc = hip.hipDeviceGetAttribute(ctypes.byref(v), HIP_ATTR_MULTIPROCESSOR_COUNT, device)
+    if rc != 0 or v.value <= 0:
+        raise RuntimeError("hipDeviceGetAttribute(multiprocessor count) failed: %d" % rc)
+    return v.value

@@ -121,8 +121,8 @@ def test_barrier_timeout_is_reported_and_recovered():
 
 
 def _cus():
-    import torch
-    return torch.cuda.get_device_properties(0).multi_processor_count
+    from helpers import cu_count
+    return cu_count()
 
 
 def test_no_overlap_beside_a_launch_of_more_tiles_than_cus():
