@@ -88,15 +88,20 @@ __global__ __launch_bounds__(64) void k_adj_tri_batch(const AdjIfWork<T> *__rest
 enum { AG_STRESS, AG_GRAD, AG_VEL, AG_GRADB };
 
 template <typename T, int PH>
-__global__ __launch_bounds__(256) void k_adj_grid_batch(const AdjGridDev<T> *__restrict__ g, const int *__restrict__ start, int n,
+__global__ __launch_bounds__(64 * (AWY > GBY ? AWY : GBY)) void k_adj_grid_batch(const AdjGridDev<T> *__restrict__ g, const int *__restrict__ start, int n,
                                                         int total)
 {
     const int b = xcd_tile(blockIdx.x, total);
     const int m = batch_find(start, n, b);
-    if (PH == AG_STRESS) adj_stress_body(g[m], b - start[m]);
-    else if (PH == AG_GRAD) adj_grad_body(g[m], b - start[m]);
-    else if (PH == AG_VEL) adj_vel_body(g[m], b - start[m]);
-    else adj_gradb_body(g[m], b - start[m]);
+    if constexpr (PH == AG_STRESS || PH == AG_VEL) {
+        __shared__ T xs[AXS];
+        if constexpr (PH == AG_STRESS) adj_stress_body(g[m], b - start[m], xs);
+        else adj_vel_body(g[m], b - start[m], xs);
+    } else if constexpr (PH == AG_GRAD) {
+        adj_grad_body(g[m], b - start[m]);
+    } else {
+        adj_gradb_body(g[m], b - start[m]);
+    }
 }
 
 // transposed ghost mirrors of every grid: gathers, then the halo zeroing
