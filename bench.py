@@ -118,9 +118,9 @@ def adjoint_run(m, steps: int, every: int, fmt: str, precision: str = "f32"):
     sim.sync()
     el = time.perf_counter() - t
     sweeps = sim.adjoint_sweeps()
-    K = sim.checkpoint_interval
+    levels = sim.checkpoint_levels   # (interval, ring, recomputed steps)
     sim.close()
-    return el, J, sweeps, K
+    return el, J, sweeps, levels
 
 
 def batch_run(steps: int, solo_steps: int, warmup: int):
@@ -458,14 +458,15 @@ def main():
 
     adjoint = None
     if args.adjoint_steps > 0:
-        ea, Ja, sweeps, K = adjoint_run(m, args.adjoint_steps, args.ckpt_every, args.ckpt_format)
+        ea, Ja, sweeps, (K, ring, rec) = adjoint_run(m, args.adjoint_steps, args.ckpt_every, args.ckpt_format)
         ga = nodes * args.adjoint_steps / ea / 1e9
         ms_a = ea / args.adjoint_steps * 1e3
         adjoint = {"value": round(ga, 3), "unit": "Gpt/s", "steps": args.adjoint_steps,
-                   "what": "forward with checkpoints + recomputation of all segments but the last + adjoint step "
+                   "what": "forward with checkpoints + recomputation (one level: all segments but the last; two "
+                           "levels when the ring is shorter than the interval) + adjoint step "
                            "+ gradient of the 9 Cij and buoyancy, FP32, misfit 1/2 sum vx_rec^2",
-                   "checkpoint_every": K, "checkpoint_format": args.ckpt_format,
-                   "recomputed_steps": args.adjoint_steps - min(K, args.adjoint_steps),
+                   "checkpoint_every": K, "ring_steps": ring, "checkpoint_format": args.ckpt_format,
+                   "recomputed_steps": rec,
                    "ms_per_step": round(ms_a, 4),
                    "cost_vs_forward_f32": round(ms_a / ms_step, 3),
                    "cost_vs_forward_f64": round(ms_a / fp64["ms_per_step"], 3) if fp64 else None,
@@ -473,10 +474,10 @@ def main():
         log("[bench] fwd+adjoint: %.2f Gpt/s, %.3f ms/step (%.2fx the FP32 forward)" % (ga, ms_a, ms_a / ms_step))
         if args.adjoint_mem_steps > 0:
             # a run short enough for every state to stay in device memory
-            em, _, _, Km = adjoint_run(m, args.adjoint_mem_steps, 0, args.ckpt_format)
+            em, _, _, (Km, _, recm) = adjoint_run(m, args.adjoint_mem_steps, 0, args.ckpt_format)
             ms_m = em / args.adjoint_mem_steps * 1e3
             adjoint["in_memory"] = {"steps": args.adjoint_mem_steps, "checkpoint_every": Km,
-                                    "recomputed_steps": args.adjoint_mem_steps - min(Km, args.adjoint_mem_steps),
+                                    "recomputed_steps": recm,
                                     "ms_per_step": round(ms_m, 4), "cost_vs_forward_f32": round(ms_m / ms_step, 3),
                                     "cost_vs_forward_f64": round(ms_m / fp64["ms_per_step"], 3) if fp64 else None}
             log("[bench] fwd+adjoint in memory (%d steps): %.3f ms/step (%.2fx the FP32 forward)"

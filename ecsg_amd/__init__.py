@@ -39,7 +39,7 @@ ABI_SYMBOLS = (
     "ecsg_checkpoint_interval", "ecsg_copy_probe", "ecsg_stream_probe_floats", "ecsg_stream_probe",
     "ecsg_set_overlap", "ecsg_overlap_active", "ecsg_step_n_timed", "ecsg_set_barrier_timeout", "ecsg_set_jacobi_tol",
     "ecsg_set_params_staged", "ecsg_cfl_dt", "ecsg_cfl_dt_noalloc", "ecsg_batch_gradient",
-    "ecsg_batch_checkpoint_interval",
+    "ecsg_batch_checkpoint_interval", "ecsg_set_ring_steps", "ecsg_checkpoint_levels",
 )
 
 CKPT_FORMATS = {"native": 0, "fp16": 1, "bf16": 2}
@@ -133,6 +133,8 @@ def load_library(path: Optional[str] = None):
     L.ecsg_reset.argtypes = [vp]
     L.ecsg_set_checkpointing.argtypes = [vp, i, i]
     L.ecsg_checkpoint_interval.argtypes = [vp]
+    L.ecsg_set_ring_steps.argtypes = [vp, i]
+    L.ecsg_checkpoint_levels.argtypes = [vp, P(i), P(i), P(C.c_longlong)]
     L.ecsg_gradient.argtypes = [vp, i, P(d), P(d)]
     L.ecsg_get_gradient.argtypes = [vp, i, i, P(d)]
     L.ecsg_adjoint_step.argtypes = [vp]
@@ -390,6 +392,20 @@ class Simulation:
     def checkpoint_interval(self) -> int:
         """The interval the last gradient call used."""
         return int(self.L.ecsg_checkpoint_interval(self.h))
+
+    def set_ring_steps(self, steps: int = 0):
+        """Longest ring of saved states a gradient may hold (0: as long as
+        the interval, or as fits in device memory); a shorter ring than the
+        interval means two checkpoint levels."""
+        self._chk(self.L.ecsg_set_ring_steps(self.h, int(steps)))
+
+    @property
+    def checkpoint_levels(self):
+        """(interval, ring length, recomputed forward steps) of the last
+        gradient call."""
+        k1, k2, rec = C.c_int(), C.c_int(), C.c_longlong()
+        self._chk(self.L.ecsg_checkpoint_levels(self.h, C.byref(k1), C.byref(k2), C.byref(rec)))
+        return k1.value, k2.value, rec.value
 
     def gradient(self, nsteps: int, observed: np.ndarray) -> float:
         """Misfit 1/2 sum (traces - observed)^2 over nsteps steps from rest and

@@ -243,7 +243,9 @@ public:
     virtual int step_timed(int nsteps, double ms[2]) = 0;
     virtual int reset() = 0;
     virtual int set_checkpointing(int every, int format) = 0;
+    virtual int set_ring_steps(int k) = 0;
     virtual int checkpoint_interval() const = 0;
+    virtual void checkpoint_levels(int *k1, int *k2, long long *recomputed) const = 0;
     virtual int gradient(int nsteps, const double *obs, double *misfit) = 0;
     virtual int get_gradient(int g, int sel, double *out) = 0;
     virtual int adjoint_step() = 0;
@@ -2743,11 +2745,48 @@ public:
         return ECSG_OK;
     }
 
-    // Checkpoint interval for ck_every == 0: the largest K whose ring (K+1
-    // states) and checkpoints (one per segment but the last) fit in the free
-    // device memory, less a reserve.  Larger K means fewer recomputed steps:
-    // the last segment is never recomputed.
-    int auto_interval(int N, int *K_out)
+    // Largest ring (saved states for the transposed steps) a gradient may
+    // hold; 0: sized from free device memory.  An interval longer than the
+    // ring is processed in two levels (gradient).
+    int ring_steps = 0;
+    int set_ring_steps(int k) override
+    {
+        if (k < 0) return fail(ECSG_EINVAL, "ring steps must be >= 0 (0: sized from free device memory)");
+        ring_steps = k;
+        return ECSG_OK;
+    }
+
+    // Recomputed forward steps of a gradient over N steps with checkpoint
+    // interval K1 and ring K2 (<= K1): every segment but the last (one
+    // level), or every segment, then every sub-segment but each segment's
+    // last (two levels).  The first segment (sub-segment) takes the
+    // remainder.
+    static long long recomputed(int N, int K1, int K2)
+    {
+        const int nseg = (N + K1 - 1) / K1;
+        if (K2 >= K1) return (long long)N - std::min(K1, N);
+        long long r = 0;
+        for (int j = 0; j < nseg; j++) {
+            const int L = j == 0 ? N - (nseg - 1) * K1 : K1;
+            r += L + (L > K2 ? L - K2 : 0);
+        }
+        return r;
+    }
+    // checkpoints a gradient holds: one per segment but the last (one
+    // level); one per segment plus one per sub-segment but the last of the
+    // longest segment (two levels)
+    static int ck_count(int N, int K1, int K2)
+    {
+        const int nseg = (N + K1 - 1) / K1;
+        if (K2 >= K1) return std::max(nseg - 1, 1);
+        return nseg + (std::min(K1, N) + K2 - 1) / K2 - 1;
+    }
+
+    // Checkpoint interval K1 and ring K2 from the free device memory, less a
+    // reserve: one level (ring = interval, the largest that fits) when that
+    // fits, else the two-level pair with the fewest recomputed steps among
+    // those that fit.  k1_fixed > 0: that interval, the ring sized to it.
+    int auto_interval(int N, int k1_fixed, int *K1_out, int *K2_out)
     {
         size_t fr = 0, tot = 0;
         HIPCHK(hipMemGetInfo(&fr, &tot));
@@ -2760,16 +2799,49 @@ public:
         // what this handle already holds for the ring and checkpoints is reusable
         double avail = (double)fr + (double)ring_alloc * state + (double)ck_alloc * ckb;
         avail -= std::max(4.0e9, 0.05 * (double)tot);
-        int K = 1;
-        for (int k = N; k >= 1; k--) {
-            const int nseg = (N + k - 1) / k;
-            if ((k + 1) * state + std::max(nseg - 1, 1) * ckb <= avail) { K = k; break; }
+        auto fits = [&](int k1, int k2) { return (k2 + 1) * state + ck_count(N, k1, k2) * ckb <= avail; };
+        const int kmax = ring_steps > 0 ? std::min(ring_steps, N) : N;
+        if (k1_fixed > 0) {
+            const int k1 = std::min(k1_fixed, N);
+            int k2 = std::min(k1, kmax);
+            while (k2 > 1 && !fits(k1, k2)) k2--;
+            if (!fits(k1, k2))
+                return fail(ECSG_ENOMEM, "gradient: the checkpoints of interval " + std::to_string(k1) +
+                                             " and a ring of one state do not fit in device memory");
+            *K1_out = k1;
+            *K2_out = k2;
+            return ECSG_OK;
         }
-        *K_out = K;
+        for (int k = kmax; k >= 1; k--)
+            if (fits(k, k)) {
+                *K1_out = *K2_out = k;
+                return ECSG_OK;
+            }
+        long long best = -1;
+        int b1 = 0, b2 = 0;
+        for (int k2 = std::min(kmax, 4096); k2 >= 1; k2--) {
+            if ((k2 + 1) * state + 2 * ckb > avail) continue;
+            for (int m = 2; (long long)m * k2 <= (long long)2 * N; m++) {
+                const int k1 = std::min(m * k2, N);
+                if (k1 <= k2 || !fits(k1, k2)) continue;
+                const long long r = recomputed(N, k1, k2);
+                if (best < 0 || r < best) { best = r; b1 = k1; b2 = k2; }
+            }
+        }
+        if (best < 0) return fail(ECSG_ENOMEM, "gradient: no checkpoint interval fits in device memory");
+        *K1_out = b1;
+        *K2_out = b2;
         return ECSG_OK;
     }
-    int last_K = 0;
+    int last_K = 0, last_K2 = 0;
+    long long last_recomputed = 0;
     int checkpoint_interval() const override { return last_K; }
+    void checkpoint_levels(int *k1, int *k2, long long *rec) const override
+    {
+        if (k1) *k1 = last_K;
+        if (k2) *k2 = last_K2;
+        if (rec) *rec = last_recomputed;
+    }
 
     int ck_prepare(int nck, int K)
     {
@@ -2890,25 +2962,30 @@ public:
         if (nrec > 0 && !obs) return fail(ECSG_EINVAL, "gradient: null observed traces");
         if (int rc = adj_build()) return rc;
         if (int rc = build_adj_receivers()) return rc;
-        int K = ck_every;
-        if (K == 0)
-            if (int rc = auto_interval(N, &K)) return rc;
+        int K = 0, K2 = 0;
+        if (int rc = auto_interval(N, ck_every, &K, &K2)) return rc;
         K = std::min(K, N);
+        K2 = std::min(K2, K);
         last_K = K;
+        last_K2 = K2;
+        last_recomputed = recomputed(N, K, K2);
+        const bool two = K2 < K;
         const int nseg = (N + K - 1) / K;
         // segment j covers steps [seg0(j), seg0(j) + seglen(j)); the first
         // takes the remainder so that the last one, which is never
-        // recomputed, is a full K steps: N - K steps are recomputed
+        // recomputed in one-level mode, is a full K steps: N - K steps are
+        // recomputed
         const int r0 = N - (nseg - 1) * K;
         auto seg0 = [&](int j) { return j == 0 ? 0 : r0 + (j - 1) * K; };
         auto seglen = [&](int j) { return j == 0 ? r0 : K; };
-        if (int rc = ck_prepare(std::max(nseg - 1, 1), K)) return rc;
+        if (int rc = ck_prepare(ck_count(N, K, K2), K2)) return rc;
         if (int rc = reset()) return rc;
-        // forward pass with checkpoints; the last segment fills the ring
-        // directly (the states the backward pass needs), so it is never
-        // recomputed
+        // forward pass with checkpoints; in one-level mode the last segment
+        // fills the ring directly (the states the backward pass needs), so
+        // it is never recomputed; in two-level mode every segment has a
+        // checkpoint
         for (int j = 0; j < nseg; j++) {
-            const bool last = j == nseg - 1;
+            const bool last = j == nseg - 1 && !two;
             const int len = seglen(j);
             if (!last) {
                 if (int rc = ck_save(j)) return rc;
@@ -2942,26 +3019,66 @@ public:
         if (int rc = upload(&d_resid, res)) return rc;
         resid_steps = N;
         if (int rc = adj_zero()) return rc;
-        // backward over the segments
-        for (int j = nseg - 1; j >= 0; j--) {
-            const int len = seglen(j);
-            if (j < nseg - 1) {
-                if (int rc = ck_restore(j, seg0(j))) return rc;
-                if (int rc = ring_save(0)) return rc;
-                for (int t = 0; t < len; t++) {
-                    // the step's kernels also write every value into ring slot t+1
-                    set_store(t + 1);
-                    const int rc = enqueue_step(t + 1 < len);
-                    set_store(-1);
-                    if (rc) return rc;
-                }
+        // recompute `len` steps from the current state into ring slots
+        // 1..len (slot 0: the state before them), then the transposed steps
+        // over them, the last first
+        auto ring_pass = [&](int len) -> int {
+            if (int rc = ring_save(0)) return rc;
+            for (int t = 0; t < len; t++) {
+                // the step's kernels also write every value into ring slot t+1
+                set_store(t + 1);
+                const int rc = enqueue_step(t + 1 < len);
+                set_store(-1);
+                if (rc) return rc;
             }
+            return ECSG_OK;
+        };
+        auto adj_pass = [&](int step0, int len) -> int {
             for (int t = len - 1; t >= 0; t--)
-                if (int rc = adj_step_impl(seg0(j) + t, t + 1, t, true)) return rc;
+                if (int rc = adj_step_impl(step0 + t, t + 1, t, true)) return rc;
+            return ECSG_OK;
+        };
+        // backward over the segments
+        int left_at = N;  // the step the last recomputation ended at
+        for (int j = nseg - 1; j >= 0; j--) {
+            const int len = seglen(j), a = seg0(j);
+            if (!two) {
+                if (j < nseg - 1) {
+                    if (int rc = ck_restore(j, a)) return rc;
+                    if (int rc = ring_pass(len)) return rc;
+                    left_at = a + len;
+                }
+                if (int rc = adj_pass(a, len)) return rc;
+                continue;
+            }
+            // two levels: sub-segment i covers [a + sub0(i), + sublen(i)),
+            // the first taking the remainder; one pass over the segment
+            // leaves a checkpoint at every sub-segment but the last, which
+            // goes into the ring; each other sub-segment is recomputed from
+            // its checkpoint into the ring before its transposed steps
+            const int nsub = (len + K2 - 1) / K2, rs = len - (nsub - 1) * K2;
+            auto sub0 = [&](int i) { return i == 0 ? 0 : rs + (i - 1) * K2; };
+            auto sublen = [&](int i) { return i == 0 ? rs : K2; };
+            if (int rc = ck_restore(j, a)) return rc;
+            for (int i = 0; i + 1 < nsub; i++) {
+                if (int rc = ck_save(nseg + i)) return rc;
+                const int sl = sublen(i);
+                for (int t = 0; t < sl; t++)
+                    if (int rc = enqueue_step(t + 1 < sl)) return rc;
+            }
+            if (int rc = ring_pass(sublen(nsub - 1))) return rc;
+            if (int rc = adj_pass(a + sub0(nsub - 1), sublen(nsub - 1))) return rc;
+            left_at = a + len;
+            for (int i = nsub - 2; i >= 0; i--) {
+                if (int rc = ck_restore(nseg + i, a + sub0(i))) return rc;
+                if (int rc = ring_pass(sublen(i))) return rc;
+                if (int rc = adj_pass(a + sub0(i), sublen(i))) return rc;
+                left_at = a + sub0(i) + sublen(i);
+            }
         }
         HIPCHK(hipStreamSynchronize(s));
         if (int rc = check_barrier()) return rc;
-        host_step = nseg > 1 ? r0 : N;  // the state the last recomputation left
+        host_step = left_at;  // the state the last recomputation left
         for (int k = 0; k < (int)I.size(); k++) {
             long long st3[3];
             HIPCHK(hipMemcpy(st3, AI[k].stats, sizeof(st3), hipMemcpyDeviceToHost));
@@ -4257,6 +4374,13 @@ int ecsg_set_checkpointing(ecsg_ctx *ctx, int every, int format)
     return ret(ctx, ctx->e->set_checkpointing(every, format));
 }
 int ecsg_checkpoint_interval(const ecsg_ctx *ctx) { return (ctx && ctx->e) ? ctx->e->checkpoint_interval() : 0; }
+int ecsg_set_ring_steps(ecsg_ctx *ctx, int steps) { CTX_CHECK(ctx); return ret(ctx, ctx->e->set_ring_steps(steps)); }
+int ecsg_checkpoint_levels(const ecsg_ctx *ctx, int *interval, int *ring, long long *recomputed)
+{
+    if (!ctx || !ctx->e) return ECSG_EINVAL;
+    ctx->e->checkpoint_levels(interval, ring, recomputed);
+    return ECSG_OK;
+}
 int ecsg_gradient(ecsg_ctx *ctx, int nsteps, const double *observed, double *misfit)
 {
     CTX_CHECK(ctx);

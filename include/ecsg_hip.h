@@ -248,17 +248,34 @@ enum { ECSG_CKPT_NATIVE = 0, ECSG_CKPT_FP16 = 1, ECSG_CKPT_BF16 = 2 };
 /* Zero every wavefield, interface warm start, trace and the step counter. */
 int ecsg_reset(ecsg_ctx *ctx);
 
-/* Checkpoint interval (default 50; 0: the largest interval whose ring of
- * recomputed states and checkpoints fit in the free device memory) and
- * storage: native precision, or IEEE half / bfloat16 of each array divided
- * by its maximum magnitude.  With interval K over N steps the first
- * segment takes the remainder and the last one K steps; the last is never
- * recomputed (the forward pass leaves its states in the ring), so N - K
- * steps are. */
+/* Checkpoint interval (default 50; 0: from the free device memory, below)
+ * and storage: native precision, or IEEE half / bfloat16 of each array
+ * divided by its maximum magnitude.  With interval K over N steps the first
+ * segment takes the remainder and the last one K steps.  One level: a ring
+ * of K + 1 saved states; the last segment is never recomputed (the forward
+ * pass leaves its states in the ring), so N - K steps are.  Two levels, when
+ * the ring is shorter than the interval (ecsg_set_ring_steps, or the memory
+ * does not hold K + 1 states): the backward pass recomputes each segment
+ * once from its checkpoint, leaving checkpoints every R steps (R = ring
+ * length) and its last R steps in the ring, then recomputes each other
+ * sub-segment from its checkpoint into the ring.  Interval 0: the largest
+ * one-level interval that fits in the free device memory (less a reserve),
+ * else the two-level interval and ring that fit with the fewest recomputed
+ * steps.  Results do not depend on the interval or the ring length (native
+ * checkpoints: bit for bit). */
 int ecsg_set_checkpointing(ecsg_ctx *ctx, int every, int format);
 
 /* The checkpoint interval the last ecsg_gradient call used (0 before any). */
 int ecsg_checkpoint_interval(const ecsg_ctx *ctx);
+
+/* Longest ring of saved states a gradient may hold (0, the default: as long
+ * as the interval, or as fits in device memory).  No counterpart in the
+ * reference (its time loop has no adjoint). */
+int ecsg_set_ring_steps(ecsg_ctx *ctx, int steps);
+
+/* Checkpoint interval, ring length and recomputed forward steps of the last
+ * ecsg_gradient call (any pointer may be null). */
+int ecsg_checkpoint_levels(const ecsg_ctx *ctx, int *interval, int *ring, long long *recomputed);
 
 /* Runs the forward pass and the adjoint pass; misfit receives J.  After the
  * call the forward state is that of an intermediate step: ecsg_reset before

@@ -272,6 +272,77 @@ def test_checkpoint_interval_does_not_change_the_gradient():
             assert np.array_equal(a, b)
 
 
+def _recomputed(N, K1, K2):
+    """Forward steps the backward pass recomputes: one level, every segment
+    but the last; two levels, every segment once and every sub-segment but
+    each segment's last again (first segment / sub-segment: the remainder)."""
+    nseg = -(-N // K1)
+    lens = [N - (nseg - 1) * K1] + [K1] * (nseg - 1)
+    if K2 >= K1:
+        return N - min(K1, N)
+    return sum(L + (L - K2 if L > K2 else 0) for L in lens)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_two_level_checkpoints_equal_one_level(precision):
+    """A ring shorter than the checkpoint interval (two levels: sub-segment
+    checkpoints, each sub-segment recomputed into the ring) gives the
+    one-level gradient and misfit bit for bit with native checkpoints, and
+    reports its interval, ring and recomputed steps; with FP16 checkpoints it
+    stays within the FP16 checkpoints' own error of the native gradient."""
+    N = 37
+    m = composite((3,), coarse=(5, 4, 4), nz_fine=(5,), nsteps=N, seed=33)
+    obs = _observed(m, N, 3)
+    res = {}
+    for every, ring, fmt in ((10, 0, "native"), (10, 3, "native"), (37, 4, "native"), (7, 1, "native"),
+                             (10, 0, "fp16"), (10, 3, "fp16")):
+        s = ecsg_amd.Simulation(m, precision=precision, check_energy=False, max_steps=N)
+        s.set_checkpointing(every, fmt)
+        s.set_ring_steps(ring)
+        J = s.gradient(N, obs)
+        k1, k2, rec = s.checkpoint_levels
+        assert (k1, k2) == (every, ring if ring else every), (k1, k2)
+        assert rec == _recomputed(N, k1, k2), (every, ring, rec)
+        # the state after the call is the one the last recomputation left
+        res[(every, ring, fmt)] = (J, np.concatenate([s.grad(g, sel).ravel() for g in range(2) for sel in range(9, 19)]))
+        s.close()
+    J0, g0 = res[(10, 0, "native")]
+    assert J0 > 0
+    for key in ((10, 3, "native"), (37, 4, "native"), (7, 1, "native")):
+        J, g = res[key]
+        assert J == J0, key
+        assert np.array_equal(g, g0), key
+    e1 = rel(res[(10, 0, "fp16")][1], g0)
+    e2 = rel(res[(10, 3, "fp16")][1], g0)
+    print("%s fp16 checkpoints: one level %.3e, two levels %.3e" % (precision, e1, e2))
+    assert e2 <= max(5 * e1, 1e-6), (e1, e2)
+
+
+def test_interval_from_memory_with_a_ring_limit():
+    """Interval 0: the longest one-level interval the memory (and the ring
+    limit) allows -- here the whole run, or the ring limit (two levels are
+    chosen only when no one-level interval fits in device memory); results
+    equal bit for bit."""
+    N = 30
+    m = composite((3,), coarse=(5, 4, 4), nz_fine=(5,), nsteps=N, seed=35)
+    obs = _observed(m, N, 3)
+    out = []
+    for ring in (0, 4):
+        s = ecsg_amd.Simulation(m, precision="f64", check_energy=False, max_steps=N)
+        s.set_checkpointing(0, "native")
+        s.set_ring_steps(ring)
+        J = s.gradient(N, obs)
+        k1, k2, rec = s.checkpoint_levels
+        if ring == 0:
+            assert (k1, k2, rec) == (N, N, 0)        # everything fits: one segment
+        else:
+            assert (k1, k2, rec) == (4, 4, N - 4)
+        out.append((J, np.concatenate([s.grad(g, sel).ravel() for g in range(2) for sel in range(9, 19)])))
+        s.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("fmt,tol", [("fp16", 2e-2), ("bf16", 1e-1)])
 def test_reduced_precision_checkpoints(fmt, tol):
     m = composite((3,), coarse=(5, 4, 4), nz_fine=(5,), nsteps=60, seed=41)
