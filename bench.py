@@ -359,7 +359,7 @@ def main():
                     help="extra steps timing every kernel kind with events (log only)")
     ap.add_argument("--solver", default="jacobi", choices=["jacobi", "chebyshev"],
                     help="vx-sxz interface solver (jacobi = the reference's sweeps)")
-    ap.add_argument("--adjoint-steps", type=int, default=100,
+    ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="steps of the timed forward+adjoint gradient (0: skip)")
     ap.add_argument("--ckpt-every", type=int, default=0,
                     help="checkpoint interval of the adjoint leg (0: the largest that fits in device memory)")

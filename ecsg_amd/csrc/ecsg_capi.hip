@@ -706,15 +706,15 @@ public:
     }
 
     // stencil tiling: BX x by threads per workgroup, zc planes per thread
-    // The overlapped step's velocity sweeps use 64 x 2 workgroups
-    // (tile_byv).  On the three-grid workload (1207 x 1207 x 81 finest grid,
-    // FP32) the step takes 4.29 against 4.31 ms with 64 x 4: the interior
-    // velocity sweep then runs while the Jacobi launch beside it is still
-    // becoming resident (two-wavefront workgroups free room on two SIMDs of
-    // a CU at a time), 1.26 against 1.39 ms, and the solve's sweeps move
-    // beside the stress sweep, 2.26 against 2.17 ms.  64 x 4 stays best for
-    // the stress sweeps (2.27 ms with 64 x 2).
-    int tile_by = 4, tile_zc = ZCHUNK, tile_byv = 2;
+    // tile_byv (ECSG_TILE_BYV): the overlapped step's velocity sweeps in
+    // 64 x tile_byv workgroups (0: the stress sweeps' height).  On the
+    // three-grid workload (1207 x 1207 x 81 finest grid, FP32) 64 x 2 takes
+    // the step from 4.31 to 4.29 ms, but only by moving the Jacobi solve's
+    // work from the velocity sweep to the stress sweep: beside two-wavefront
+    // workgroups the grid-barrier launch becomes resident only as the
+    // velocity sweep ends (velocity sweep 1.39 -> 1.26 ms, stress sweep
+    // 2.17 -> 2.27 ms).  Off by default.
+    int tile_by = 4, tile_zc = ZCHUNK, tile_byv = 0;
     static void set_tiling(DevGrid<T> &d, int by, int zc)
     {
         GridDev<T> &v = d.dev;
