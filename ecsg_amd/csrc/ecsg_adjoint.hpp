@@ -23,10 +23,10 @@ namespace ecsg {
 
 // f[q] += v (adjoint fields; plain loads and stores: the stencils' cache
 // hints made these sweeps slower)
-template <typename T>
-__device__ __forceinline__ void acc_st(T *f, long q, T v)
+template <typename T, typename IT>
+__device__ __forceinline__ void acc_st(T *f, IT q, T v)
 {
-    f[q] += v;
+    *at(f, q) += v;
 }
 // The gradient-only sweeps' read-modify-writes of their accumulators
 // (ECSG_GRAD_NT: with the cache hints) as a load (gld) and a store (gst)
@@ -36,22 +36,22 @@ __device__ __forceinline__ void acc_st(T *f, long q, T v)
 #ifndef ECSG_GRAD_NT
 #define ECSG_GRAD_NT 1
 #endif
-template <typename T>
-__device__ __forceinline__ T gld(const T *f, long q)
+template <typename T, typename IT>
+__device__ __forceinline__ T gld(const T *f, IT q)
 {
 #if ECSG_GRAD_NT
     return ldc(f, q);
 #else
-    return f[q];
+    return *at(f, q);
 #endif
 }
-template <typename T>
-__device__ __forceinline__ void gst(T *f, long q, T v)
+template <typename T, typename IT>
+__device__ __forceinline__ void gst(T *f, IT q, T v)
 {
 #if ECSG_GRAD_NT
     stc(f, q, v);
 #else
-    f[q] = v;
+    *at(f, q) = v;
 #endif
 }
 
@@ -137,7 +137,7 @@ __device__ __forceinline__ T shfl_upw(T v) { return __shfl_up(v, 1, 64); }
 // re-read the y-neighbour rows: 1.41x the sweep's algorithmic bytes.
 constexpr int AXS = 2 * 3 * AWY * 64;  // LDS entries of the exchange
 
-template <typename T>
+template <typename T, typename IT>
 __device__ __forceinline__ void adj_stress_body(const AdjGridDev<T> &a, const int tile, T *xs)
 {
     const int bx = tile % a.gx;
@@ -154,26 +154,26 @@ __device__ __forceinline__ void adj_stress_body(const AdjGridDev<T> &a, const in
     const bool out = row && lane >= 1 && lane <= AXO && ix <= nx;
     const int z0 = bz * AZ2 - 1;
     const int z1 = min(z0 + AZ2, nz + 1);
-    const long P = a.P, S2 = a.S2;
+    const IT P = (IT)a.P, S2 = (IT)a.S2;
     const T cx = a.cx, cy = a.cy, cz = a.cz;
     const T *a11 = a.ac[0], *a22 = a.ac[1], *a33 = a.ac[2], *a23 = a.ac[3], *a13 = a.ac[4], *a12 = a.ac[5];
     const T *h44 = a.ac[6], *h55 = a.ac[7], *h66 = a.ac[8];
     const T *lxx = a.lam[3], *lyy = a.lam[4], *lzz = a.lam[5], *lyz = a.lam[6], *lxz = a.lam[7], *lxy = a.lam[8];
     const bool ym_ok = iy - 1 >= -1;
-    const long qb = a.o + (long)iy * P + ix;
+    const IT qb = (IT)a.o + (IT)iy * P + (IT)ix;
     // adjoint weights of this thread's point at plane z (zero outside the array box)
     auto own = [&](int z, T &ex, T &ey, T &ez, T &hyz, T &hxz, T &hxy) {
         ex = ey = ez = hyz = hxz = hxy = T(0);
         if (!inx || z < -1 || z > nz + 1) return;
-        const long q = qb + (long)z * S2;
-        const T l1 = lxx[q], l2 = lyy[q], l3 = lzz[q];
-        const T A11 = a11[q], A22 = a22[q], A33 = a33[q], A23 = a23[q], A13 = a13[q], A12 = a12[q];
+        const IT q = qb + (IT)z * S2;
+        const T l1 = (*at(lxx, q)), l2 = (*at(lyy, q)), l3 = (*at(lzz, q));
+        const T A11 = (*at(a11, q)), A22 = (*at(a22, q)), A33 = (*at(a33, q)), A23 = (*at(a23, q)), A13 = (*at(a13, q)), A12 = (*at(a12, q));
         ex = cx * (A11 * l1 + A12 * l2 + A13 * l3);
         ey = cy * (A12 * l1 + A22 * l2 + A23 * l3);
         ez = cz * (A13 * l1 + A23 * l2 + A33 * l3);
-        hyz = h44[q] * lyz[q];
-        hxz = h55[q] * lxz[q];
-        hxy = h66[q] * lxy[q];
+        hyz = (*at(h44, q)) * (*at(lyz, q));
+        hxz = (*at(h55, q)) * (*at(lxz, q));
+        hxy = (*at(h66, q)) * (*at(lxy, q));
     };
     // the neighbouring rows' weights at plane z
     int buf = 0;
@@ -189,16 +189,16 @@ __device__ __forceinline__ void adj_stress_body(const AdjGridDev<T> &a, const in
         if (ty + 1 < AWY) {
             eyp = e[(ty + 1) * 64 + lane];
         } else if (zok) {
-            const long qp = qb + (long)z * S2 + P;
-            eyp = cy * (a12[qp] * lxx[qp] + a22[qp] * lyy[qp] + a23[qp] * lzz[qp]);
+            const IT qp = qb + (IT)z * S2 + P;
+            eyp = cy * ((*at(a12, qp)) * (*at(lxx, qp)) + (*at(a22, qp)) * (*at(lyy, qp)) + (*at(a23, qp)) * (*at(lzz, qp)));
         }
         if (ty > 0) {
             hxym = hy[(ty - 1) * 64 + lane];
             hyzm = hz[(ty - 1) * 64 + lane];
         } else if (zok && ym_ok) {
-            const long qm = qb + (long)z * S2 - P;
-            hxym = h66[qm] * lxy[qm];
-            hyzm = h44[qm] * lyz[qm];
+            const IT qm = qb + (IT)z * S2 - P;
+            hxym = (*at(h66, qm)) * (*at(lxy, qm));
+            hyzm = (*at(h44, qm)) * (*at(lyz, qm));
         }
     };
     T ex, ey, ez, hyz, hxz, hxy, eyp, hxym, hyzm;
@@ -207,7 +207,7 @@ __device__ __forceinline__ void adj_stress_body(const AdjGridDev<T> &a, const in
     own(z0, ex, ey, ez, hyz, hxz, hxy);
     nbr(z0, ey, hyz, hxy, eyp, hxym, hyzm);
     for (int iz = z0; iz < z1; iz++) {
-        const long q = qb + (long)iz * S2;
+        const IT q = qb + (IT)iz * S2;
         T ex_n, ey_n, ez_n, hyz_n, hxz_n, hxy_n, eyp_n, hxym_n, hyzm_n;
         own(iz + 1, ex_n, ey_n, ez_n, hyz_n, hxz_n, hxy_n);
         nbr(iz + 1, ey_n, hyz_n, hxy_n, eyp_n, hxym_n, hyzm_n);
@@ -223,11 +223,11 @@ __device__ __forceinline__ void adj_stress_body(const AdjGridDev<T> &a, const in
         eyp = eyp_n; hxym = hxym_n; hyzm = hyzm_n;
     }
 }
-template <typename T>
+template <typename T, typename IT>
 __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
 {
     __shared__ T xs[AXS];
-    adj_stress_body(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz), xs);
+    adj_stress_body<T, IT>(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz), xs);
 }
 
 
@@ -247,7 +247,7 @@ __host__ __device__ __forceinline__ int adj_grad_tiles(const AdjGridDev<T> &a)
     return ((a.nx + 64) / 64) * ((a.ny + GBY) / GBY) * ((a.nz + GZC) / GZC);
 }
 
-template <typename T>
+template <typename T, typename IT>
 __device__ __forceinline__ void adj_grad_body(const AdjGridDev<T> &a, const int tile)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -256,12 +256,12 @@ __device__ __forceinline__ void adj_grad_body(const AdjGridDev<T> &a, const int 
     const int ix = bx * 64 + threadIdx.x, iy = by * GBY + threadIdx.y;
     if (ix > nx || iy > ny) return;
     const int z0 = bz * GZC, z1 = min(z0 + GZC, nz + 1);
-    const long P = a.P, S2 = a.S2;
+    const IT P = (IT)a.P, S2 = (IT)a.S2;
     const T cx = a.cx, cy = a.cy, cz = a.cz;
     const T *lxx = a.lam[3], *lyy = a.lam[4], *lzz = a.lam[5], *lyz = a.lam[6], *lxz = a.lam[7], *lxy = a.lam[8];
     const T *fvx = a.fv[0], *fvy = a.fv[1], *fvz = a.fv[2];
-    long q = a.o + (long)z0 * S2 + (long)iy * P + ix;
-    T fvz_m = fvz[q - S2], fvx_c = fvx[q], fvy_c = fvy[q];
+    IT q = (IT)a.o + (IT)z0 * S2 + (IT)iy * P + (IT)ix;
+    T fvz_m = (*at(fvz, q - S2)), fvx_c = (*at(fvx, q)), fvy_c = (*at(fvy, q));
     T *const *gc = a.gc;
     for (int iz = z0; iz < z1; iz++, q += S2) {
         // Every operand of the plane, the nine accumulators included, is
@@ -269,10 +269,10 @@ __device__ __forceinline__ void adj_grad_body(const AdjGridDev<T> &a, const int 
         // halo-padded arrays for ix <= nx, iy <= ny, iz <= nz): one group of
         // loads in flight per plane instead of one round trip per
         // read-modify-write (1207 x 1207 x 81 grid, FP32: 2.93 -> 2.53 ms).
-        const T fvx_n = fvx[q + S2], fvy_n = fvy[q + S2], fvz_c = fvz[q];
-        const T l1 = lxx[q], l2 = lyy[q], l3 = lzz[q], l4 = lyz[q], l5 = lxz[q], l6 = lxy[q];
-        const T fvx_e = fvx[q + 1], fvy_s = fvy[q - P], fvz_n = fvz[q + P], fvz_w = fvz[q - 1];
-        const T fvx_nn = fvx[q + P], fvy_w = fvy[q - 1];
+        const T fvx_n = (*at(fvx, q + S2)), fvy_n = (*at(fvy, q + S2)), fvz_c = (*at(fvz, q));
+        const T l1 = (*at(lxx, q)), l2 = (*at(lyy, q)), l3 = (*at(lzz, q)), l4 = (*at(lyz, q)), l5 = (*at(lxz, q)), l6 = (*at(lxy, q));
+        const T fvx_e = (*at(fvx, q + 1)), fvy_s = (*at(fvy, q - P)), fvz_n = (*at(fvz, q + P)), fvz_w = (*at(fvz, q - 1));
+        const T fvx_nn = (*at(fvx, q + P)), fvy_w = (*at(fvy, q - 1));
         T g[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) g[k] = gld(gc[k], q);
@@ -280,7 +280,7 @@ __device__ __forceinline__ void adj_grad_body(const AdjGridDev<T> &a, const int 
             const T X = cx * (fvx_e - fvx_c);
             const T Y = cy * (fvy_c - fvy_s);
             if (a.free_top && iz == 0) {
-                const T c33 = a.c[2][q], c23 = a.c[3][q], c13 = a.c[4][q];
+                const T c33 = (*at(a.c[2], q)), c23 = (*at(a.c[3], q)), c13 = (*at(a.c[4], q));
                 const T u = c13 / c33, v = c23 / c33;
                 gst(gc[0], q, g[0] + l1 * X);
                 gst(gc[1], q, g[1] + l2 * Y);
@@ -309,10 +309,10 @@ __device__ __forceinline__ void adj_grad_body(const AdjGridDev<T> &a, const int 
         fvy_c = fvy_n;
     }
 }
-template <typename T>
+template <typename T, typename IT>
 __global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
 {
-    adj_grad_body(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
+    adj_grad_body<T, IT>(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
 }
 
 
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
 //   vy += mean_xy(b) (cx dx sxy + cz dz syz + cy dy syy)       on the vy update set
 //   vz += mean_xz(b) (cx dx sxz + cz dz szz + cy dy syz)       on the vz update set
 // with the adjoint weights w = (masked buoyancy) * lam_v.
-template <typename T>
+template <typename T, typename IT>
 __device__ __forceinline__ void adj_vel_body(const AdjGridDev<T> &a, const int tile, T *xs)
 {
     const int bx = tile % a.gx;
@@ -340,19 +340,19 @@ __device__ __forceinline__ void adj_vel_body(const AdjGridDev<T> &a, const int t
     const bool out = row && lane >= 1 && lane <= AXO && ix <= nx;
     const int z0 = bz * AZ2 - 1;
     const int z1 = min(z0 + AZ2, nz + 1);
-    const long P = a.P, S2 = a.S2;
+    const IT P = (IT)a.P, S2 = (IT)a.S2;
     const T cx = a.cx, cy = a.cy, cz = a.cz;
     const T *mbx = a.ac[9], *mby = a.ac[10], *mbz = a.ac[11];
     const T *lvx = a.lam[0], *lvy = a.lam[1], *lvz = a.lam[2];
     const bool ym_ok = iy - 1 >= -1;
-    const long qb = a.o + (long)iy * P + ix;
+    const IT qb = (IT)a.o + (IT)iy * P + (IT)ix;
     auto own = [&](int z, T &wx, T &wy, T &wz) {
         wx = wy = wz = T(0);
         if (!inx || z < -1 || z > nz + 1) return;
-        const long q = qb + (long)z * S2;
-        wx = mbx[q] * lvx[q];
-        wy = mby[q] * lvy[q];
-        wz = mbz[q] * lvz[q];
+        const IT q = qb + (IT)z * S2;
+        wx = (*at(mbx, q)) * (*at(lvx, q));
+        wy = (*at(mby, q)) * (*at(lvy, q));
+        wz = (*at(mbz, q)) * (*at(lvz, q));
     };
     // the neighbouring rows' weights (wx, wz at iy+1; wy at iy-1) through
     // LDS, as in adj_stress_body
@@ -370,14 +370,14 @@ __device__ __forceinline__ void adj_vel_body(const AdjGridDev<T> &a, const int t
             wxp = ex[(ty + 1) * 64 + lane];
             wzp = ez[(ty + 1) * 64 + lane];
         } else if (zok) {
-            const long qp = qb + (long)z * S2 + P;
-            wxp = mbx[qp] * lvx[qp];
-            wzp = mbz[qp] * lvz[qp];
+            const IT qp = qb + (IT)z * S2 + P;
+            wxp = (*at(mbx, qp)) * (*at(lvx, qp));
+            wzp = (*at(mbz, qp)) * (*at(lvz, qp));
         }
         if (ty > 0) wym = ey[(ty - 1) * 64 + lane];
         else if (zok && ym_ok) {
-            const long qm = qb + (long)z * S2 - P;
-            wym = mby[qm] * lvy[qm];
+            const IT qm = qb + (IT)z * S2 - P;
+            wym = (*at(mby, qm)) * (*at(lvy, qm));
         }
     };
     T wx, wy, wz, wxp, wzp, wym, d0, d1, wz_m;
@@ -385,7 +385,7 @@ __device__ __forceinline__ void adj_vel_body(const AdjGridDev<T> &a, const int t
     own(z0, wx, wy, wz);
     nbr(z0, wx, wy, wz, wxp, wzp, wym);
     for (int iz = z0; iz < z1; iz++) {
-        const long q = qb + (long)iz * S2;
+        const IT q = qb + (IT)iz * S2;
         T wx_n, wy_n, wz_n, wxp_n, wzp_n, wym_n;
         own(iz + 1, wx_n, wy_n, wz_n);
         nbr(iz + 1, wx_n, wy_n, wz_n, wxp_n, wzp_n, wym_n);
@@ -407,11 +407,11 @@ __device__ __forceinline__ void adj_vel_body(const AdjGridDev<T> &a, const int t
         wx = wx_n; wy = wy_n; wz = wz_n; wxp = wxp_n; wzp = wzp_n; wym = wym_n;
     }
 }
-template <typename T>
+template <typename T, typename IT>
 __global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
 {
     __shared__ T xs[AXS];
-    adj_vel_body(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz), xs);
+    adj_vel_body<T, IT>(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz), xs);
 }
 
 
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
 // xz mean): lam_v (after the transposed velocity mirrors) times the stress
 // differences of this step's forward stresses s^n.  Own sweep, like
 // k_adj_grad; lam_v is not written by k_adj_vel.
-template <typename T>
+template <typename T, typename IT>
 __device__ __forceinline__ void adj_gradb_body(const AdjGridDev<T> &a, const int tile)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -428,18 +428,18 @@ __device__ __forceinline__ void adj_gradb_body(const AdjGridDev<T> &a, const int
     const int ix = bx * 64 + threadIdx.x, iy = by * GBY + threadIdx.y;
     if (ix > nx || iy > ny) return;
     const int z0 = bz * GZC, z1 = min(z0 + GZC, nz + 1);
-    const long P = a.P, S2 = a.S2;
+    const IT P = (IT)a.P, S2 = (IT)a.S2;
     const T cx = a.cx, cy = a.cy, cz = a.cz;
     const T *lvx = a.lam[0], *lvy = a.lam[1], *lvz = a.lam[2];
     const T *sxx = a.fs[0], *syy = a.fs[1], *szz = a.fs[2], *syz = a.fs[3], *sxz = a.fs[4], *sxy = a.fs[5];
-    long q = a.o + (long)z0 * S2 + (long)iy * P + ix;
-    T sxz_m = sxz[q - S2], syz_m = syz[q - S2], szz_c = szz[q];
+    IT q = (IT)a.o + (IT)z0 * S2 + (IT)iy * P + (IT)ix;
+    T sxz_m = (*at(sxz, q - S2)), syz_m = (*at(syz, q - S2)), szz_c = (*at(szz, q));
     for (int iz = z0; iz < z1; iz++, q += S2) {
         // all loads of the plane before the accumulators are stored
-        const T sxz_c = sxz[q], syz_c = syz[q], szz_n = szz[q + S2];
-        const T l1 = lvx[q], l2 = lvy[q], l3 = lvz[q];
-        const T sxx_c = sxx[q], sxx_w = sxx[q - 1], sxy_c = sxy[q], sxy_s = sxy[q - P], sxy_e = sxy[q + 1];
-        const T syy_c = syy[q], syy_n = syy[q + P], sxz_e = sxz[q + 1], syz_s = syz[q - P];
+        const T sxz_c = (*at(sxz, q)), syz_c = (*at(syz, q)), szz_n = (*at(szz, q + S2));
+        const T l1 = (*at(lvx, q)), l2 = (*at(lvy, q)), l3 = (*at(lvz, q));
+        const T sxx_c = (*at(sxx, q)), sxx_w = (*at(sxx, q - 1)), sxy_c = (*at(sxy, q)), sxy_s = (*at(sxy, q - P)), sxy_e = (*at(sxy, q + 1));
+        const T syy_c = (*at(syy, q)), syy_n = (*at(syy, q + P)), sxz_e = (*at(sxz, q + 1)), syz_s = (*at(syz, q - P));
         const T g0 = gld(a.gb[0], q), g1 = gld(a.gb[1], q), g2 = gld(a.gb[2], q);
         const bool zin = iz >= a.top_eff && iz <= nz - a.bottom;
         if (ix >= 1 && ix <= nx - 1 && iy >= 1 && iy <= ny - 1 && zin)
@@ -453,10 +453,10 @@ __device__ __forceinline__ void adj_gradb_body(const AdjGridDev<T> &a, const int
         szz_c = szz_n;
     }
 }
-template <typename T>
+template <typename T, typename IT>
 __global__ __launch_bounds__(64 * GBY) void k_adj_gradb(AdjGridDev<T> a)
 {
-    adj_gradb_body(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
+    adj_gradb_body<T, IT>(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
 }
 
 

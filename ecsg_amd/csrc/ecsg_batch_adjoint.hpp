@@ -87,7 +87,7 @@ __global__ __launch_bounds__(64) void k_adj_tri_batch(const AdjIfWork<T> *__rest
 // adjoint stencils and gradient sweeps over the tiles of every grid
 enum { AG_STRESS, AG_GRAD, AG_VEL, AG_GRADB };
 
-template <typename T, int PH>
+template <typename T, int PH, typename IT>
 __global__ __launch_bounds__(64 * (AWY > GBY ? AWY : GBY)) void k_adj_grid_batch(const AdjGridDev<T> *__restrict__ g, const int *__restrict__ start, int n,
                                                         int total)
 {
@@ -95,12 +95,12 @@ __global__ __launch_bounds__(64 * (AWY > GBY ? AWY : GBY)) void k_adj_grid_batch
     const int m = batch_find(start, n, b);
     if constexpr (PH == AG_STRESS || PH == AG_VEL) {
         __shared__ T xs[AXS];
-        if constexpr (PH == AG_STRESS) adj_stress_body(g[m], b - start[m], xs);
-        else adj_vel_body(g[m], b - start[m], xs);
+        if constexpr (PH == AG_STRESS) adj_stress_body<T, IT>(g[m], b - start[m], xs);
+        else adj_vel_body<T, IT>(g[m], b - start[m], xs);
     } else if constexpr (PH == AG_GRAD) {
-        adj_grad_body(g[m], b - start[m]);
+        adj_grad_body<T, IT>(g[m], b - start[m]);
     } else {
-        adj_gradb_body(g[m], b - start[m]);
+        adj_gradb_body<T, IT>(g[m], b - start[m]);
     }
 }
 

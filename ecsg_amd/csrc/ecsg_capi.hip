@@ -105,6 +105,13 @@ bool fits32(const ecsg::GridDev<T> &v)
 {
     return (double)(v.o + (long)(v.nz + 3) * v.S2) * sizeof(T) < 4294967296.0;
 }
+// the same for the adjoint sweeps of a grid (every array they touch has the
+// grid's layout)
+template <typename T>
+bool fits32(const ecsg::AdjGridDev<T> &a)
+{
+    return (double)(a.o + (long)(a.nz + 3) * a.S2) * sizeof(T) < 4294967296.0;
+}
 
 // wavefield / stiffness positions (init_del_sim.cpp:163-173,
 // init_del_par_field.cpp:82-88 extended to the nine stiffnesses)
@@ -2641,9 +2648,14 @@ public:
         for (int g = ng - 1; g >= 0; g--) {
             launch_mirror(g, AG[g].sm);
             AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
-            hipLaunchKernelGGL((k_adj_stress<T>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
             const int gb = ((a.nx + 64) / 64) * ((a.ny + GBY) / GBY) * ((a.nz + GZC) / GZC);
-            hipLaunchKernelGGL((k_adj_grad<T>), dim3(gb), dim3(64, GBY), 0, s, a);
+            if (fits32(a)) {
+                hipLaunchKernelGGL((k_adj_stress<T, unsigned>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
+                hipLaunchKernelGGL((k_adj_grad<T, unsigned>), dim3(gb), dim3(64, GBY), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((k_adj_stress<T, long>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
+                hipLaunchKernelGGL((k_adj_grad<T, long>), dim3(gb), dim3(64, GBY), 0, s, a);
+            }
         }
         // vz-szz (tridiagonal)
         for (int k = ni - 1; k >= 0; k--) {
@@ -2671,9 +2683,14 @@ public:
             launch_mirror(g, AG[g].vA);
             launch_mirror(g, AG[g].vB);
             AdjGridDev<T> a = adj_dev(g, slot_f, slot_p);
-            hipLaunchKernelGGL((k_adj_vel<T>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
             const int gb = ((a.nx + 64) / 64) * ((a.ny + GBY) / GBY) * ((a.nz + GZC) / GZC);
-            hipLaunchKernelGGL((k_adj_gradb<T>), dim3(gb), dim3(64, GBY), 0, s, a);
+            if (fits32(a)) {
+                hipLaunchKernelGGL((k_adj_vel<T, unsigned>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
+                hipLaunchKernelGGL((k_adj_gradb<T, unsigned>), dim3(gb), dim3(64, GBY), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((k_adj_vel<T, long>), dim3(a.gx * a.gy * a.gz), dim3(64, AWY), 0, s, a);
+                hipLaunchKernelGGL((k_adj_gradb<T, long>), dim3(gb), dim3(64, GBY), 0, s, a);
+            }
         }
         HIPCHK(hipGetLastError());
         return ECSG_OK;
@@ -3612,6 +3629,7 @@ public:
     JacLds<T> *d_jlF = nullptr;        // [slot][interface]
     PointArgs<T> *d_paF = nullptr;     // [slot][member]
     AdjGridDev<T> *d_ag = nullptr;     // [t][grid]: transposed step over ring slots t+1, t
+    bool adj32 = false;                // every grid's adjoint sweeps index in 32 bits
     AdjIfWork<T> *d_aw = nullptr;      // [t][interface, by level]
     MirrorWork<T> *d_mw = nullptr;     // [set][grid], sets: free surface, vy y walls, other velocity mirrors
     JacLds<T> *d_jla = nullptr;        // transposed vx-sxz systems
@@ -3722,10 +3740,12 @@ public:
         std::vector<AdjGridDev<T>> ag;
         std::vector<int> gt, gtg;
         gt_tot = gtg_tot = 0;
+        adj32 = true;
         for (int t = 0; t < K; t++)
             for (Engine<T> *e : M)
                 for (int g = 0; g < (int)e->G.size(); g++) {
                     const AdjGridDev<T> a = e->adj_dev(g, t + 1, t);
+                    adj32 = adj32 && fits32(a);
                     ag.push_back(a);
                     if (t == 0) {
                         gt.push_back(gt_tot); gt_tot += a.gx * a.gy * a.gz;
@@ -3848,8 +3868,12 @@ public:
     {
         const bool grad = PH == AG_GRAD || PH == AG_GRADB;
         const int tot = grad ? gtg_tot : gt_tot;
-        hipLaunchKernelGGL((k_adj_grid_batch<T, PH>), dim3(tot), dim3(64, grad ? GBY : AWY), 0, s,
-                           (const AdjGridDev<T> *)(d_ag + (long)t * ng), (const int *)(grad ? d_gtg : d_gt), ng, tot);
+        if (adj32)
+            hipLaunchKernelGGL((k_adj_grid_batch<T, PH, unsigned>), dim3(tot), dim3(64, grad ? GBY : AWY), 0, s,
+                               (const AdjGridDev<T> *)(d_ag + (long)t * ng), (const int *)(grad ? d_gtg : d_gt), ng, tot);
+        else
+            hipLaunchKernelGGL((k_adj_grid_batch<T, PH, long>), dim3(tot), dim3(64, grad ? GBY : AWY), 0, s,
+                               (const AdjGridDev<T> *)(d_ag + (long)t * ng), (const int *)(grad ? d_gtg : d_gt), ng, tot);
     }
 
     // one transposed step of every member (adj_step_impl's phases and order):
