@@ -133,13 +133,20 @@ def batch_run(steps: int, solo_steps: int, warmup: int):
     import ecsg_amd
     from ecsg_amd import model as M
     t0 = time.time()
-    ms = M.batch_workload(nsteps=warmup + steps + solo_steps + 10)
+    cands = [(8, 8), (8, 4), (4, 8), (4, 4)]
+    tune = (len(cands) + 1) * (16 + 8)
+    ms = M.batch_workload(nsteps=tune + warmup + steps + solo_steps + 10)
     nodes = sum(m.total_nodes for m in ms)
-    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=warmup + steps + solo_steps + 10,
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False,
+                                max_steps=tune + warmup + steps + solo_steps + 10,
                                 sxz_solver="jacobi_lds") for m in ms]
     log("[bench] batch: %d members, %.1f M nodes, ready in %.1fs" % (len(ms), nodes / 1e6, time.time() - t0))
     b = ecsg_amd.Batch(sims)
-    by, zc = b.autotune(reps=3)
+    # the stencil-only autotuner's pick and four tilings, each timed over 16
+    # real steps (these steps are part of the warm-up)
+    by, zc = b.autotune(reps=10)
+    tuned = (by, zc)
+    by, zc = b.tune_steps([tuned] + [c for c in cands if c != tuned], steps=16)
     b.step(warmup)
     b.sync()
     t = time.perf_counter()
@@ -158,7 +165,8 @@ def batch_run(steps: int, solo_steps: int, warmup: int):
         sm.close()
     return {"value": round(nodes * steps / el / 1e9, 3), "unit": "Gpt/s", "members": len(ms), "models": 16,
             "shots_per_model": 4, "nodes": nodes, "steps": steps, "ms_per_step": round(el / steps * 1e3, 4),
-            "tiling": {"rows": by, "z_planes": zc}, "graph": True,
+            "tiling": {"rows": by, "z_planes": zc, "stencil_only_autotune": list(tuned)}, "graph": True,
+            "interface_overlap": os.environ.get("ECSG_BATCH_OVERLAP", "1").strip() not in ("0", ""),
             "one_at_a_time_value": round(nodes * solo_steps / el1 / 1e9, 3), "one_at_a_time_steps": solo_steps,
             "workload": "16 ratio-3 two-grid models, fine (nx,ny) in [120,240] (multiples of 3), nz in [30,60] cells, "
                         "coarse (nx/3, ny/3, 2nz); 4 Ricker shots each (10-25 Hz); FP32"}
@@ -183,7 +191,7 @@ def batch_gradient_run(steps: int, every: int):
     for s in sims:
         s.set_checkpointing(every, "native")
     b = ecsg_amd.Batch(sims)
-    b.autotune(reps=2)
+    b.autotune(reps=10)
     obs = [np.zeros((len(m.rec_loc), steps)) for m in ms]
     groups = [k // 4 for k in range(len(ms))]
     t0 = time.perf_counter()

@@ -12,6 +12,7 @@ from __future__ import annotations
 import atexit
 import ctypes as C
 import os
+import time
 import weakref
 from typing import Optional
 
@@ -509,6 +510,29 @@ class Batch:
         by, zc = C.c_int(), C.c_int()
         self._chk(self.L.ecsg_batch_autotune(self.h, int(reps), C.byref(by), C.byref(zc)))
         return by.value, zc.value
+
+    def tune_steps(self, candidates, steps: int = 16):
+        """Times `steps` real batched steps with each (rows, z_planes) tiling
+        of `candidates` and keeps the fastest; returns it.  Unlike autotune,
+        which times the stencil launches alone on a saved copy of the state,
+        this sees the interface solves that run beside them -- and advances
+        every member by len(candidates) x (steps + 8) time steps (8 per
+        candidate for the graph capture, untimed)."""
+        best = None
+        for by, zc in candidates:
+            self.set_tiling(by, zc)
+            self.step(8)
+            self.sync()
+            t = time.perf_counter()
+            self.step(steps)
+            self.sync()
+            el = time.perf_counter() - t
+            if best is None or el < best[0]:
+                best = (el, int(by), int(zc))
+        if best is None:
+            raise EcsgError("tune_steps needs at least one candidate tiling")
+        self.set_tiling(best[1], best[2])
+        return best[1], best[2]
 
     def gradient(self, nsteps: int, observed, group=None) -> np.ndarray:
         """ecsg_batch_gradient: every member's misfit and gradient (read with

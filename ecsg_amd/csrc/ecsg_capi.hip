@@ -3337,20 +3337,45 @@ public:
     PointArgs<T> *d_pa = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // Interface overlap of the batched step (as Engine::enqueue_step_overlap,
+    // schedule 1): the stencils over four plane sets -- velocity interiors
+    // and edge planes, stress interiors and edge planes of every member grid
+    // (tables d_pg[q], tile offsets d_ps[q]) -- on s, the vz ghosts and the
+    // sxz/syz ghosts of every interface on s2.  A captured graph holds
+    // OV_STEPS steps, each step's sxz/syz ghosts beside the next one's
+    // velocity interiors.
+    static constexpr int OV_STEPS = 8;
+    bool ov = false;                   // ECSG_BATCH_OVERLAP=0: the serial step
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    GridDev<T> *d_pg[4] = {nullptr, nullptr, nullptr, nullptr};
+    int *d_ps[4] = {nullptr, nullptr, nullptr, nullptr};
+    int pn[4] = {0, 0, 0, 0}, ptot[4] = {0, 0, 0, 0};
+    hipGraph_t graph_ov[2] = {nullptr, nullptr};        // OV_STEPS steps, one step
+    hipGraphExec_t gexec_ov[2] = {nullptr, nullptr};
 
     ~Batch() override
     {
         drop_graph();
         hipFree(d_g); hipFree(d_gs); hipFree(d_if); hipFree(d_jl); hipFree(d_pa);
         for (auto p : d_is) hipFree(p);
+        for (auto p : d_pg) hipFree(p);
+        for (auto p : d_ps) hipFree(p);
         free_grad_tables();
         if (done) hipEventDestroy(done);
+        for (auto e : ev)
+            if (e) hipEventDestroy(e);
+        if (s2) hipStreamDestroy(s2);
         if (s) hipStreamDestroy(s);
     }
     void drop_graph()
     {
         if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+        for (int q = 0; q < 2; q++) {
+            if (gexec_ov[q]) { hipGraphExecDestroy(gexec_ov[q]); gexec_ov[q] = nullptr; }
+            if (graph_ov[q]) { hipGraphDestroy(graph_ov[q]); graph_ov[q] = nullptr; }
+        }
     }
     template <typename P>
     int upload(P **dst, const std::vector<P> &v)
@@ -3381,6 +3406,14 @@ public:
         }
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        bool any_iface = false;
+        for (Engine<T> *e : M) any_iface = any_iface || !e->I.empty();
+        ov = any_iface;
+        if (const char *v = getenv("ECSG_BATCH_OVERLAP")) ov = ov && atoi(v) != 0;
+        if (ov) {
+            HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            for (auto &e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
         return build_tables();
     }
 
@@ -3454,6 +3487,37 @@ public:
         for (const auto &v : gs) all32 = all32 && fits32(v);
         if (int rc = upload(&d_g, gs)) return rc;
         if (int rc = upload(&d_gs, st)) return rc;
+        // the plane sets of the overlapped step (Engine::vlo .. shi)
+        for (int q = 0; q < 4; q++) {
+            std::vector<GridDev<T>> pg;
+            std::vector<int> ps;
+            int pacc = 0;
+            int k = 0;
+            for (Engine<T> *e : M)
+                for (int g = 0; g < (int)e->G.size(); g++, k++) {
+                    const int n1 = e->G[g].n[2] + 1;
+                    const int lo = q < 2 ? e->vlo(g) : e->slo(g), hi = q < 2 ? e->vhi(g) : e->shi(g);
+                    int r[2][2] = {{lo, hi}, {0, 0}};
+                    if (q == 1 || q == 3) {
+                        if (lo >= hi) { r[0][0] = 0; r[0][1] = n1; }
+                        else { r[0][0] = 0; r[0][1] = lo; r[1][0] = hi; r[1][1] = n1; }
+                    }
+                    for (auto &ab : r) {
+                        if (ab[1] <= ab[0]) continue;
+                        GridDev<T> v = gs[k];
+                        v.zb = ab[0];
+                        v.ze = ab[1];
+                        v.nblk = v.gx * v.gy * ((ab[1] - ab[0] + zc - 1) / zc);
+                        pg.push_back(v);
+                        ps.push_back(pacc);
+                        pacc += v.nblk;
+                    }
+                }
+            pn[q] = (int)pg.size();
+            ptot[q] = pacc;
+            if (int rc = upload(&d_pg[q], pg)) return rc;
+            if (int rc = upload(&d_ps[q], ps)) return rc;
+        }
         drop_graph();
         return ECSG_OK;
     }
@@ -3479,6 +3543,87 @@ public:
         case 8: stencils<8>(vel, g); break;
         default: stencils<4>(vel, g); break;
         }
+    }
+
+    template <int BYv>
+    void stencils_set(bool vel, int q)
+    {
+        if (ptot[q] == 0) return;
+        const dim3 gr(ptot[q]), bl(BX, BYv);
+        const GridDev<T> *g = d_pg[q];
+        const int *st = d_ps[q];
+        if (all32) {
+            if (vel) hipLaunchKernelGGL((k_vel_batch<T, BYv, unsigned>), gr, bl, 0, s, g, st, pn[q], ptot[q]);
+            else hipLaunchKernelGGL((k_stress_batch<T, BYv, unsigned>), gr, bl, 0, s, g, st, pn[q], ptot[q]);
+        } else {
+            if (vel) hipLaunchKernelGGL((k_vel_batch<T, BYv, long>), gr, bl, 0, s, g, st, pn[q], ptot[q]);
+            else hipLaunchKernelGGL((k_stress_batch<T, BYv, long>), gr, bl, 0, s, g, st, pn[q], ptot[q]);
+        }
+    }
+    void launch_set(int q)
+    {
+        switch (by) {
+        case 1: stencils_set<1>(q < 2, q); break;
+        case 2: stencils_set<2>(q < 2, q); break;
+        case 8: stencils_set<8>(q < 2, q); break;
+        default: stencils_set<4>(q < 2, q); break;
+        }
+    }
+
+    // nsteps batched time steps with the interface overlap: every member
+    // grid's velocity interior beside the previous step's sxz/syz ghosts,
+    // then (after them) the velocity edge planes and the receivers/sources;
+    // the vz ghosts beside the stress interiors, the stress edge planes after
+    // them; the last step's sxz/syz ghosts are joined at the end.  Every
+    // entry sees the operations of the serial step in the same order.
+    int enqueue_ov(int nsteps)
+    {
+        for (int i = 0; i < nsteps; i++) {
+            launch_set(0);
+            if (i > 0) HIPCHK(hipStreamWaitEvent(s, ev[3], 0));
+            launch_set(1);
+            hipLaunchKernelGGL((k_recsrc_batch<T>), dim3((int)M.size()), dim3(256), 0, s, (const PointArgs<T> *)d_pa);
+            HIPCHK(hipEventRecord(ev[0], s));
+            HIPCHK(hipStreamWaitEvent(s2, ev[0], 0));
+            hipLaunchKernelGGL((k_gvz_rhs_batch<T>), dim3(itot[0]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[0], ni);
+            hipLaunchKernelGGL((k_gvz_solve_wave_batch<T>), dim3(itot[1]), dim3(64), (size_t)(32 * maxn_vz), s2, (const IfWork<T> *)d_if,
+                               (const int *)d_is[1], ni);
+            hipLaunchKernelGGL((k_gvz_fine_batch<T>), dim3(itot[2]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[2], ni);
+            HIPCHK(hipEventRecord(ev[1], s2));
+            launch_set(2);
+            HIPCHK(hipStreamWaitEvent(s, ev[1], 0));
+            launch_set(3);
+            HIPCHK(hipEventRecord(ev[2], s));
+            HIPCHK(hipStreamWaitEvent(s2, ev[2], 0));
+            hipLaunchKernelGGL((k_gsxz_rhs_batch<T>), dim3(itot[3]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[3], ni);
+            JacParams jp{};
+            jp.tol_cap = M[0]->tol_cap; jp.tol_fac = M[0]->tol_fac;
+            jp.floor = 1024 * 2.220446049250313e-16;
+            jp.cap = JACOBI_CAP;
+            hipLaunchKernelGGL((k_jacobi_lds_batch<T>), dim3(ni), dim3(JL), (size_t)(2 * maxw * sizeof(double)), s2,
+                               (const JacLds<T> *)d_jl, jp);
+            hipLaunchKernelGGL((k_gsyz_batch<T>), dim3(itot[4]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[4], ni);
+            HIPCHK(hipEventRecord(ev[3], s2));
+        }
+        if (nsteps > 0) HIPCHK(hipStreamWaitEvent(s, ev[3], 0));
+        HIPCHK(hipGetLastError());
+        return ECSG_OK;
+    }
+    int capture_ov(int q, int nsteps)
+    {
+        if (gexec_ov[q]) return ECSG_OK;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_ov(nsteps);
+        hipGraph_t gr = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s, &gr);
+        if (rc != ECSG_OK || e2 != hipSuccess || !gr) {
+            if (gr) hipGraphDestroy(gr);
+            hipGetLastError();
+            return fail(ECSG_EHIP, "graph capture of the overlapped batched step failed");
+        }
+        graph_ov[q] = gr;
+        HIPCHK(hipGraphInstantiate(&gexec_ov[q], gr, nullptr, nullptr, 0));
+        return ECSG_OK;
     }
 
     // one batched time step (main.cpp:165-261 for every member)
@@ -3521,21 +3666,37 @@ public:
                 if (int rc = build_tables()) return rc;
                 break;
             }
-        if (use_graph && !gexec) {
-            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue();
-            hipGraph_t gr = nullptr;
-            const hipError_t e2 = hipStreamEndCapture(s, &gr);
-            if (rc != ECSG_OK || e2 != hipSuccess || !gr) {
-                if (gr) hipGraphDestroy(gr);
-                return fail(ECSG_EHIP, "graph capture of the batched step failed");
+        if (ov) {
+            if (use_graph) {
+                int left = n;
+                if (left >= OV_STEPS) {
+                    if (int rc = capture_ov(0, OV_STEPS)) return rc;
+                    for (; left >= OV_STEPS; left -= OV_STEPS) HIPCHK(hipGraphLaunch(gexec_ov[0], s));
+                }
+                if (left > 0) {
+                    if (int rc = capture_ov(1, 1)) return rc;
+                    for (; left > 0; left--) HIPCHK(hipGraphLaunch(gexec_ov[1], s));
+                }
+            } else if (int rc = enqueue_ov(n)) {
+                return rc;
             }
-            graph = gr;
-            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-        }
-        for (int i = 0; i < n; i++) {
-            if (use_graph) HIPCHK(hipGraphLaunch(gexec, s));
-            else if (int rc = enqueue()) return rc;
+        } else {
+            if (use_graph && !gexec) {
+                HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                const int rc = enqueue();
+                hipGraph_t gr = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(s, &gr);
+                if (rc != ECSG_OK || e2 != hipSuccess || !gr) {
+                    if (gr) hipGraphDestroy(gr);
+                    return fail(ECSG_EHIP, "graph capture of the batched step failed");
+                }
+                graph = gr;
+                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            }
+            for (int i = 0; i < n; i++) {
+                if (use_graph) HIPCHK(hipGraphLaunch(gexec, s));
+                else if (int rc = enqueue()) return rc;
+            }
         }
         // members' own calls (traces, fields) are ordered after the batch
         HIPCHK(hipEventRecord(done, s));
@@ -3582,24 +3743,39 @@ public:
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         const int bys[4] = {1, 2, 4, 8}, zcs[4] = {2, 4, 8, 16};
-        float best = 1e30f;
-        int bb = by, bz = zc;
-        for (int b : bys)
-            for (int z : zcs) {
-                if (int rc = set_tiling(b, z)) return rc;
+        // the stencil launches of the step that runs: with the interface
+        // overlap, its four plane sets in their order
+        auto sweeps = [&]() {
+            if (ov) {
+                for (int q = 0; q < 4; q++) launch_set(q);
+            } else {
                 launch_stencils(true);
                 launch_stencils(false);
-                HIPCHK(hipEventRecord(e0, s));
-                for (int r = 0; r < reps; r++) {
-                    launch_stencils(true);
-                    launch_stencils(false);
-                }
-                HIPCHK(hipEventRecord(e1, s));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; bb = b; bz = z; }
             }
+        };
+        // two passes over the candidates, each candidate's faster one counts
+        // (one pass picked tilings whose steps differed by 8 % from run to run)
+        float tbest[4][4];
+        for (auto &row : tbest)
+            for (auto &t : row) t = 1e30f;
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    if (int rc = set_tiling(bys[i], zcs[j])) return rc;
+                    sweeps();
+                    HIPCHK(hipEventRecord(e0, s));
+                    for (int r = 0; r < reps; r++) sweeps();
+                    HIPCHK(hipEventRecord(e1, s));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    tbest[i][j] = std::min(tbest[i][j], ms);
+                }
+        float best = 1e30f;
+        int bb = by, bz = zc;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                if (tbest[i][j] < best) { best = tbest[i][j]; bb = bys[i]; bz = zcs[j]; }
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         size_t q = 0;

@@ -3,7 +3,10 @@
 A batch runs the same kernels' arithmetic as a single model (the stencil
 updates are pointwise, the interface solves per interface), so a batch of
 models of different sizes, ratios and shots must reproduce each model run on
-its own bit for bit, in FP32 and FP64, with and without graph replay.
+its own bit for bit, in FP32 and FP64, with and without graph replay, with
+the interface work beside the stencils (the default for batches with
+interfaces: graphs of 8 steps and of one) or after them
+(ECSG_BATCH_OVERLAP=0).
 """
 import numpy as np
 import pytest
@@ -32,10 +35,12 @@ def _state(sim, m):
     return [sim.get(g, c) for g in range(m.num_grids) for c in FIELDS], sim.traces()
 
 
+@pytest.mark.parametrize("overlap", ["1", "0"])
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_reproduces_each_model_alone(precision, graph):
-    nsteps = 40
+def test_batch_reproduces_each_model_alone(precision, graph, overlap, monkeypatch):
+    monkeypatch.setenv("ECSG_BATCH_OVERLAP", overlap)
+    nsteps = 43
     ms = _members(nsteps)
     solo = []
     for m in ms:
@@ -131,6 +136,27 @@ def test_autotune_preserves_state_and_results():
         assert np.array_equal(alone.traces(), sm.traces())
         alone.close()
     b.close()
+
+
+def test_tune_steps_keeps_results():
+    # two candidates x (4 timed + 8 capture steps), then 6 more steps: the
+    # members' traces equal those of each member stepped alone for 30 steps
+    ms = _members(30, seed=44)
+    sims = [ecsg_amd.Simulation(mm, precision="f32", check_energy=False, max_steps=30) for mm in ms]
+    b = ecsg_amd.Batch(sims)
+    by, zc = b.tune_steps([(8, 8), (4, 2)], steps=4)
+    assert (by, zc) in ((8, 8), (4, 2))
+    b.step(6)
+    b.sync()
+    for mm, sm in zip(ms, sims):
+        alone = ecsg_amd.Simulation(mm, precision="f32", check_energy=False, max_steps=30)
+        alone.step(30, graph=False)
+        alone.sync()
+        assert np.array_equal(alone.traces(), sm.traces())
+        alone.close()
+    b.close()
+    with pytest.raises(ecsg_amd.EcsgError):
+        ecsg_amd.Batch(sims).tune_steps([])
 
 
 def test_batch_rejects_energy_check_members():
