@@ -123,10 +123,10 @@ def adjoint_run(m, steps: int, every: int, fmt: str, precision: str = "f32"):
     return el, J, sweeps, levels
 
 
-def batch_run(steps: int, solo_steps: int, warmup: int):
+def batch_run(steps: int, solo_steps: int, warmup: int, tiling=None):
     """Batched shots over uneven two-grid models (16 models x 4 shots):
-    one launch per phase for all 64 members, graph-captured, autotuned
-    tiling; beside it the same members stepped one at a time (each with its
+    one launch per phase for all 64 members, graph-captured, tiling chosen
+    from timed steps (or the fixed `tiling` = (rows, planes)); beside it the same members stepped one at a time (each with its
     own graph-captured step on its own stream, so that several are in flight
     at once; their vx-sxz systems are solved by one workgroup each,
     sxz_solver "jacobi_lds", which lets them share the GPU)."""
@@ -146,7 +146,11 @@ def batch_run(steps: int, solo_steps: int, warmup: int):
     # real steps (these steps are part of the warm-up)
     by, zc = b.autotune(reps=10)
     tuned = (by, zc)
-    by, zc = b.tune_steps([tuned] + [c for c in cands if c != tuned], steps=16)
+    if tiling:
+        by, zc = tiling
+        b.set_tiling(by, zc)
+    else:
+        by, zc = b.tune_steps([tuned] + [c for c in cands if c != tuned], steps=16)
     b.step(warmup)
     b.sync()
     t = time.perf_counter()
@@ -376,6 +380,8 @@ def main():
                     help="steps of a second gradient run that keeps every state in memory (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400, help="steps of the batched-shots leg (0: skip)")
     ap.add_argument("--batch-solo-steps", type=int, default=50)
+    ap.add_argument("--batch-tiling", default=None,
+                    help="fixed tiling ROWSxPLANES of the batched leg (default: chosen from timed steps)")
     ap.add_argument("--batch-grad-steps", type=int, default=200,
                     help="steps of each batch member's gradient (0: skip)")
     ap.add_argument("--batch-grad-every", type=int, default=0,
@@ -493,7 +499,8 @@ def main():
 
     batch = None
     if args.batch_steps > 0:
-        batch = batch_run(args.batch_steps, args.batch_solo_steps, args.warmup)
+        bt = tuple(int(x) for x in args.batch_tiling.split("x")) if args.batch_tiling else None
+        batch = batch_run(args.batch_steps, args.batch_solo_steps, args.warmup, bt)
         log("[bench] batch: %s" % batch)
         if args.batch_grad_steps > 0:
             batch["gradient"] = batch_gradient_run(args.batch_grad_steps, args.batch_grad_every)
