@@ -354,7 +354,11 @@ int ecsg_autotune(ecsg_ctx *ctx, int reps, int *by_out, int *zc_out);
  * tiles of every grid of every member in one launch, the interface kernels
  * over every interface, one workgroup per member for receivers/source and
  * per interface for the Jacobi solve.  With use_graph the batched step is
- * captured once in a hipGraph and replayed.  Members keep their state,
+ * captured once in a hipGraph and replayed.  When a member has an
+ * interface the step runs the interface solves on a second stream beside
+ * the stencil planes away from the interfaces (eight steps per graph; the
+ * same results bit for bit; the environment variable ECSG_BATCH_OVERLAP=0
+ * at ecsg_batch_create selects the serial step).  Members keep their state,
  * traces and accessors; calls on a member after ecsg_batch_step_n are
  * ordered after the batch's work.  Members must outlive the batch. --- */
 typedef struct ecsg_batch ecsg_batch;
@@ -363,7 +367,11 @@ void ecsg_batch_destroy(ecsg_batch *b);
 const char *ecsg_batch_last_error(const ecsg_batch *b);
 int ecsg_batch_step_n(ecsg_batch *b, int nsteps, int use_graph);
 int ecsg_batch_synchronize(ecsg_batch *b);
-/* one tiling for the whole batch, or the fastest of the candidates above */
+/* one tiling for the whole batch, or the fastest of 4 heights x 4 z chunks
+ * by the time of the step's stencil launches alone (on a saved copy of the
+ * state; with the overlap this does not see the solves beside them: time
+ * real steps per candidate to choose among tilings, as ecsg_amd.Batch's
+ * tune_steps does) */
 int ecsg_batch_set_tiling(ecsg_batch *b, int by, int zc);
 int ecsg_batch_autotune(ecsg_batch *b, int reps, int *by, int *zc);
 
