@@ -3406,6 +3406,12 @@ public:
         }
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        if (const char *v = getenv("ECSG_BATCH_PERSIST")) persist = atoi(v);
+        {
+            int dev = 0;
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        }
         bool any_iface = false;
         for (Engine<T> *e : M) any_iface = any_iface || !e->I.empty();
         ov = any_iface;
@@ -3522,10 +3528,20 @@ public:
         return ECSG_OK;
     }
 
+    // workgroups of a batched stencil launch over `tiles` tiles: one per
+    // tile, or (ECSG_BATCH_PERSIST = k > 0) k per CU, a multiple of 8, each
+    // looping over tiles
+    int persist = 0, ncu = 0;
+    int wgs(int tiles) const
+    {
+        if (persist <= 0 || ncu <= 0) return tiles;
+        const int w = (persist * ncu + 7) / 8 * 8;
+        return std::min(tiles, w);
+    }
     template <int BYv>
     void stencils(bool vel, const GridDev<T> *g)
     {
-        const dim3 gr(gtot), bl(BX, BYv);
+        const dim3 gr(wgs(gtot)), bl(BX, BYv);
         if (all32) {
             if (vel) hipLaunchKernelGGL((k_vel_batch<T, BYv, unsigned>), gr, bl, 0, s, g, (const int *)d_gs, ng, gtot);
             else hipLaunchKernelGGL((k_stress_batch<T, BYv, unsigned>), gr, bl, 0, s, g, (const int *)d_gs, ng, gtot);
@@ -3549,7 +3565,7 @@ public:
     void stencils_set(bool vel, int q)
     {
         if (ptot[q] == 0) return;
-        const dim3 gr(ptot[q]), bl(BX, BYv);
+        const dim3 gr(wgs(ptot[q])), bl(BX, BYv);
         const GridDev<T> *g = d_pg[q];
         const int *st = d_ps[q];
         if (all32) {

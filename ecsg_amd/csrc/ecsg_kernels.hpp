@@ -481,14 +481,19 @@ __device__ __forceinline__ int batch_find(const int *__restrict__ start, int n, 
     return lo;
 }
 
+// A launch of fewer workgroups than tiles is persistent: workgroup w takes
+// the tiles w, w + gridDim.x, ... of the XCD-ordered list (gridDim.x a
+// multiple of 8 keeps each workgroup within its XCD's range of tiles).
 template <typename T, int BYv, typename IT>
 __global__ __launch_bounds__(BX * BYv) void k_vel_batch(const GridDev<T> *__restrict__ gs, const int *__restrict__ start, int n,
                                                         int total)
 {
     __shared__ double lds[BYv];
-    const int b = xcd_tile(blockIdx.x, total);
-    const int m = batch_find(start, n, b);
-    vel_tile<T, false, BYv, IT>(gs[m], b - start[m], nullptr, nullptr, nullptr, lds);
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const int b = xcd_tile(v, total);
+        const int m = batch_find(start, n, b);
+        vel_tile<T, false, BYv, IT>(gs[m], b - start[m], nullptr, nullptr, nullptr, lds);
+    }
 }
 
 template <typename T, int BYv, typename IT>
@@ -496,9 +501,11 @@ __global__ __launch_bounds__(BX * BYv) void k_stress_batch(const GridDev<T> *__r
                                                            int total)
 {
     __shared__ double lds[BYv];
-    const int b = xcd_tile(blockIdx.x, total);
-    const int m = batch_find(start, n, b);
-    stress_tile<T, false, BYv, IT>(gs[m], b - start[m], nullptr, lds);
+    for (int v = blockIdx.x; v < total; v += gridDim.x) {
+        const int b = xcd_tile(v, total);
+        const int m = batch_find(start, n, b);
+        stress_tile<T, false, BYv, IT>(gs[m], b - start[m], nullptr, lds);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -358,7 +358,9 @@ int ecsg_autotune(ecsg_ctx *ctx, int reps, int *by_out, int *zc_out);
  * interface the step runs the interface solves on a second stream beside
  * the stencil planes away from the interfaces (eight steps per graph; the
  * same results bit for bit; the environment variable ECSG_BATCH_OVERLAP=0
- * at ecsg_batch_create selects the serial step).  Members keep their state,
+ * at ecsg_batch_create selects the serial step; ECSG_BATCH_PERSIST=k > 0
+ * makes the stencil launches persistent, k workgroups per CU looping over
+ * the tiles, slower on the workloads measured).  Members keep their state,
  * traces and accessors; calls on a member after ecsg_batch_step_n are
  * ordered after the batch's work.  Members must outlive the batch. --- */
 typedef struct ecsg_batch ecsg_batch;

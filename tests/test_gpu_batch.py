@@ -62,6 +62,37 @@ def test_batch_reproduces_each_model_alone(precision, graph, overlap, monkeypatc
     b.close()
 
 
+@pytest.mark.parametrize("overlap", ["1", "0"])
+def test_persistent_stencil_launches_match_members(overlap, monkeypatch):
+    # ECSG_BATCH_PERSIST=1: one workgroup per CU, each looping over several
+    # tiles of the ~800-tile batch (two ratio-3 and two ratio-2 models of
+    # 120 x 120 x 20 fine cells); the same results as each member alone
+    monkeypatch.setenv("ECSG_BATCH_OVERLAP", overlap)
+    monkeypatch.setenv("ECSG_BATCH_PERSIST", "1")
+    nsteps = 24
+    ms = [composite((3,), coarse=(40, 40, 10), nz_fine=(20,), nsteps=nsteps, seed=61),
+          composite((3,), coarse=(40, 39, 12), nz_fine=(18,), nsteps=nsteps, seed=62),
+          composite((2,), coarse=(60, 60, 10), nz_fine=(20,), nsteps=nsteps, seed=63),
+          composite((2,), coarse=(58, 61, 8), nz_fine=(16,), iso=True, nsteps=nsteps, seed=64)]
+    solo = []
+    for m in ms:
+        s = ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=nsteps)
+        s.step(nsteps, graph=False)
+        s.sync()
+        solo.append(_state(s, m))
+        s.close()
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=nsteps) for m in ms]
+    b = ecsg_amd.Batch(sims)
+    b.step(nsteps)
+    b.sync()
+    for k, (s, m) in enumerate(zip(sims, ms)):
+        fields, tr = _state(s, m)
+        assert np.array_equal(tr, solo[k][1]), "member %d traces" % k
+        for a, c in zip(fields, solo[k][0]):
+            assert np.array_equal(a, c), "member %d fields" % k
+    b.close()
+
+
 def test_batch_graph_replays_interleaved_with_reads():
     """Graph replays in chunks with reads of the members in between: every
     chunk equals the eager batch."""
