@@ -392,6 +392,9 @@ public:
 
     ~Engine() override
     {
+        // let enqueued steps finish before their graph and arrays go
+        if (s) hipStreamSynchronize(s);
+        if (s2) hipStreamSynchronize(s2);
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         for (auto &g : G) {
@@ -3356,6 +3359,9 @@ public:
 
     ~Batch() override
     {
+        // let launched steps finish before their graphs and tables go
+        if (s) hipStreamSynchronize(s);
+        if (s2) hipStreamSynchronize(s2);
         drop_graph();
         hipFree(d_g); hipFree(d_gs); hipFree(d_if); hipFree(d_jl); hipFree(d_pa);
         for (auto p : d_is) hipFree(p);
