@@ -176,7 +176,7 @@ def batch_run(steps: int, solo_steps: int, warmup: int, tiling=None):
                         "coarse (nx/3, ny/3, 2nz); 4 Ricker shots each (10-25 Hz); FP32"}
 
 
-def batch_gradient_run(steps: int, every: int):
+def batch_gradient_run(steps: int, every: int, tiling=None):
     """The batched-shots workload's gradients with one launch per phase
     (ecsg_batch_gradient): 64 members (16 models x 4 shots), forward with
     checkpoints + recomputation + transposed steps + 9 Cij and buoyancy
@@ -185,7 +185,10 @@ def batch_gradient_run(steps: int, every: int):
     interval `every` (0: the largest that fits every member's ring in device
     memory).  A first call of the same length does the set-up (rings,
     checkpoints, interface term lists), as the first gradient of an iterative
-    inversion would; the second is timed."""
+    inversion would; the second is timed.  The stencils' tiling is the
+    fixed `tiling` or, as in the batched forward leg, the fastest of the
+    autotuner's pick and four tall tilings over 16 timed forward steps (the
+    gradient starts from rest whatever these steps did)."""
     import ecsg_amd
     from ecsg_amd import model as M
     ms = M.batch_workload(nsteps=steps + 2)
@@ -195,7 +198,12 @@ def batch_gradient_run(steps: int, every: int):
     for s in sims:
         s.set_checkpointing(every, "native")
     b = ecsg_amd.Batch(sims)
-    b.autotune(reps=10)
+    tuned = b.autotune(reps=10)
+    if tiling:
+        b.set_tiling(*tiling)
+    else:
+        cands = [(8, 8), (8, 4), (4, 8), (4, 4)]
+        tiling = b.tune_steps([tuned] + [c for c in cands if c != tuned], steps=16)
     obs = [np.zeros((len(m.rec_loc), steps)) for m in ms]
     groups = [k // 4 for k in range(len(ms))]
     t0 = time.perf_counter()
@@ -212,6 +220,7 @@ def batch_gradient_run(steps: int, every: int):
     return {"value": round(nodes * steps / el / 1e9, 3), "unit": "Gpt/s", "members": len(ms), "steps": steps,
             "checkpoint_every": K, "recomputed_steps": max(0, steps - K), "seconds": round(el, 3),
             "first_call_seconds": round(setup, 3), "ms_per_step": round(el / steps * 1e3, 3),
+            "tiling": {"rows": tiling[0], "z_planes": tiling[1]},
             "misfit_per_model": [float("%.6e" % j) for j in per_model],
             "what": "ecsg_batch_gradient: forward with checkpoints + recomputation + transposed step + 9 Cij and "
                     "buoyancy gradients of all 64 members, one launch per phase; gradients summed per model on the "
@@ -503,7 +512,7 @@ def main():
         batch = batch_run(args.batch_steps, args.batch_solo_steps, args.warmup, bt)
         log("[bench] batch: %s" % batch)
         if args.batch_grad_steps > 0:
-            batch["gradient"] = batch_gradient_run(args.batch_grad_steps, args.batch_grad_every)
+            batch["gradient"] = batch_gradient_run(args.batch_grad_steps, args.batch_grad_every, bt)
             batch["gradient"]["cost_vs_batched_forward"] = round(batch["value"] / batch["gradient"]["value"], 3)
             log("[bench] batch gradients: %s" % batch["gradient"])
 

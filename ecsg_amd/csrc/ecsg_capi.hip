@@ -3503,35 +3503,44 @@ public:
         for (int q = 0; q < 4; q++) {
             std::vector<GridDev<T>> pg;
             std::vector<int> ps;
-            int pacc = 0;
-            int k = 0;
-            for (Engine<T> *e : M)
-                for (int g = 0; g < (int)e->G.size(); g++, k++) {
-                    const int n1 = e->G[g].n[2] + 1;
-                    const int lo = q < 2 ? e->vlo(g) : e->slo(g), hi = q < 2 ? e->vhi(g) : e->shi(g);
-                    int r[2][2] = {{lo, hi}, {0, 0}};
-                    if (q == 1 || q == 3) {
-                        if (lo >= hi) { r[0][0] = 0; r[0][1] = n1; }
-                        else { r[0][0] = 0; r[0][1] = lo; r[1][0] = hi; r[1][1] = n1; }
-                    }
-                    for (auto &ab : r) {
-                        if (ab[1] <= ab[0]) continue;
-                        GridDev<T> v = gs[k];
-                        v.zb = ab[0];
-                        v.ze = ab[1];
-                        v.nblk = v.gx * v.gy * ((ab[1] - ab[0] + zc - 1) / zc);
-                        pg.push_back(v);
-                        ps.push_back(pacc);
-                        pacc += v.nblk;
-                    }
-                }
+            ptot[q] = plane_set(q, gs.data(), pg, ps);
             pn[q] = (int)pg.size();
-            ptot[q] = pacc;
             if (int rc = upload(&d_pg[q], pg)) return rc;
             if (int rc = upload(&d_ps[q], ps)) return rc;
         }
         drop_graph();
         return ECSG_OK;
+    }
+    // Plane set q of the overlapped step (0: velocity interiors, 1: velocity
+    // edge planes, 2: stress interiors, 3: stress edge planes) of the member
+    // grids gs[0..] in member order: descriptors appended to pg, their first
+    // tiles to ps; returns the set's tile count.  The same for every table
+    // of one tiling, so one ps serves the gradient's ring-slot tables too.
+    int plane_set(int q, const GridDev<T> *gs, std::vector<GridDev<T>> &pg, std::vector<int> &ps) const
+    {
+        int pacc = 0;
+        int k = 0;
+        for (Engine<T> *e : M)
+            for (int g = 0; g < (int)e->G.size(); g++, k++) {
+                const int n1 = e->G[g].n[2] + 1;
+                const int lo = q < 2 ? e->vlo(g) : e->slo(g), hi = q < 2 ? e->vhi(g) : e->shi(g);
+                int r[2][2] = {{lo, hi}, {0, 0}};
+                if (q == 1 || q == 3) {
+                    if (lo >= hi) { r[0][0] = 0; r[0][1] = n1; }
+                    else { r[0][0] = 0; r[0][1] = lo; r[1][0] = hi; r[1][1] = n1; }
+                }
+                for (auto &ab : r) {
+                    if (ab[1] <= ab[0]) continue;
+                    GridDev<T> v = gs[k];
+                    v.zb = ab[0];
+                    v.ze = ab[1];
+                    v.nblk = v.gx * v.gy * ((ab[1] - ab[0] + zc - 1) / zc);
+                    pg.push_back(v);
+                    ps.push_back(pacc);
+                    pacc += v.nblk;
+                }
+            }
+        return pacc;
     }
 
     // workgroups of a batched stencil launch over `tiles` tiles: one per
@@ -3568,11 +3577,10 @@ public:
     }
 
     template <int BYv>
-    void stencils_set(bool vel, int q)
+    void stencils_set(bool vel, int q, const GridDev<T> *g)
     {
         if (ptot[q] == 0) return;
         const dim3 gr(wgs(ptot[q])), bl(BX, BYv);
-        const GridDev<T> *g = d_pg[q];
         const int *st = d_ps[q];
         if (all32) {
             if (vel) hipLaunchKernelGGL((k_vel_batch<T, BYv, unsigned>), gr, bl, 0, s, g, st, pn[q], ptot[q]);
@@ -3582,15 +3590,16 @@ public:
             else hipLaunchKernelGGL((k_stress_batch<T, BYv, long>), gr, bl, 0, s, g, st, pn[q], ptot[q]);
         }
     }
-    void launch_set(int q)
+    void launch_set(int q, const GridDev<T> *g)
     {
         switch (by) {
-        case 1: stencils_set<1>(q < 2, q); break;
-        case 2: stencils_set<2>(q < 2, q); break;
-        case 8: stencils_set<8>(q < 2, q); break;
-        default: stencils_set<4>(q < 2, q); break;
+        case 1: stencils_set<1>(q < 2, q, g); break;
+        case 2: stencils_set<2>(q < 2, q, g); break;
+        case 8: stencils_set<8>(q < 2, q, g); break;
+        default: stencils_set<4>(q < 2, q, g); break;
         }
     }
+    void launch_set(int q) { launch_set(q, d_pg[q]); }
 
     // nsteps batched time steps with the interface overlap: every member
     // grid's velocity interior beside the previous step's sxz/syz ghosts,
@@ -3598,33 +3607,50 @@ public:
     // the vz ghosts beside the stress interiors, the stress edge planes after
     // them; the last step's sxz/syz ghosts are joined at the end.  Every
     // entry sees the operations of the serial step in the same order.
+    // tables of one overlapped step: the plane sets (as d_pg), interface
+    // work, LDS Jacobi and point descriptors
+    struct OvTables {
+        const GridDev<T> *pg[4];
+        const IfWork<T> *w;
+        const JacLds<T> *jl;
+        const PointArgs<T> *pa;
+    };
+    OvTables ov_tables() const
+    {
+        return OvTables{{d_pg[0], d_pg[1], d_pg[2], d_pg[3]}, d_if, d_jl, d_pa};
+    }
     int enqueue_ov(int nsteps)
     {
+        return enqueue_ov_steps(nsteps, [this](int) { return ov_tables(); });
+    }
+    template <typename F>
+    int enqueue_ov_steps(int nsteps, F tables)
+    {
         for (int i = 0; i < nsteps; i++) {
-            launch_set(0);
+            const OvTables tb = tables(i);
+            launch_set(0, tb.pg[0]);
             if (i > 0) HIPCHK(hipStreamWaitEvent(s, ev[3], 0));
-            launch_set(1);
-            hipLaunchKernelGGL((k_recsrc_batch<T>), dim3((int)M.size()), dim3(256), 0, s, (const PointArgs<T> *)d_pa);
+            launch_set(1, tb.pg[1]);
+            hipLaunchKernelGGL((k_recsrc_batch<T>), dim3((int)M.size()), dim3(256), 0, s, tb.pa);
             HIPCHK(hipEventRecord(ev[0], s));
             HIPCHK(hipStreamWaitEvent(s2, ev[0], 0));
-            hipLaunchKernelGGL((k_gvz_rhs_batch<T>), dim3(itot[0]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[0], ni);
-            hipLaunchKernelGGL((k_gvz_solve_wave_batch<T>), dim3(itot[1]), dim3(64), (size_t)(32 * maxn_vz), s2, (const IfWork<T> *)d_if,
+            hipLaunchKernelGGL((k_gvz_rhs_batch<T>), dim3(itot[0]), dim3(256), 0, s2, tb.w, (const int *)d_is[0], ni);
+            hipLaunchKernelGGL((k_gvz_solve_wave_batch<T>), dim3(itot[1]), dim3(64), (size_t)(32 * maxn_vz), s2, tb.w,
                                (const int *)d_is[1], ni);
-            hipLaunchKernelGGL((k_gvz_fine_batch<T>), dim3(itot[2]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[2], ni);
+            hipLaunchKernelGGL((k_gvz_fine_batch<T>), dim3(itot[2]), dim3(256), 0, s2, tb.w, (const int *)d_is[2], ni);
             HIPCHK(hipEventRecord(ev[1], s2));
-            launch_set(2);
+            launch_set(2, tb.pg[2]);
             HIPCHK(hipStreamWaitEvent(s, ev[1], 0));
-            launch_set(3);
+            launch_set(3, tb.pg[3]);
             HIPCHK(hipEventRecord(ev[2], s));
             HIPCHK(hipStreamWaitEvent(s2, ev[2], 0));
-            hipLaunchKernelGGL((k_gsxz_rhs_batch<T>), dim3(itot[3]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[3], ni);
+            hipLaunchKernelGGL((k_gsxz_rhs_batch<T>), dim3(itot[3]), dim3(256), 0, s2, tb.w, (const int *)d_is[3], ni);
             JacParams jp{};
             jp.tol_cap = M[0]->tol_cap; jp.tol_fac = M[0]->tol_fac;
             jp.floor = 1024 * 2.220446049250313e-16;
             jp.cap = JACOBI_CAP;
-            hipLaunchKernelGGL((k_jacobi_lds_batch<T>), dim3(ni), dim3(JL), (size_t)(2 * maxw * sizeof(double)), s2,
-                               (const JacLds<T> *)d_jl, jp);
-            hipLaunchKernelGGL((k_gsyz_batch<T>), dim3(itot[4]), dim3(256), 0, s2, (const IfWork<T> *)d_if, (const int *)d_is[4], ni);
+            hipLaunchKernelGGL((k_jacobi_lds_batch<T>), dim3(ni), dim3(JL), (size_t)(2 * maxw * sizeof(double)), s2, tb.jl, jp);
+            hipLaunchKernelGGL((k_gsyz_batch<T>), dim3(itot[4]), dim3(256), 0, s2, tb.w, (const int *)d_is[4], ni);
             HIPCHK(hipEventRecord(ev[3], s2));
         }
         if (nsteps > 0) HIPCHK(hipStreamWaitEvent(s, ev[3], 0));
@@ -3823,6 +3849,7 @@ public:
     int K_used = 0;
     int last_interval() const override { return K_used; }
     GridDev<T> *d_gF = nullptr;        // [slot][grid]: forward stencils, ring slot `slot` (0: none)
+    GridDev<T> *d_pgF[4] = {nullptr, nullptr, nullptr, nullptr};  // [slot][pn[q]]: plane sets of the overlapped step
     IfWork<T> *d_ifF = nullptr;        // [slot][interface]
     JacLds<T> *d_jlF = nullptr;        // [slot][interface]
     PointArgs<T> *d_paF = nullptr;     // [slot][member]
@@ -3848,6 +3875,7 @@ public:
                         (void *)d_jla, (void *)d_ra, (void *)d_gt, (void *)d_gtg, (void *)d_ras, (void *)d_aps})
             hipFree(p);
         for (auto &p : d_ms) { hipFree(p); p = nullptr; }
+        for (auto &p : d_pgF) { hipFree(p); p = nullptr; }
         d_gF = nullptr; d_ifF = nullptr; d_jlF = nullptr; d_paF = nullptr; d_ag = nullptr; d_aw = nullptr; d_mw = nullptr;
         d_jla = nullptr; d_ra = nullptr; d_gt = nullptr; d_gtg = nullptr; d_ras = nullptr; d_aps = nullptr;
     }
@@ -3931,6 +3959,18 @@ public:
             for (Engine<T> *e : M) e->set_store(-1);
         }
         if (int rc = upload(&d_gF, gF)) return rc;
+        if (ov) {
+            // the overlapped step's plane sets of every slot ([slot][pn[q]])
+            for (int q = 0; q < 4; q++) {
+                std::vector<GridDev<T>> pg;
+                std::vector<int> ps;
+                for (int sl = 0; sl <= K; sl++) {
+                    ps.clear();
+                    plane_set(q, gF.data() + (size_t)sl * ng, pg, ps);
+                }
+                if (int rc = upload(&d_pgF[q], pg)) return rc;
+            }
+        }
         if (int rc = upload(&d_ifF, ifF)) return rc;
         if (int rc = upload(&d_jlF, jlF)) return rc;
         if (int rc = upload(&d_paF, paF)) return rc;
@@ -4130,6 +4170,28 @@ public:
         const long nif = ni, nm = (long)M.size();
         return enqueue_tables(d_gF + (long)slot * ng, d_ifF + (long)slot * nif, d_jlF + (long)slot * nif, d_paF + (long)slot * nm);
     }
+    // n forward steps, step i into ring slot `ring ? slot0 + i : slot0`;
+    // with the interface overlap the steps are enqueued as one overlapped
+    // sequence (each step's sxz/syz ghosts beside the next one's velocity
+    // interiors)
+    int fwd_steps(int slot0, int n, bool ring)
+    {
+        if (!ov) {
+            for (int i = 0; i < n; i++)
+                if (int rc = fwd_step(ring ? slot0 + i : slot0)) return rc;
+            return ECSG_OK;
+        }
+        const long nif = ni, nm = (long)M.size();
+        return enqueue_ov_steps(n, [&](int i) {
+            const long sl = ring ? slot0 + i : slot0;
+            OvTables tb;
+            for (int q = 0; q < 4; q++) tb.pg[q] = d_pgF[q] + sl * pn[q];
+            tb.w = d_ifF + sl * nif;
+            tb.jl = d_jlF + sl * nif;
+            tb.pa = d_paF + sl * nm;
+            return tb;
+        });
+    }
 
     int gradient(int N, const double *const *obs, double *misfit, const int *group) override
     {
@@ -4185,13 +4247,11 @@ public:
             if (j < nseg - 1) {
                 for (Engine<T> *e : M)
                     if (int rc = e->ck_save(j)) return fail(rc, e->st.msg);
-                for (int t = 0; t < len; t++)
-                    if (int rc = fwd_step(0)) return rc;
+                if (int rc = fwd_steps(0, len, false)) return rc;
             } else {
                 for (Engine<T> *e : M)
                     if (int rc = e->ring_save(0)) return fail(rc, e->st.msg);
-                for (int t = 0; t < len; t++)
-                    if (int rc = fwd_step(t + 1)) return rc;
+                if (int rc = fwd_steps(1, len, true)) return rc;
             }
         }
         HIPCHK(hipStreamSynchronize(s));
@@ -4224,8 +4284,7 @@ public:
                     if (int rc = e->ck_restore(j, seg0(j))) return fail(rc, e->st.msg);
                     if (int rc = e->ring_save(0)) return fail(rc, e->st.msg);
                 }
-                for (int t = 0; t < len; t++)
-                    if (int rc = fwd_step(t + 1)) return rc;
+                if (int rc = fwd_steps(1, len, true)) return rc;
             }
             for (int t = len - 1; t >= 0; t--)
                 if (int rc = adj_step(seg0(j) + t, t, true)) return rc;
