@@ -191,10 +191,12 @@ def batch_gradient_run(steps: int, every: int, tiling=None):
     gradient starts from rest whatever these steps did)."""
     import ecsg_amd
     from ecsg_amd import model as M
-    ms = M.batch_workload(nsteps=steps + 2)
+    cands = [(8, 8), (8, 4), (4, 8), (4, 4)]
+    tune = (len(cands) + 1) * (16 + 8)   # forward steps the tiling choice may take
+    ms = M.batch_workload(nsteps=max(steps, tune) + 2)
     nodes = sum(m.total_nodes for m in ms)
-    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=steps, sxz_solver="jacobi_lds")
-            for m in ms]
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=max(steps, tune),
+                                sxz_solver="jacobi_lds") for m in ms]
     for s in sims:
         s.set_checkpointing(every, "native")
     b = ecsg_amd.Batch(sims)
@@ -202,7 +204,6 @@ def batch_gradient_run(steps: int, every: int, tiling=None):
     if tiling:
         b.set_tiling(*tiling)
     else:
-        cands = [(8, 8), (8, 4), (4, 8), (4, 4)]
         tiling = b.tune_steps([tuned] + [c for c in cands if c != tuned], steps=16)
     obs = [np.zeros((len(m.rec_loc), steps)) for m in ms]
     groups = [k // 4 for k in range(len(ms))]
