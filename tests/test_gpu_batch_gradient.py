@@ -4,7 +4,8 @@ batched-shots workload (16 uneven two-grid models x 4 shots).
 The batched transposed step runs the single-model adjoint kernels' bodies
 over flat work lists and the batched forward step the single-model step's
 arithmetic, so every member's misfit and gradient must equal its own
-ecsg_gradient bit for bit (FP32 and FP64, with recomputed segments); a
+ecsg_gradient bit for bit (FP32 and FP64, with recomputed segments, with
+the forward steps' interface solves beside the stencils or after them); a
 model's gradient -- its shots' accumulators added in member order on the
 device, in the storage precision -- equals the sum of the shots' own
 gradients to that precision's rounding (1e-5 FP32, 1e-12 FP64, relative).
@@ -47,9 +48,13 @@ def _solo(m, precision, nsteps, every, obs):
     return out
 
 
+@pytest.mark.parametrize("overlap", ["1", "0"])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("every", [7, 0])
-def test_batch_gradient_equals_each_member_alone(precision, every):
+def test_batch_gradient_equals_each_member_alone(precision, every, overlap, monkeypatch):
+    # forward and recomputed steps overlapped (the default for members with
+    # interfaces) or serial (ECSG_BATCH_OVERLAP=0)
+    monkeypatch.setenv("ECSG_BATCH_OVERLAP", overlap)
     nsteps = 30
     ms = _members(nsteps)
     obs = [_observed(m, nsteps, 10 + k) for k, m in enumerate(ms)]
