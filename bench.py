@@ -275,6 +275,23 @@ def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: 
         out[name + "_GBps"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
     del buf
     torch.cuda.empty_cache()
+    # the copy ceiling: the runtime's device-to-device copy of 4 GiB (bytes
+    # read + bytes written per second)
+    n = (4 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    out["copy_GBps"] = round(2 * 4 * n / (ms * 1e-3) / 1e9, 1)
+    del src, dst
+    torch.cuda.empty_cache()
     return out
 
 
