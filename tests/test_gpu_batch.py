@@ -93,6 +93,28 @@ def test_persistent_stencil_launches_match_members(overlap, monkeypatch):
     b.close()
 
 
+def test_batch_closed_with_steps_in_flight():
+    # closing a batch right after enqueueing its steps (no synchronize):
+    # the steps complete before its graphs and tables go, and the members
+    # then read the stepped state
+    nsteps = 40
+    ms = _members(nsteps, seed=70)
+    solo = []
+    for m in ms:
+        s = ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=nsteps)
+        s.step(nsteps, graph=False)
+        s.sync()
+        solo.append(s.traces())
+        s.close()
+    sims = [ecsg_amd.Simulation(m, precision="f32", check_energy=False, max_steps=nsteps) for m in ms]
+    b = ecsg_amd.Batch(sims)
+    b.step(nsteps)
+    b.close()
+    for k, s in enumerate(sims):
+        assert np.array_equal(s.traces(), solo[k]), "member %d" % k
+        s.close()
+
+
 def test_batch_graph_replays_interleaved_with_reads():
     """Graph replays in chunks with reads of the members in between: every
     chunk equals the eager batch."""
