@@ -53,6 +53,27 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
     return x * q + (x < rem ? x : rem) + k;
 }
 
+// Tile order of the forward stencil sweeps: bands of ECSG_TILE_BAND tile
+// rows (gx x band tiles, one z chunk) dealt to the 8 XCDs in turn, so that
+// the XCDs sweep neighbouring bands of the same planes at the same time;
+// the tiles past the last whole group of 8 bands keep their own index.
+// ECSG_TILE_BAND = 0: one contiguous range of tiles per XCD (xcd_tile).
+#ifndef ECSG_TILE_BAND
+#define ECSG_TILE_BAND 4
+#endif
+__device__ __forceinline__ int sweep_tile(int b, int n, int gx)
+{
+#if ECSG_TILE_BAND > 0
+    const int gs = gx * ECSG_TILE_BAND, full = n / (8 * gs) * (8 * gs);
+    if (b >= full) return b;
+    const int x = b & 7, k = b >> 3;
+    return ((k / gs) * 8 + x) * gs + k % gs;
+#else
+    (void)gx;
+    return xcd_tile(b, n);
+#endif
+}
+
 // Cache hints of the stencil sweeps (ECSG_NT bit 0: streaming stores of the
 // updated fields; bit 1: non-temporal loads of the single-use streams, the
 // stiffnesses and the values being updated).  Both together took the
@@ -296,7 +317,7 @@ __global__ __launch_bounds__(BX * BYv) void k_vel(GridDev<T> g, double *__restri
                                                   const int *__restrict__ src, T *__restrict__ src_old)
 {
     __shared__ double lds[BYv];
-    vel_tile<T, EN, BYv, IT>(g, xcd_tile(blockIdx.x, g.nblk), part, src, src_old, lds);
+    vel_tile<T, EN, BYv, IT>(g, sweep_tile(blockIdx.x, g.nblk, g.gx), part, src, src_old, lds);
 }
 
 // compliance of the normal block at one point (energy weights)
@@ -463,7 +484,7 @@ template <typename T, bool EN, int BYv, typename IT>
 __global__ __launch_bounds__(BX * BYv) void k_stress(GridDev<T> g, double *__restrict__ part)
 {
     __shared__ double lds[BYv];
-    stress_tile<T, EN, BYv, IT>(g, xcd_tile(blockIdx.x, g.nblk), part, lds);
+    stress_tile<T, EN, BYv, IT>(g, sweep_tile(blockIdx.x, g.nblk, g.gx), part, lds);
 }
 
 // Batched stencil launches: the workgroups of many grids (of many models) in
