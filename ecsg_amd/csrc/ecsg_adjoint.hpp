@@ -130,6 +130,9 @@ __device__ __forceinline__ T shfl_dn(T v) { return __shfl_down(v, 1, 64); }
 template <typename T>
 __device__ __forceinline__ T shfl_upw(T v) { return __shfl_up(v, 1, 64); }
 
+// Workgroups take their tiles in the forward sweeps' banded XCD order
+// (sweep_tile): k_adj_stress 2.51 -> 2.29 ms and k_adj_vel 2.04 -> 1.85 ms on
+// the 1207x1207x81 grid against one contiguous tile range per XCD.
 // The weights of the neighbouring rows (ey at iy+1; hxy, hyz at iy-1) come
 // from the other wavefronts of the workgroup through LDS (xs: two buffers of
 // three AWY x 64 planes, one barrier per z plane); only the workgroup's edge
@@ -227,7 +230,7 @@ template <typename T, typename IT>
 __global__ __launch_bounds__(64 *AWY) void k_adj_stress(AdjGridDev<T> a)
 {
     __shared__ T xs[AXS];
-    adj_stress_body<T, IT>(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz), xs);
+    adj_stress_body<T, IT>(a, sweep_tile(blockIdx.x, a.gx * a.gy * a.gz, a.gx), xs);
 }
 
 
@@ -312,7 +315,7 @@ __device__ __forceinline__ void adj_grad_body(const AdjGridDev<T> &a, const int 
 template <typename T, typename IT>
 __global__ __launch_bounds__(64 * GBY) void k_adj_grad(AdjGridDev<T> a)
 {
-    adj_grad_body<T, IT>(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
+    adj_grad_body<T, IT>(a, sweep_tile(blockIdx.x, adj_grad_tiles(a), (a.nx + 64) / 64));
 }
 
 
@@ -411,7 +414,7 @@ template <typename T, typename IT>
 __global__ __launch_bounds__(64 *AWY) void k_adj_vel(AdjGridDev<T> a)
 {
     __shared__ T xs[AXS];
-    adj_vel_body<T, IT>(a, xcd_tile(blockIdx.x, a.gx * a.gy * a.gz), xs);
+    adj_vel_body<T, IT>(a, sweep_tile(blockIdx.x, a.gx * a.gy * a.gz, a.gx), xs);
 }
 
 
@@ -456,7 +459,7 @@ __device__ __forceinline__ void adj_gradb_body(const AdjGridDev<T> &a, const int
 template <typename T, typename IT>
 __global__ __launch_bounds__(64 * GBY) void k_adj_gradb(AdjGridDev<T> a)
 {
-    adj_gradb_body<T, IT>(a, xcd_tile(blockIdx.x, adj_grad_tiles(a)));
+    adj_gradb_body<T, IT>(a, sweep_tile(blockIdx.x, adj_grad_tiles(a), (a.nx + 64) / 64));
 }
 
 
