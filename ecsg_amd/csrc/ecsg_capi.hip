@@ -120,7 +120,8 @@ const int SIM_POS[9][3] = {{0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 0, 0}, {1, 0, 0}
 const int ST_POS[9][3] = {{1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0},
                           {1, 0, 0}, {1, 1, 1}, {0, 0, 1}, {0, 1, 0}};
 
-constexpr int ZCHUNK = 8;
+constexpr int ZCHUNK = 8;        // planes per lane of the batched sweeps
+constexpr int ZCHUNK_MODEL = 1;  // planes per lane of a single model's sweeps (banded tile order)
 constexpr int JACOBI_CAP = 100000;
 constexpr size_t DIRECT_MAX_BYTES = (size_t)512 << 20;  // direct vx-sxz factors per interface
 
@@ -724,7 +725,7 @@ public:
     // workgroups the grid-barrier launch becomes resident only as the
     // velocity sweep ends (velocity sweep 1.39 -> 1.26 ms, stress sweep
     // 2.17 -> 2.27 ms).  Off by default.
-    int tile_by = 4, tile_zc = ZCHUNK, tile_byv = 0;
+    int tile_by = 4, tile_zc = ZCHUNK_MODEL, tile_byv = 0;
     static void set_tiling(DevGrid<T> &d, int by, int zc)
     {
         GridDev<T> &v = d.dev;
@@ -3249,7 +3250,7 @@ public:
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        const int bys[4] = {1, 2, 4, 8}, zcs[4] = {2, 4, 8, 16};
+        const int bys[4] = {1, 2, 4, 8}, zcs[5] = {1, 2, 4, 8, 16};
         for (int g = 0; g < (int)G.size(); g++) {
             DevGrid<T> &d = G[g];
             T *save = nullptr;

@@ -342,7 +342,7 @@ int ecsg_set_jacobi_tol(ecsg_ctx *ctx, double tol_cap, double tol_fac);
 int ecsg_set_fused(ecsg_ctx *ctx, int on);
 
 /* Stencil tiling of each grid (workgroup height by_out[g] in {1,2,4,8} rows
- * of 64, z chunk zc_out[g] in {2,4,8,16} planes per thread) chosen by timing
+ * of 64, z chunk zc_out[g] in {1,2,4,8,16} planes per thread) chosen by timing
  * the velocity and stress launches of every candidate reps times on a saved
  * copy of the state, which is restored.  New (the reference has no tiling). */
 int ecsg_autotune(ecsg_ctx *ctx, int reps, int *by_out, int *zc_out);

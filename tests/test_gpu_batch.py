@@ -167,7 +167,7 @@ def test_autotune_preserves_state_and_results():
     s.sync()
     before = [s.get(g, c).copy() for g in range(2) for c in FIELDS]
     choice = s.autotune(reps=2)
-    assert len(choice) == 2 and all(b in (1, 2, 4, 8) and z in (2, 4, 8, 16) for b, z in choice)
+    assert len(choice) == 2 and all(b in (1, 2, 4, 8) and z in (1, 2, 4, 8, 16) for b, z in choice)
     after = [s.get(g, c) for g in range(2) for c in FIELDS]
     for a, c in zip(after, before):
         assert np.array_equal(a, c)
