@@ -228,13 +228,13 @@ def batch_gradient_run(steps: int, every: int, tiling=None):
                     "device"}
 
 
-def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 8, reps: int = 10, tilings=(),
+def stream_bound(nx: int, ny: int, planes: int, by: int = 4, zc: int = 1, reps: int = 10, tilings=(),
                  vel_planes: int = 0):
     """The streaming bound of the stress sweep's traversal: ecsg_stream_probe
     (18 FP32 arrays read, 6 written back in place, no arithmetic beyond one
     multiply-add) over the same nodes -- nx+1 by ny+1 nodes on `planes`
     planes, the same row pitch and plane stride -- in the stencil tiling
-    (64 x by rows x zc planes, XCD order) and as one flat pass over the
+    (64 x by rows x zc planes, the sweeps' banded XCD order) and as one flat pass over the
     arrays' planes (halo rows and padding included); mean of `reps` launches
     each, HIP events on the launch stream.  `tilings`: further (by, zc)
     pairs, reported as "tiles_<by>x<zc>_ms".  vel_planes > 0: also the

@@ -4387,7 +4387,7 @@ __device__ __forceinline__ void stream_node(const StreamProbe &g, long p)
 template <int BYv, int NR, int NW>
 __global__ __launch_bounds__(64 * BYv) void k_stream_tiles(StreamProbe g)
 {
-    const int tile = xcd_tile(blockIdx.x, g.nblk);
+    const int tile = sweep_tile(blockIdx.x, g.nblk, g.gx);
     const int bx = tile % g.gx, rest = tile / g.gx, by = rest % g.gy, bz = rest / g.gy;
     const int ix = bx * 64 + threadIdx.x, iy = by * BYv + threadIdx.y;
     if (ix > g.nx || iy > g.ny) return;

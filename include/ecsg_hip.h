@@ -414,7 +414,7 @@ int ecsg_copy_probe(float *dst, const float *src, uint64_t n, int hints, void *s
  * over the arrays read (W = the number written back), summed left to right
  * in FP32, with the sweeps' non-temporal hints.  zc >= 1: over the (nx+1)(ny+1)(nz+1) nodes in the stencil tiling
  * (64 x by rows x zc planes per workgroup, by = 1, 2, 4 or 8, workgroups in
- * XCD order); zc = 0: one flat pass over every entry of planes 0..nz (halo
+ * the sweeps' banded XCD order); zc = 0: one flat pass over every entry of planes 0..nz (halo
  * rows and padding included).  Asynchronous on stream (NULL = the null
  * stream); ECSG_EINVAL for other by or pattern, negative zc, empty grids or
  * a null base.
